@@ -1,0 +1,394 @@
+// pdlp_api.cpp — the C ABI of libpdlp_hip.so (declared in include/pdlp_hip.h).
+#include <hip/hip_runtime_api.h>
+#include <math.h>
+#include <string.h>
+
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "pdlp_engine.h"
+#include "pdlp_hip.h"
+#include "pdlp_prep.h"
+
+using namespace pdlp;
+
+struct pdlp_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  DevicePool pool;
+  std::string err;
+};
+
+static std::mutex g_err_mu;
+static std::string g_err;
+
+static void set_global_error(const std::string &e) {
+  std::lock_guard<std::mutex> lk(g_err_mu);
+  g_err = e;
+}
+
+extern "C" {
+
+int pdlp_abi_version(void) { return PDLP_ABI_VERSION; }
+
+int32_t pdlp_abi_layout(int64_t *out, int32_t max) {
+  const int64_t sizes[] = {(int64_t)sizeof(pdlp_lp),
+                           (int64_t)sizeof(pdlp_termination_criteria),
+                           (int64_t)sizeof(pdlp_restart_params),
+                           (int64_t)sizeof(pdlp_params),
+                           (int64_t)sizeof(pdlp_convergence_information),
+                           (int64_t)sizeof(pdlp_infeasibility_information),
+                           (int64_t)sizeof(pdlp_iteration_stats),
+                           (int64_t)sizeof(pdlp_result)};
+  const int32_t cnt = (int32_t)(sizeof(sizes) / sizeof(sizes[0]));
+  for (int32_t i = 0; i < cnt && i < max; ++i) out[i] = sizes[i];
+  return cnt < max ? cnt : max;
+}
+
+void pdlp_default_params(pdlp_params *p) {
+  memset(p, 0, sizeof(*p));
+  // cuPDLP.Optimizer() — src/MOI_wrapper.jl:58-78
+  p->l_inf_ruiz_iterations = 10;
+  p->l2_norm_rescaling = 0;
+  p->has_pock_chambolle_alpha = 1;
+  p->pock_chambolle_alpha = 1.0;
+  p->primal_importance = 1.0;
+  p->scale_invariant_initial_primal_weight = 1;
+  p->verbosity = 2;
+  p->record_iteration_stats = 1;
+  p->termination_evaluation_frequency = 64;
+  p->step_size_policy = PDLP_STEP_ADAPTIVE;
+  p->reduction_exponent = 0.3;
+  p->growth_exponent = 0.6;
+  // construct_termination_criteria() — src/termination.jl:79-99
+  p->termination_criteria.optimality_norm = PDLP_L2;
+  p->termination_criteria.eps_optimal_absolute = 1.0e-6;
+  p->termination_criteria.eps_optimal_relative = 1.0e-6;
+  p->termination_criteria.eps_primal_infeasible = 1.0e-8;
+  p->termination_criteria.eps_dual_infeasible = 1.0e-8;
+  p->termination_criteria.time_sec_limit = INFINITY;
+  p->termination_criteria.iteration_limit = 2147483647;
+  p->termination_criteria.kkt_matrix_pass_limit = INFINITY;
+  p->restart_params.restart_scheme = PDLP_ADAPTIVE_KKT;
+  p->restart_params.restart_to_current_metric = PDLP_KKT_GREEDY;
+  p->restart_params.restart_frequency_if_fixed = 1000;
+  p->restart_params.artificial_restart_threshold = 0.36;
+  p->restart_params.sufficient_reduction_for_restart = 0.2;
+  p->restart_params.necessary_reduction_for_restart = 0.8;
+  p->restart_params.primal_weight_update_smoothing = 0.5;
+  p->precision = PDLP_PRECISION_FP64;
+  p->refine_max_outer = 12;
+  p->refine_inner_tolerance = 1.0e-4;
+  p->refine_max_scale = 1.0e6;
+  p->device_preconditioning = 1;
+}
+
+int pdlp_ctx_create(int device, pdlp_ctx **out) {
+  if (!out) return 1;
+  *out = nullptr;
+  int ndev = 0;
+  hipError_t e = hipGetDeviceCount(&ndev);
+  if (e != hipSuccess || ndev <= 0) {
+    set_global_error(std::string("no HIP device available: ") + hipGetErrorString(e));
+    return -1;
+  }
+  if (device < 0 || device >= ndev) {
+    set_global_error("device index out of range");
+    return 1;
+  }
+  e = hipSetDevice(device);
+  if (e != hipSuccess) {
+    set_global_error(std::string("hipSetDevice: ") + hipGetErrorString(e));
+    return -(int)e;
+  }
+  auto *c = new pdlp_ctx();
+  c->device = device;
+  e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    set_global_error(std::string("hipStreamCreate: ") + hipGetErrorString(e));
+    delete c;
+    return -(int)e;
+  }
+  *out = c;
+  return 0;
+}
+
+void pdlp_ctx_destroy(pdlp_ctx *ctx) {
+  if (!ctx) return;
+  (void)hipSetDevice(ctx->device);
+  (void)hipStreamSynchronize(ctx->stream);
+  ctx->pool.release_all();
+  (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+}
+
+const char *pdlp_last_error(const pdlp_ctx *ctx) {
+  if (ctx) return ctx->err.c_str();
+  std::lock_guard<std::mutex> lk(g_err_mu);
+  return g_err.c_str();
+}
+
+}  // extern "C"
+
+static int check_params(const pdlp_params *p, std::string *err) {
+  const pdlp_restart_params &rp = p->restart_params;
+  // construct_restart_parameters asserts — src/saddle_point_gpu.jl:397-403
+  if (!(rp.restart_frequency_if_fixed > 1) ||
+      !(0.0 < rp.artificial_restart_threshold && rp.artificial_restart_threshold <= 1.0) ||
+      !(0.0 < rp.sufficient_reduction_for_restart &&
+        rp.sufficient_reduction_for_restart <= rp.necessary_reduction_for_restart &&
+        rp.necessary_reduction_for_restart <= 1.0) ||
+      !(0.0 <= rp.primal_weight_update_smoothing && rp.primal_weight_update_smoothing <= 1.0)) {
+    *err = "invalid restart parameters";
+    return 2;
+  }
+  if (p->termination_evaluation_frequency <= 0) {
+    *err = "termination_evaluation_frequency must be positive";
+    return 2;
+  }
+  // src/primal_dual_hybrid_gradient_gpu.jl:483-485
+  if (!(p->primal_importance > 0) || !isfinite(p->primal_importance)) {
+    *err = "primal_importance must be positive and finite";
+    return 3;
+  }
+  if (p->has_pock_chambolle_alpha && !(p->pock_chambolle_alpha >= 0 && p->pock_chambolle_alpha <= 2)) {
+    *err = "pock_chambolle_alpha must be in [0, 2]";
+    return 2;
+  }
+  return 0;
+}
+
+namespace pdlp {
+int prepare_instance(const pdlp_lp *lp, const pdlp_params *prm, HostInst *h, std::string *err);
+}  // namespace pdlp
+
+int pdlp::prepare_instance(const pdlp_lp *lp, const pdlp_params *prm, HostInst *h,
+                           std::string *err) {
+  int rc = load_problem(lp, &h->P, err);
+  if (rc) return rc;
+  rescale_host(prm, &h->P);
+  return 0;
+}
+
+extern "C" int pdlp_solve(pdlp_ctx *ctx, const pdlp_lp *lp, const pdlp_params *params,
+                          pdlp_result *result, pdlp_stats_cb cb, void *user) {
+  if (!ctx) return 1;
+  ctx->err.clear();
+  if (!lp || !params || !result) {
+    ctx->err = "null argument";
+    return 1;
+  }
+  if ((lp->n > 0 && !result->x) || (lp->m > 0 && !result->y)) {
+    ctx->err = "result vectors must be allocated by the caller";
+    return 1;
+  }
+  const double t0 = now_sec();
+  int rc = check_params(params, &ctx->err);
+  if (rc) return rc;
+  (void)hipSetDevice(ctx->device);
+  auto h = std::make_unique<HostInst>();
+  rc = prepare_instance(lp, params, h.get(), &ctx->err);
+  if (rc) return rc;
+  h->res = result;
+  h->cb = cb;
+  h->user = user;
+  Engine eng(ctx->stream, &ctx->pool, *params);
+  rc = eng.add(std::move(h), &ctx->err);
+  if (rc) return rc;
+  result->preprocess_time_sec = now_sec() - t0;
+  rc = eng.run(&ctx->err);
+  if (rc) return rc;
+  result->solve_time_sec = now_sec() - t0;
+  result->refinement_rounds = 0;
+  return 0;
+}
+
+extern "C" int pdlp_batch_solve(pdlp_ctx *ctx, int32_t count, const pdlp_lp *lps,
+                                const pdlp_params *params, pdlp_result *results) {
+  if (!ctx) return 1;
+  ctx->err.clear();
+  if (count < 0 || (count > 0 && (!lps || !results)) || !params) {
+    ctx->err = "invalid batch arguments";
+    return 1;
+  }
+  const double t0 = now_sec();
+  int rc = check_params(params, &ctx->err);
+  if (rc) return rc;
+  (void)hipSetDevice(ctx->device);
+  Engine eng(ctx->stream, &ctx->pool, *params);
+  for (int32_t i = 0; i < count; ++i) {
+    if ((lps[i].n > 0 && !results[i].x) || (lps[i].m > 0 && !results[i].y)) {
+      ctx->err = "result vectors must be allocated by the caller (instance " + std::to_string(i) + ")";
+      return 1;
+    }
+    auto h = std::make_unique<HostInst>();
+    rc = prepare_instance(&lps[i], params, h.get(), &ctx->err);
+    if (rc) {
+      ctx->err = "instance " + std::to_string(i) + ": " + ctx->err;
+      return rc;
+    }
+    h->res = &results[i];
+    rc = eng.add(std::move(h), &ctx->err);
+    if (rc) return rc;
+  }
+  const double t1 = now_sec();
+  rc = eng.run(&ctx->err);
+  if (rc) return rc;
+  const double t2 = now_sec();
+  for (int32_t i = 0; i < count; ++i) {
+    results[i].preprocess_time_sec = t1 - t0;
+    results[i].solve_time_sec = t2 - t0;
+  }
+  return 0;
+}
+
+// ------------------------------------------------------------------ serving
+struct ServeReq {
+  int64_t id;
+  std::unique_ptr<HostInst> inst;
+  double tol;
+  size_t bytes;
+};
+
+struct ServeDone {
+  std::vector<double> x, y;
+  int64_t n = 0, m = 0;
+  pdlp_result res;
+};
+
+struct pdlp_server {
+  pdlp_ctx *ctx = nullptr;
+  pdlp_params prm{};
+  double budget = 0;
+  int32_t max_slots = 0;
+  std::unique_ptr<Engine> eng;
+  std::deque<ServeReq> queue;
+  std::map<int64_t, std::unique_ptr<ServeDone>> running;  // result storage of admitted ids
+  std::map<int64_t, std::unique_ptr<ServeDone>> done;
+  std::vector<int64_t> finished;
+  int64_t next_id = 0;
+  double in_use = 0;
+};
+
+static size_t footprint(const Prepared &P, int prec) {
+  const size_t ts = prec ? 4 : 8;
+  return (size_t)P.nnz * 2 * (4 + ts) + (size_t)(P.n + P.m + 2) * 4 + ts * (14 * P.n + 11 * P.m) +
+         8 * (4 * P.n + 2 * P.m) + 64 * 1024;
+}
+
+extern "C" int pdlp_serve_open(pdlp_ctx *ctx, const pdlp_params *params, double hbm_budget_bytes,
+                               int32_t max_slots, pdlp_server **out) {
+  if (!ctx || !params || !out) return 1;
+  ctx->err.clear();
+  int rc = check_params(params, &ctx->err);
+  if (rc) return rc;
+  auto *s = new pdlp_server();
+  s->ctx = ctx;
+  s->prm = *params;
+  s->budget = hbm_budget_bytes > 0 ? hbm_budget_bytes : 1e18;
+  s->max_slots = max_slots > 0 ? max_slots : 1 << 30;
+  s->eng = std::make_unique<Engine>(ctx->stream, &ctx->pool, *params);
+  *out = s;
+  return 0;
+}
+
+extern "C" int pdlp_serve_submit(pdlp_server *s, const pdlp_lp *lp, double tolerance, int64_t *id) {
+  if (!s || !lp || !id) return 1;
+  auto h = std::make_unique<HostInst>();
+  int rc = pdlp::prepare_instance(lp, &s->prm, h.get(), &s->ctx->err);
+  if (rc) return rc;
+  h->tolerance_override = tolerance;
+  ServeReq r;
+  r.id = s->next_id++;
+  r.tol = tolerance;
+  r.bytes = footprint(h->P, s->prm.precision == PDLP_PRECISION_FP32_REFINE);
+  if ((double)r.bytes > s->budget) {
+    s->ctx->err = "request exceeds the HBM budget";
+    return 5;
+  }
+  h->tag = r.id;
+  r.inst = std::move(h);
+  *id = r.id;
+  s->queue.push_back(std::move(r));
+  return 0;
+}
+
+extern "C" int pdlp_serve_poll(pdlp_server *s, int64_t *ids, int32_t max_ids, int32_t *finished) {
+  if (!s || !finished) return 1;
+  std::string *err = &s->ctx->err;
+  (void)hipSetDevice(s->ctx->device);
+  // admit by footprint and slot count
+  while (!s->queue.empty() && (int32_t)s->eng->count() < s->max_slots &&
+         s->in_use + (double)s->queue.front().bytes <= s->budget) {
+    ServeReq r = std::move(s->queue.front());
+    s->queue.pop_front();
+    auto d = std::make_unique<ServeDone>();
+    d->x.assign(std::max<int64_t>(r.inst->P.n, 1), 0.0);
+    d->y.assign(std::max<int64_t>(r.inst->P.m, 1), 0.0);
+    d->n = r.inst->P.n;
+    d->m = r.inst->P.m;
+    memset(&d->res, 0, sizeof d->res);
+    d->res.x = d->x.data();
+    d->res.y = d->y.data();
+    r.inst->res = &d->res;
+    s->in_use += (double)r.bytes;
+    s->running[r.id] = std::move(d);
+    int rc = s->eng->add(std::move(r.inst), err);
+    if (rc) return rc;
+  }
+  if (s->eng->count() > 0) {
+    int rc = s->eng->step_round(err);
+    if (rc < 0) return rc;
+    for (size_t q = 0; q < s->eng->count(); ++q) {
+      HostInst &h = s->eng->inst(q);
+      if (!h.done) continue;
+      auto it = s->running.find(h.tag);
+      if (it != s->running.end()) {
+        s->in_use -= (double)footprint(h.P, s->prm.precision == PDLP_PRECISION_FP32_REFINE);
+        s->done[h.tag] = std::move(it->second);
+        s->running.erase(it);
+        s->finished.push_back(h.tag);
+      }
+    }
+    s->eng->remove_done();
+  }
+  int32_t k = 0;
+  while (k < max_ids && !s->finished.empty()) {
+    ids[k++] = s->finished.front();
+    s->finished.erase(s->finished.begin());
+  }
+  *finished = k;
+  return 0;
+}
+
+extern "C" int pdlp_serve_result(pdlp_server *s, int64_t id, pdlp_result *result) {
+  if (!s || !result) return 1;
+  auto it = s->done.find(id);
+  if (it == s->done.end()) {
+    s->ctx->err = "request not finished or unknown";
+    return 6;
+  }
+  ServeDone &d = *it->second;
+  double *x = result->x, *y = result->y;
+  double *xc = result->x_current, *yc = result->y_current;
+  *result = d.res;
+  result->x = x;
+  result->y = y;
+  result->x_current = xc;
+  result->y_current = yc;
+  if (x && d.n > 0) memcpy(x, d.x.data(), sizeof(double) * (size_t)d.n);
+  if (y && d.m > 0) memcpy(y, d.y.data(), sizeof(double) * (size_t)d.m);
+  s->done.erase(it);
+  return 0;
+}
+
+extern "C" int pdlp_serve_close(pdlp_server *s) {
+  if (!s) return 1;
+  s->eng.reset();
+  delete s;
+  return 0;
+}

@@ -1,0 +1,83 @@
+// pdlp_device.h — device-side data layout shared by the HIP kernels and the
+// host engine.  No HIP types appear here, so the host C++ can include it.
+//
+// Layout in HBM (per LP "instance"; a single solve is a batch of one):
+//   A   (m x n) CSR : a_ptr[m+1] int32, a_col[nnz] int32, a_val[nnz] T
+//   A^T (n x m) CSR : t_ptr[n+1] int32, t_col[nnz] int32, t_val[nnz] T
+//   scaled data     : c, l, u (n) and b (m), type T
+//   original data   : oc, ol, ou (n), ob (m), D (n), E (m), double
+//   iterates        : x[2], aty, dx (n); y[2], ax[2] (m)          (T)
+//   weighted sums   : sx, saty (n); sy, sax (m)                      (T)
+//   evaluation      : avg_x, avg_g, gbuf, lx, lg (n); avg_y, avg_ax, ly, lax (m)
+// Rows of A and of A^T are cut into "row blocks" of at most kBlockNnz
+// nonzeros (and kBlockRows rows); a row longer than kBlockNnz is split into
+// chunks whose partial sums are combined in chunk order by the last chunk.
+// Blocks never straddle instances, so an instance's arithmetic does not
+// depend on what else is packed beside it.
+#pragma once
+#include <stdint.h>
+
+namespace pdlp {
+
+constexpr int kThreads = 256;    // work-group size of every kernel (4 waves)
+constexpr int kBlockNnz = 2048;  // nonzeros staged in LDS per row block
+constexpr int kBlockRows = 1024; // rows per row block
+
+// Evaluation accumulators (see eval kernel): n-side and m-side.
+constexpr int kEvalN = 25;
+constexpr int kEvalM = 18;
+constexpr int kEvalOut = kEvalN + kEvalM;
+
+struct BlockDesc {
+  int32_t inst;   // instance index
+  int32_t row0;   // first row (or the long row)
+  int32_t nrows;  // > 0: rows [row0, row0+nrows); < 0: chunk (-nrows-1) of long row row0
+  int32_t aux;    // long-row id for chunks
+};
+
+struct LongRow {
+  int32_t slot;     // first partial-sum slot
+  int32_t nchunks;  // number of chunks
+};
+
+struct InstDev {
+  int32_t n, m, neq, pad0;
+  int32_t pb0, pb1;  // block range in the A^T (primal) table
+  int32_t db0, db1;  // block range in the A (dual) table
+  const int32_t *a_ptr, *a_col;
+  const void *a_val;
+  const int32_t *t_ptr, *t_col;
+  const void *t_val;
+  const void *c, *l, *u, *b;  // scaled (T)
+  const double *oc, *ol, *ou, *ob, *D, *E;
+  void *x0, *x1, *y0, *y1, *ax0, *ax1, *aty, *dx;
+  void *sx, *saty, *sy, *sax;
+  void *avg_x, *avg_g, *gbuf, *lx, *lg;
+  void *avg_y, *avg_ax, *ly, *lax;
+};
+
+// Per-instance control block.  Written by the step kernel and by the host
+// between chunks (the stream is idle then).
+struct Ctl {
+  double eta;        // step size of the next attempt (take_step!'s local step_size)
+  double eta_entry;  // solver_state.step_size (weight of the accepted point)
+  double omega;      // primal weight
+  double w_pending;  // weight of an accepted point whose sums are not yet added
+  double wsum;       // sum of weights
+  double kkt_passes; // cumulative_kkt_passes
+  double last_interaction, last_movement;
+  int64_t count;     // sum_primal_solutions_count
+  int64_t attempts;  // total_number_iterations (attempts)
+  int64_t accepted;  // completed take_step! calls
+  int64_t target;    // stop when accepted reaches this
+  int32_t cur;       // which of x0/x1 (y0/y1, ax0/ax1) is current
+  int32_t pending;   // sums of the last accepted point still to add
+  int32_t aty_valid; // aty holds A^T y for the current y
+  int32_t active;    // attempts run for this instance
+  int32_t numerical_error;
+  int32_t eval_flag;      // instance is being evaluated
+  int32_t use_current;    // evaluation averages := current point
+  int32_t restart_action; // 0 none, 1 reset sums to current, 2 restart to average
+};
+
+}  // namespace pdlp

@@ -1,0 +1,733 @@
+// pdlp_engine.cpp — host driver of the packed restarted-PDHG engine.
+#include "pdlp_engine.h"
+
+#include <hip/hip_runtime_api.h>
+#include <math.h>
+#include <string.h>
+#include <time.h>
+
+#include <algorithm>
+#include <cstdio>
+
+#include "pdlp_kernels.h"
+
+namespace pdlp {
+
+double now_sec() {
+  struct timespec ts;
+  clock_gettime(CLOCK_MONOTONIC, &ts);
+  return (double)ts.tv_sec + 1e-9 * (double)ts.tv_nsec;
+}
+
+static inline double jl_max(double a, double b) {
+  if (a != a || b != b) return a + b;
+  if (a > b) return a;
+  if (b > a) return b;
+  return signbit(a) ? b : a;
+}
+
+#define HIPCHK(expr)                                                              \
+  do {                                                                            \
+    hipError_t e_ = (expr);                                                       \
+    if (e_ != hipSuccess) {                                                       \
+      if (err) *err = std::string(#expr) + ": " + hipGetErrorString(e_);          \
+      return -(int)(e_ == 0 ? 1 : e_);                                            \
+    }                                                                             \
+  } while (0)
+
+// ---------------------------------------------------------------- pool
+DevicePool::~DevicePool() { release_all(); }
+
+void *DevicePool::get(size_t bytes) {
+  bytes = (bytes + 255) & ~(size_t)255;
+  if (bytes == 0) bytes = 256;
+  size_t best = (size_t)-1;
+  for (size_t i = 0; i < free_.size(); ++i)
+    if (free_[i].bytes >= bytes && free_[i].bytes <= 2 * bytes + (1 << 20) &&
+        (best == (size_t)-1 || free_[i].bytes < free_[best].bytes))
+      best = i;
+  if (best != (size_t)-1) {
+    Blk b = free_[best];
+    free_.erase(free_.begin() + best);
+    used_.push_back(b);
+    in_use_ += b.bytes;
+    return b.p;
+  }
+  void *p = nullptr;
+  if (hipMalloc(&p, bytes) != hipSuccess) {
+    // release cached blocks and retry once
+    for (auto &b : free_) (void)hipFree(b.p);
+    free_.clear();
+    (void)hipGetLastError();
+    if (hipMalloc(&p, bytes) != hipSuccess) {
+      (void)hipGetLastError();
+      return nullptr;
+    }
+  }
+  used_.push_back({p, bytes});
+  in_use_ += bytes;
+  return p;
+}
+
+void DevicePool::put(void *p) {
+  if (!p) return;
+  for (size_t i = 0; i < used_.size(); ++i)
+    if (used_[i].p == p) {
+      in_use_ -= used_[i].bytes;
+      free_.push_back(used_[i]);
+      used_.erase(used_.begin() + i);
+      return;
+    }
+}
+
+void DevicePool::release_all() {
+  for (auto &b : free_) (void)hipFree(b.p);
+  for (auto &b : used_) (void)hipFree(b.p);
+  free_.clear();
+  used_.clear();
+  in_use_ = 0;
+}
+
+// ---------------------------------------------------------------- engine
+Engine::Engine(hipStream_t s, DevicePool *pool, const pdlp_params &prm)
+    : s_(s), pool_(pool), prm_(prm) {
+  prec_ = prm.precision == PDLP_PRECISION_FP32_REFINE ? 1 : 0;
+}
+
+Engine::~Engine() {
+  (void)hipStreamSynchronize(s_);
+  for (auto &h : inst_)
+    if (h->arena) pool_->put(h->arena);
+  for (void *p : table_allocs_) pool_->put(p);
+  pool_->put(d_red_);
+  pool_->put(d_grow_);
+  if (ctl_pinned_) (void)hipHostFree(ctl_pinned_);
+  if (out_pinned_) (void)hipHostFree(out_pinned_);
+}
+
+namespace {
+struct Carver {
+  size_t off = 0;
+  size_t take(size_t bytes) {
+    const size_t o = off;
+    off += (bytes + 255) & ~(size_t)255;
+    return o;
+  }
+};
+}  // namespace
+
+int Engine::upload(HostInst &h, std::string *err) {
+  const Prepared &P = h.P;
+  const size_t n = (size_t)P.n, m = (size_t)P.m, nnz = (size_t)P.nnz;
+  const size_t ts = prec_ ? sizeof(float) : sizeof(double);
+  Carver cv;
+  const size_t o_aptr = cv.take(4 * (m + 1)), o_acol = cv.take(4 * nnz), o_aval = cv.take(ts * nnz);
+  const size_t o_tptr = cv.take(4 * (n + 1)), o_tcol = cv.take(4 * nnz), o_tval = cv.take(ts * nnz);
+  // T-typed n-vectors: c l u x0 x1 aty dx sx saty avg_x avg_g gbuf lx lg
+  size_t o_n[14];
+  for (int k = 0; k < 14; ++k) o_n[k] = cv.take(ts * n);
+  // T-typed m-vectors: b y0 y1 ax0 ax1 sy sax avg_y avg_ax ly lax
+  size_t o_m[11];
+  for (int k = 0; k < 11; ++k) o_m[k] = cv.take(ts * m);
+  // double: oc ol ou D (n); ob E (m)
+  size_t o_dn[4], o_dm[2];
+  for (int k = 0; k < 4; ++k) o_dn[k] = cv.take(8 * n);
+  for (int k = 0; k < 2; ++k) o_dm[k] = cv.take(8 * m);
+  h.arena_bytes = cv.off;
+  h.arena = pool_->get(cv.off);
+  if (!h.arena) {
+    if (err) *err = "device allocation of " + std::to_string(cv.off) + " bytes failed";
+    return -2;
+  }
+  char *base = (char *)h.arena;
+  HIPCHK(hipMemsetAsync(base, 0, cv.off, s_));
+  auto put = [&](size_t off, const void *src, size_t bytes) -> hipError_t {
+    if (bytes == 0) return hipSuccess;
+    return hipMemcpyAsync(base + off, src, bytes, hipMemcpyHostToDevice, s_);
+  };
+  // values and vectors in the iteration precision
+  std::vector<float> tmp;
+  auto putT = [&](size_t off, const std::vector<double> &v) -> hipError_t {
+    if (v.empty()) return hipSuccess;
+    if (!prec_) return put(off, v.data(), 8 * v.size());
+    tmp.assign(v.begin(), v.end());
+    hipError_t e = put(off, tmp.data(), 4 * v.size());
+    if (e == hipSuccess) e = hipStreamSynchronize(s_);
+    return e;
+  };
+  HIPCHK(put(o_aptr, P.a_ptr.data(), 4 * (m + 1)));
+  HIPCHK(put(o_acol, P.a_col.data(), 4 * nnz));
+  HIPCHK(putT(o_aval, P.a_val));
+  HIPCHK(put(o_tptr, P.t_ptr.data(), 4 * (n + 1)));
+  HIPCHK(put(o_tcol, P.t_col.data(), 4 * nnz));
+  HIPCHK(putT(o_tval, P.t_val));
+  HIPCHK(putT(o_n[0], P.c));
+  HIPCHK(putT(o_n[1], P.l));
+  HIPCHK(putT(o_n[2], P.u));
+  HIPCHK(putT(o_n[11], P.c));  // gbuf = c - A^T 0
+  HIPCHK(putT(o_n[13], P.c));  // last restart gradient
+  HIPCHK(putT(o_m[0], P.b));
+  HIPCHK(put(o_dn[0], P.oc.data(), 8 * n));
+  HIPCHK(put(o_dn[1], P.ol.data(), 8 * n));
+  HIPCHK(put(o_dn[2], P.ou.data(), 8 * n));
+  HIPCHK(put(o_dn[3], P.D.data(), 8 * n));
+  HIPCHK(put(o_dm[0], P.ob.data(), 8 * m));
+  HIPCHK(put(o_dm[1], P.E.data(), 8 * m));
+  InstDev &d = h.dev;
+  d.n = (int32_t)n; d.m = (int32_t)m; d.neq = (int32_t)P.neq;
+  d.a_ptr = (const int32_t *)(base + o_aptr); d.a_col = (const int32_t *)(base + o_acol);
+  d.a_val = base + o_aval;
+  d.t_ptr = (const int32_t *)(base + o_tptr); d.t_col = (const int32_t *)(base + o_tcol);
+  d.t_val = base + o_tval;
+  d.c = base + o_n[0]; d.l = base + o_n[1]; d.u = base + o_n[2];
+  d.x0 = base + o_n[3]; d.x1 = base + o_n[4]; d.aty = base + o_n[5]; d.dx = base + o_n[6];
+  d.sx = base + o_n[7]; d.saty = base + o_n[8]; d.avg_x = base + o_n[9]; d.avg_g = base + o_n[10];
+  d.gbuf = base + o_n[11]; d.lx = base + o_n[12]; d.lg = base + o_n[13];
+  d.b = base + o_m[0]; d.y0 = base + o_m[1]; d.y1 = base + o_m[2]; d.ax0 = base + o_m[3];
+  d.ax1 = base + o_m[4]; d.sy = base + o_m[5]; d.sax = base + o_m[6]; d.avg_y = base + o_m[7];
+  d.avg_ax = base + o_m[8]; d.ly = base + o_m[9]; d.lax = base + o_m[10];
+  d.oc = (const double *)(base + o_dn[0]); d.ol = (const double *)(base + o_dn[1]);
+  d.ou = (const double *)(base + o_dn[2]); d.D = (const double *)(base + o_dn[3]);
+  d.ob = (const double *)(base + o_dm[0]); d.E = (const double *)(base + o_dm[1]);
+  HIPCHK(hipStreamSynchronize(s_));
+  return 0;
+}
+
+int Engine::add(std::unique_ptr<HostInst> h, std::string *err) {
+  if (prm_.step_size_policy != PDLP_STEP_ADAPTIVE) {
+    if (err) *err = "ConstantStepsizeParams is not supported by the device engine";
+    return 4;
+  }
+  int rc = upload(*h, err);
+  if (rc) return rc;
+  Ctl c;
+  memset(&c, 0, sizeof c);
+  const Prepared &P = h->P;
+  // 1 / norm(A, Inf) with the elementwise norm (src/primal_dual_hybrid_gradient_gpu.jl:569-571);
+  // a matrix with no entries (m = 0) uses 1.0, where the reference fails.
+  const double eta0 = P.max_abs_scaled > 0.0 ? 1.0 / P.max_abs_scaled : 1.0;
+  c.eta = eta0;
+  c.eta_entry = eta0;
+  double omega = prm_.primal_importance;
+  if (prm_.scale_invariant_initial_primal_weight) {
+    // select_initial_primal_weight — src/saddle_point_gpu.jl:767-785
+    auto nrm = [](const std::vector<double> &v) {
+      double s = 0.0;
+      for (size_t k = 0; k < v.size(); k += 4096) {
+        double t = 0.0;
+        const size_t e = std::min(v.size(), k + 4096);
+        for (size_t i = k; i < e; ++i) t += v[i] * v[i];
+        s += t;
+      }
+      return sqrt(s);
+    };
+    const double rhs = sqrt(1.0) * nrm(P.b), obj = sqrt(1.0) * nrm(P.c);
+    if (obj > 0.0 && rhs > 0.0) omega = prm_.primal_importance * (obj / rhs);
+  }
+  c.omega = omega;
+  c.kkt_passes = 0.5;
+  c.aty_valid = 1;
+  h->needs_eval = true;
+  h->eval_iter = 1;
+  h->start_time = now_sec();
+  inst_.push_back(std::move(h));
+  ctl_.push_back(c);
+  tables_dirty_ = true;
+  return 0;
+}
+
+static void partition_rows(const std::vector<int32_t> &ptr, int64_t R, int inst,
+                           std::vector<BlockDesc> &blocks, std::vector<LongRow> &longs,
+                           int *nslots) {
+  int64_t r = 0;
+  while (r < R) {
+    const int64_t len = ptr[r + 1] - ptr[r];
+    if (len > kBlockNnz) {
+      const int nch = (int)((len + kBlockNnz - 1) / kBlockNnz);
+      const int lid = (int)longs.size();
+      longs.push_back({*nslots, nch});
+      *nslots += nch;
+      for (int c = 0; c < nch; ++c) blocks.push_back({inst, (int32_t)r, -(c + 1), lid});
+      ++r;
+      continue;
+    }
+    const int64_t start = r;
+    int64_t nz = 0;
+    while (r < R && r - start < kBlockRows) {
+      const int64_t l = ptr[r + 1] - ptr[r];
+      if (l > kBlockNnz || nz + l > kBlockNnz) break;
+      nz += l;
+      ++r;
+    }
+    blocks.push_back({inst, (int32_t)start, (int32_t)(r - start), 0});
+  }
+}
+
+int Engine::rebuild_tables(std::string *err) {
+  HIPCHK(hipStreamSynchronize(s_));
+  for (void *p : table_allocs_) pool_->put(p);
+  table_allocs_.clear();
+  std::vector<BlockDesc> pb, db;
+  std::vector<LongRow> longs;
+  int nslots = 0;
+  for (size_t q = 0; q < inst_.size(); ++q) {
+    HostInst &h = *inst_[q];
+    h.dev.pb0 = (int32_t)pb.size();
+    partition_rows(h.P.t_ptr, h.P.n, (int)q, pb, longs, &nslots);
+    h.dev.pb1 = (int32_t)pb.size();
+  }
+  const int nlong_p = (int)longs.size();
+  for (size_t q = 0; q < inst_.size(); ++q) {
+    HostInst &h = *inst_[q];
+    h.dev.db0 = (int32_t)db.size();
+    partition_rows(h.P.a_ptr, h.P.m, (int)q, db, longs, &nslots);
+    h.dev.db1 = (int32_t)db.size();
+  }
+  (void)nlong_p;
+  np_ = (int)pb.size();
+  nd_ = (int)db.size();
+  nlong_ = (int)longs.size();
+  nslots_ = nslots;
+  const size_t ninst = inst_.size();
+  auto alloc = [&](size_t bytes) -> void * {
+    void *p = pool_->get(std::max<size_t>(bytes, 256));
+    if (p) table_allocs_.push_back(p);
+    return p;
+  };
+  d_pblocks_ = (BlockDesc *)alloc(sizeof(BlockDesc) * pb.size());
+  d_dblocks_ = (BlockDesc *)alloc(sizeof(BlockDesc) * db.size());
+  d_longs_ = (LongRow *)alloc(sizeof(LongRow) * longs.size());
+  d_long_cnt_ = (int32_t *)alloc(sizeof(int32_t) * longs.size());
+  d_long_part_ = (double *)alloc(sizeof(double) * nslots);
+  d_insts_ = (InstDev *)alloc(sizeof(InstDev) * ninst);
+  d_ctl_ = (Ctl *)alloc(sizeof(Ctl) * ninst);
+  d_ppart_ = (double *)alloc(sizeof(double) * pb.size());
+  d_dpart_ = (double *)alloc(sizeof(double) * 2 * db.size());
+  d_epn_ = (double *)alloc(sizeof(double) * kEvalN * pb.size());
+  d_epm_ = (double *)alloc(sizeof(double) * kEvalM * db.size());
+  d_out_ = (double *)alloc(sizeof(double) * kEvalOut * ninst);
+  if (!d_pblocks_ || !d_dblocks_ || !d_longs_ || !d_long_cnt_ || !d_long_part_ || !d_insts_ ||
+      !d_ctl_ || !d_ppart_ || !d_dpart_ || !d_epn_ || !d_epm_ || !d_out_) {
+    if (err) *err = "device allocation of block tables failed";
+    return -2;
+  }
+  if (!pb.empty())
+    HIPCHK(hipMemcpyAsync(d_pblocks_, pb.data(), sizeof(BlockDesc) * pb.size(), hipMemcpyHostToDevice, s_));
+  if (!db.empty())
+    HIPCHK(hipMemcpyAsync(d_dblocks_, db.data(), sizeof(BlockDesc) * db.size(), hipMemcpyHostToDevice, s_));
+  if (!longs.empty()) {
+    HIPCHK(hipMemcpyAsync(d_longs_, longs.data(), sizeof(LongRow) * longs.size(), hipMemcpyHostToDevice, s_));
+    HIPCHK(hipMemsetAsync(d_long_cnt_, 0, sizeof(int32_t) * longs.size(), s_));
+  }
+  std::vector<InstDev> devs(ninst);
+  for (size_t q = 0; q < ninst; ++q) devs[q] = inst_[q]->dev;
+  HIPCHK(hipMemcpyAsync(d_insts_, devs.data(), sizeof(InstDev) * ninst, hipMemcpyHostToDevice, s_));
+  if (ctl_pinned_) (void)hipHostFree(ctl_pinned_);
+  ctl_pinned_ = nullptr;
+  HIPCHK(hipHostMalloc((void **)&ctl_pinned_, sizeof(Ctl) * std::max<size_t>(ninst, 1), 0));
+  if (out_cap_ < ninst) {
+    if (out_pinned_) (void)hipHostFree(out_pinned_);
+    out_pinned_ = nullptr;
+    HIPCHK(hipHostMalloc((void **)&out_pinned_, sizeof(double) * kEvalOut * std::max<size_t>(ninst, 1), 0));
+    out_cap_ = ninst;
+  }
+  memcpy(ctl_pinned_, ctl_.data(), sizeof(Ctl) * ninst);
+  HIPCHK(hipMemcpyAsync(d_ctl_, ctl_pinned_, sizeof(Ctl) * ninst, hipMemcpyHostToDevice, s_));
+  HIPCHK(hipStreamSynchronize(s_));
+  tables_dirty_ = false;
+  return 0;
+}
+
+int Engine::ensure_step_table(int64_t need, std::string *err) {
+  if (need < tab_len_) return 0;
+  int64_t len = std::max<int64_t>(65536, tab_len_);
+  while (len <= need) len *= 2;
+  std::vector<double> red(len), grow(len);
+  // (1 - (k+1)^-r) and (1 + (k+1)^-g) for attempt number k —
+  // src/primal_dual_hybrid_gradient_gpu.jl:405-409
+  for (int64_t k = 0; k < len; ++k) {
+    const double k1 = (double)(k + 1);
+    red[k] = 1.0 - 1.0 / pow(k1, prm_.reduction_exponent);
+    grow[k] = 1.0 + 1.0 / pow(k1, prm_.growth_exponent);
+  }
+  HIPCHK(hipStreamSynchronize(s_));
+  pool_->put(d_red_);
+  pool_->put(d_grow_);
+  d_red_ = (double *)pool_->get(sizeof(double) * len);
+  d_grow_ = (double *)pool_->get(sizeof(double) * len);
+  if (!d_red_ || !d_grow_) {
+    if (err) *err = "device allocation of the step table failed";
+    return -2;
+  }
+  HIPCHK(hipMemcpy(d_red_, red.data(), sizeof(double) * len, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(d_grow_, grow.data(), sizeof(double) * len, hipMemcpyHostToDevice));
+  tab_len_ = len;
+  return 0;
+}
+
+int Engine::run_attempts(int64_t count, std::string *err) {
+  int64_t max_att = 0;
+  for (size_t q = 0; q < inst_.size(); ++q) max_att = std::max(max_att, ctl_[q].attempts);
+  int rc = ensure_step_table(max_att + count + 2, err);
+  if (rc) return rc;
+  const double t0 = now_sec();
+  for (int64_t a = 0; a < count; ++a) {
+    int e = launch_primal(s_, prec_, false, np_, d_pblocks_, d_longs_, d_long_cnt_, d_long_part_,
+                          d_insts_, d_ctl_, d_ppart_);
+    if (!e)
+      e = launch_dual(s_, prec_, false, nd_, d_dblocks_, d_longs_, d_long_cnt_, d_long_part_,
+                      d_insts_, d_ctl_, d_dpart_);
+    if (!e)
+      e = launch_step(s_, (int)inst_.size(), d_insts_, d_ctl_, d_ppart_, d_dpart_, d_red_, d_grow_,
+                      tab_len_, prm_.reduction_exponent, prm_.growth_exponent);
+    if (e) {
+      if (err) *err = std::string("kernel launch failed: ") + hipGetErrorString((hipError_t)e);
+      return -e;
+    }
+  }
+  const size_t ninst = inst_.size();
+  HIPCHK(hipMemcpyAsync(ctl_pinned_, d_ctl_, sizeof(Ctl) * ninst, hipMemcpyDeviceToHost, s_));
+  HIPCHK(hipStreamSynchronize(s_));
+  memcpy(ctl_.data(), ctl_pinned_, sizeof(Ctl) * ninst);
+  const double dt = now_sec() - t0;
+  for (size_t q = 0; q < ninst; ++q) {
+    HostInst &h = *inst_[q];
+    if (h.done) continue;
+    h.basic_time += dt;
+    Ctl &C = ctl_[q];
+    if (C.numerical_error) {
+      h.eval_iter = h.iter_start + (C.accepted - h.acc_at_eval) + 1;
+      h.needs_eval = true;
+    } else if (C.accepted >= C.target) {
+      h.needs_eval = true;
+    }
+  }
+  return 0;
+}
+
+static int check_termination(const pdlp_termination_criteria &tc, const QpCache &qc,
+                             const pdlp_iteration_stats &st) {
+  const pdlp_convergence_information &ci = st.convergence_information;
+  const double abs_obj = fabs(ci.primal_objective) + fabs(ci.dual_objective);
+  const double gap = fabs(ci.primal_objective - ci.dual_objective);
+  double pe, pb, de, db;
+  if (tc.optimality_norm == PDLP_L_INF) {
+    pe = ci.l_inf_primal_residual; pb = qc.l_inf_b; de = ci.l_inf_dual_residual; db = qc.l_inf_c;
+  } else {
+    pe = ci.l2_primal_residual; pb = qc.l2_b; de = ci.l2_dual_residual; db = qc.l2_c;
+  }
+  const double ea = tc.eps_optimal_absolute, er = tc.eps_optimal_relative;
+  if (de < ea + er * db && pe < ea + er * pb && gap < ea + er * abs_obj)
+    return PDLP_TERMINATION_REASON_OPTIMAL;
+  const pdlp_infeasibility_information &ii = st.infeasibility_information;
+  if (!(ii.dual_ray_objective <= 0.0) &&
+      ii.max_dual_ray_infeasibility / ii.dual_ray_objective <= tc.eps_primal_infeasible)
+    return PDLP_TERMINATION_REASON_PRIMAL_INFEASIBLE;
+  if (!(ii.primal_ray_linear_objective >= 0.0) &&
+      ii.max_primal_ray_infeasibility / (-ii.primal_ray_linear_objective) <= tc.eps_dual_infeasible)
+    return PDLP_TERMINATION_REASON_DUAL_INFEASIBLE;
+  if ((int64_t)st.iteration_number >= (int64_t)tc.iteration_limit)
+    return PDLP_TERMINATION_REASON_ITERATION_LIMIT;
+  if (st.cumulative_kkt_matrix_passes >= tc.kkt_matrix_pass_limit)
+    return PDLP_TERMINATION_REASON_KKT_MATRIX_PASS_LIMIT;
+  if (st.cumulative_time_sec >= tc.time_sec_limit) return PDLP_TERMINATION_REASON_TIME_LIMIT;
+  return 0;
+}
+
+static int64_t next_eval_iteration(int64_t it, int64_t freq, int64_t limit) {
+  int64_t j = it + 1;
+  while (!((j - 1) % freq == 0 || j == limit + 1 || j <= 10)) ++j;
+  return j;
+}
+
+void Engine::host_eval_logic(size_t q, const double *o) {
+  HostInst &h = *inst_[q];
+  Ctl &C = ctl_[q];
+  const Prepared &P = h.P;
+  const double *N = o, *M = o + kEvalN;
+  pdlp_termination_criteria tc = prm_.termination_criteria;
+  if (h.tolerance_override > 0) {
+    tc.eps_optimal_absolute = h.tolerance_override;
+    tc.eps_optimal_relative = h.tolerance_override;
+  }
+  const double c0 = P.c0;
+  C.kkt_passes += 2.0;
+  const double eps_ratio = tc.eps_optimal_absolute / tc.eps_optimal_relative;
+  pdlp_iteration_stats st;
+  memset(&st, 0, sizeof st);
+  st.iteration_number = (int32_t)(h.eval_iter - 1);
+  st.cumulative_kkt_matrix_passes = C.kkt_passes;
+  st.cumulative_time_sec = now_sec() - h.start_time;
+  // compute_convergence_information — src/iteration_stats_utils_gpu.jl:251-309
+  pdlp_convergence_information &ci = st.convergence_information;
+  ci.candidate_type = PDLP_POINT_TYPE_AVERAGE_ITERATE;
+  ci.primal_objective = c0 + N[2];
+  ci.l_inf_primal_residual = jl_max(M[1], N[1]);
+  ci.l2_primal_residual = sqrt(M[0] + N[0]);
+  ci.relative_l_inf_primal_residual = ci.l_inf_primal_residual / (eps_ratio + P.qc.l_inf_b);
+  ci.relative_l2_primal_residual = ci.l2_primal_residual / (eps_ratio + P.qc.l2_b);
+  ci.l_inf_primal_variable = N[3];
+  ci.l2_primal_variable = sqrt(N[4]);
+  const double dri = jl_max(M[3], N[6]);
+  ci.dual_objective = (M[4] + c0) + N[7];
+  ci.l_inf_dual_residual = dri;
+  ci.l2_dual_residual = sqrt(M[2] + N[5]);
+  ci.relative_l_inf_dual_residual = ci.l_inf_dual_residual / (eps_ratio + P.qc.l_inf_c);
+  ci.relative_l2_dual_residual = ci.l2_dual_residual / (eps_ratio + P.qc.l2_c);
+  ci.l_inf_dual_variable = M[5];
+  ci.l2_dual_variable = sqrt(M[6]);
+  ci.corrected_dual_objective = dri == 0.0 ? ci.dual_objective : -INFINITY;
+  {
+    const double gap = fabs(ci.primal_objective - ci.dual_objective);
+    const double abs_obj = fabs(ci.primal_objective) + fabs(ci.dual_objective);
+    ci.relative_optimality_gap = gap / (eps_ratio + abs_obj);
+  }
+  // compute_infeasibility_information — src/iteration_stats_utils_gpu.jl:314-372
+  pdlp_infeasibility_information &ii = st.infeasibility_information;
+  ii.candidate_type = PDLP_POINT_TYPE_AVERAGE_ITERATE;
+  {
+    const double s = N[3];
+    const double bmax = jl_max(M[7], N[11]);
+    ii.max_primal_ray_infeasibility = s != 0.0 ? bmax / s : bmax;
+    ii.primal_ray_linear_objective = s != 0.0 ? N[2] / s : N[2];
+    const double dri2 = jl_max(M[3], N[8]);
+    const double dobj2 = (M[4] + c0) + N[9];
+    const double scaling = jl_max(M[5], N[10]);
+    if (scaling != 0.0) {
+      ii.max_dual_ray_infeasibility = dri2 / scaling;
+      ii.dual_ray_objective = dobj2 / scaling;
+    }
+  }
+  st.step_size = C.eta_entry;
+  st.primal_weight = C.omega;
+  st.time_spent_doing_basic_algorithm = h.basic_time;
+  st.total_attempts = C.attempts;
+
+  int reason = check_termination(tc, P.qc, st);
+  if (C.numerical_error && reason == 0) reason = PDLP_TERMINATION_REASON_NUMERICAL_ERROR;
+  if (h.eval_iter < 10 && (reason == PDLP_TERMINATION_REASON_PRIMAL_INFEASIBLE ||
+                           reason == PDLP_TERMINATION_REASON_DUAL_INFEASIBLE))
+    reason = 0;
+  // the flush applied the pending sums and A^T y
+  C.pending = 0;
+  C.aty_valid = 1;
+  C.restart_action = 0;
+  if (reason != 0) {
+    h.done = true;
+    h.reason = reason;
+    h.last_stats = st;
+    C.active = 0;
+    if (h.cb && prm_.record_iteration_stats) h.cb(&st, h.user);
+    return;
+  }
+  // run_restart_scheme — src/saddle_point_gpu.jl:479-622 (scaled space)
+  const pdlp_restart_params &rp = prm_.restart_params;
+  const double omega = C.omega;
+  const double pnp = 1.0 / C.eta_entry * omega;
+  const double dnp = 1.0 / C.eta_entry / omega;
+  auto kkt = [&](int c) {
+    const double l2p = sqrt(M[8 + 3 * c] + N[12 + 4 * c]);
+    const double pobj = c0 + N[13 + 4 * c];
+    const double l2d = sqrt(M[9 + 3 * c] + N[14 + 4 * c]);
+    const double dobj = (M[10 + 3 * c] + c0) + N[15 + 4 * c];
+    const double gap = fabs(pobj - dobj);
+    return sqrt(omega * (l2p * l2p) + 1.0 / omega * (l2d * l2d) + gap * gap);
+  };
+  int32_t restart_used = PDLP_RESTART_CHOICE_NO_RESTART;
+  if (C.count > 0) {
+    const int64_t len = C.count;
+    bool do_restart = (double)len >= rp.artificial_restart_threshold * (double)(h.eval_iter - 1);
+    bool reset_to_average = false;
+    double cand = NAN;
+    if (rp.restart_scheme != PDLP_NO_RESTARTS) {
+      const double cur = kkt(0), avg = kkt(1);
+      reset_to_average = rp.restart_to_current_metric == PDLP_KKT_GREEDY ? (cur >= avg) : true;
+      cand = reset_to_average ? avg : cur;
+    }
+    if (!do_restart) {
+      if (rp.restart_scheme == PDLP_ADAPTIVE_KKT) {
+        const double ratio = cand / kkt(2);
+        if (ratio < rp.necessary_reduction_for_restart) {
+          if (ratio < rp.sufficient_reduction_for_restart)
+            do_restart = true;
+          else if (ratio > h.last_trial)
+            do_restart = true;
+        }
+        h.last_trial = ratio;
+      } else if (rp.restart_scheme == PDLP_FIXED_FREQUENCY &&
+                 rp.restart_frequency_if_fixed <= len) {
+        do_restart = true;
+      }
+    }
+    if (do_restart) {
+      C.restart_action = reset_to_average ? 2 : 1;
+      h.pdist = sqrt(pnp) * sqrt(N[24]) / sqrt(omega);
+      h.ddist = sqrt(dnp) * sqrt(M[17]) * sqrt(omega);
+      C.count = 0;
+      C.wsum = 0.0;
+      restart_used = reset_to_average ? PDLP_RESTART_CHOICE_RESTART_TO_AVERAGE
+                                      : PDLP_RESTART_CHOICE_WEIGHTED_AVERAGE_RESET;
+    }
+  }
+  st.restart_used = restart_used;
+  if (restart_used != PDLP_RESTART_CHOICE_NO_RESTART) {
+    // compute_new_primal_weight — src/saddle_point_gpu.jl:627-654
+    const double eps = 2.220446049250313e-16;
+    if (h.pdist > eps && h.ddist > eps) {
+      const double s = rp.primal_weight_update_smoothing;
+      const double est = h.ddist / h.pdist;
+      const double lw = s * log(est) + (1.0 - s) * log(C.omega);
+      C.omega = exp(lw);
+    }
+  }
+  h.last_stats = st;
+  if (h.cb && prm_.record_iteration_stats) h.cb(&st, h.user);
+  const int64_t next = next_eval_iteration(h.eval_iter, prm_.termination_evaluation_frequency,
+                                           tc.iteration_limit);
+  h.iter_start = h.eval_iter;
+  h.acc_at_eval = C.accepted;
+  C.target = C.accepted + (next - h.eval_iter);
+  h.eval_iter = next;
+  C.active = 1;
+  h.needs_eval = false;
+}
+
+int Engine::evaluate(std::string *err) {
+  const size_t ninst = inst_.size();
+  int neval = 0;
+  for (size_t q = 0; q < ninst; ++q) {
+    HostInst &h = *inst_[q];
+    Ctl &C = ctl_[q];
+    const bool ev = !h.done && h.needs_eval;
+    C.eval_flag = ev ? 1 : 0;
+    C.restart_action = 0;
+    if (ev) {
+      C.active = 0;
+      C.use_current = (C.numerical_error || C.count == 0) ? 1 : 0;
+      ++neval;
+    }
+  }
+  if (neval == 0) return 0;
+  memcpy(ctl_pinned_, ctl_.data(), sizeof(Ctl) * ninst);
+  HIPCHK(hipMemcpyAsync(d_ctl_, ctl_pinned_, sizeof(Ctl) * ninst, hipMemcpyHostToDevice, s_));
+  int e = launch_primal(s_, prec_, true, np_, d_pblocks_, d_longs_, d_long_cnt_, d_long_part_,
+                        d_insts_, d_ctl_, d_ppart_);
+  if (!e)
+    e = launch_dual(s_, prec_, true, nd_, d_dblocks_, d_longs_, d_long_cnt_, d_long_part_,
+                    d_insts_, d_ctl_, d_dpart_);
+  if (!e)
+    e = launch_eval(s_, prec_, np_, d_pblocks_, nd_, d_dblocks_, d_insts_, d_ctl_, d_epn_, d_epm_,
+                    (int)ninst, d_out_);
+  if (e) {
+    if (err) *err = std::string("evaluation launch failed: ") + hipGetErrorString((hipError_t)e);
+    return -e;
+  }
+  HIPCHK(hipMemcpyAsync(out_pinned_, d_out_, sizeof(double) * kEvalOut * ninst,
+                        hipMemcpyDeviceToHost, s_));
+  HIPCHK(hipStreamSynchronize(s_));
+  for (size_t q = 0; q < ninst; ++q)
+    if (ctl_[q].eval_flag) host_eval_logic(q, out_pinned_ + q * kEvalOut);
+  memcpy(ctl_pinned_, ctl_.data(), sizeof(Ctl) * ninst);
+  HIPCHK(hipMemcpyAsync(d_ctl_, ctl_pinned_, sizeof(Ctl) * ninst, hipMemcpyHostToDevice, s_));
+  e = launch_restart(s_, prec_, np_, d_pblocks_, nd_, d_dblocks_, d_insts_, d_ctl_);
+  if (e) {
+    if (err) *err = std::string("restart launch failed: ") + hipGetErrorString((hipError_t)e);
+    return -e;
+  }
+  for (size_t q = 0; q < ninst; ++q) {
+    if (ctl_[q].eval_flag && inst_[q]->done) {
+      int rc = finish_instance(q, err);
+      if (rc) return rc;
+    }
+  }
+  return 0;
+}
+
+int Engine::finish_instance(size_t q, std::string *err) {
+  HostInst &h = *inst_[q];
+  const Ctl &C = ctl_[q];
+  pdlp_result *r = h.res;
+  if (!r) return 0;
+  const size_t n = (size_t)h.P.n, m = (size_t)h.P.m;
+  const size_t ts = prec_ ? 4 : 8;
+  std::vector<char> bx(ts * std::max<size_t>(n, 1)), by(ts * std::max<size_t>(m, 1));
+  auto fetch = [&](const void *src, std::vector<char> &dst, size_t cnt) -> hipError_t {
+    if (cnt == 0) return hipSuccess;
+    return hipMemcpyAsync(dst.data(), src, ts * cnt, hipMemcpyDeviceToHost, s_);
+  };
+  auto val = [&](const std::vector<char> &b, size_t i) -> double {
+    return prec_ ? (double)((const float *)b.data())[i] : ((const double *)b.data())[i];
+  };
+  HIPCHK(fetch(h.dev.avg_x, bx, n));
+  HIPCHK(fetch(h.dev.avg_y, by, m));
+  HIPCHK(hipStreamSynchronize(s_));
+  for (size_t j = 0; j < n; ++j) r->x[j] = val(bx, j) / h.P.D[j];
+  for (size_t i = 0; i < m; ++i) r->y[i] = val(by, i) / h.P.E[i];
+  if (r->x_current || r->y_current) {
+    HIPCHK(fetch(C.cur ? h.dev.x1 : h.dev.x0, bx, n));
+    HIPCHK(fetch(C.cur ? h.dev.y1 : h.dev.y0, by, m));
+    HIPCHK(hipStreamSynchronize(s_));
+    if (r->x_current)
+      for (size_t j = 0; j < n; ++j) r->x_current[j] = val(bx, j) / h.P.D[j];
+    if (r->y_current)
+      for (size_t i = 0; i < m; ++i) r->y_current[i] = val(by, i) / h.P.E[i];
+  }
+  r->termination_reason = h.reason;
+  r->iteration_count = (int32_t)(h.eval_iter - 1);
+  r->final_stats = h.last_stats;
+  r->total_attempts = C.attempts;
+  return 0;
+}
+
+int Engine::step_round(std::string *err) {
+  if (tables_dirty_) {
+    int rc = rebuild_tables(err);
+    if (rc) return rc;
+  }
+  bool any_eval = false;
+  for (auto &h : inst_)
+    if (!h->done && h->needs_eval) any_eval = true;
+  if (any_eval) {
+    int rc = evaluate(err);
+    if (rc) return rc;
+  }
+  int64_t R = 0;
+  bool all_done = true;
+  for (size_t q = 0; q < inst_.size(); ++q) {
+    if (inst_[q]->done) continue;
+    all_done = false;
+    R = std::max<int64_t>(R, ctl_[q].target - ctl_[q].accepted);
+  }
+  if (all_done) return 1;
+  if (R <= 0) return 0;
+  R = std::min<int64_t>(R, std::max<int32_t>(prm_.termination_evaluation_frequency, 10));
+  int rc = run_attempts(R, err);
+  return rc < 0 ? rc : 0;
+}
+
+int Engine::run(std::string *err) {
+  for (;;) {
+    int r = step_round(err);
+    if (r < 0) return r;
+    if (r == 1) return 0;
+  }
+}
+
+void Engine::remove_done() {
+  std::vector<std::unique_ptr<HostInst>> keep;
+  std::vector<Ctl> kc;
+  for (size_t q = 0; q < inst_.size(); ++q) {
+    if (inst_[q]->done) {
+      if (inst_[q]->arena) pool_->put(inst_[q]->arena);
+      inst_[q]->arena = nullptr;
+      continue;
+    }
+    keep.push_back(std::move(inst_[q]));
+    kc.push_back(ctl_[q]);
+  }
+  if (keep.size() != inst_.size()) tables_dirty_ = true;
+  inst_ = std::move(keep);
+  ctl_ = std::move(kc);
+}
+
+}  // namespace pdlp

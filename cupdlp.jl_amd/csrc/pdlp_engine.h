@@ -1,0 +1,119 @@
+// pdlp_engine.h — the host driver of the restarted-PDHG engine.
+//
+// An Engine owns a packed set of LP instances on one HIP stream.  It runs the
+// reference's optimize() loop (src/primal_dual_hybrid_gradient_gpu.jl:457-798)
+// for every instance at once: step attempts run on the device without host
+// synchronisation; the host takes over only at the evaluation points
+// (iterations 1..10, then every termination_evaluation_frequency), where it
+// reads back a few dozen reduced scalars per instance and applies the
+// termination test (src/termination.jl:206-246), the restart scheme
+// (src/saddle_point_gpu.jl:479-622) and the primal-weight update (:627-654).
+#pragma once
+#include <hip/hip_runtime_api.h>
+#include <stdint.h>
+
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "pdlp_device.h"
+#include "pdlp_hip.h"
+#include "pdlp_prep.h"
+
+namespace pdlp {
+
+// Caching device allocator owned by a context.
+class DevicePool {
+ public:
+  ~DevicePool();
+  void *get(size_t bytes);
+  void put(void *p);
+  void release_all();
+  size_t in_use() const { return in_use_; }
+
+ private:
+  struct Blk { void *p; size_t bytes; };
+  std::vector<Blk> free_, used_;
+  size_t in_use_ = 0;
+};
+
+struct HipError {
+  hipError_t code;
+  std::string where;
+};
+
+// Per-instance host state of the optimize() loop.
+struct HostInst {
+  Prepared P;
+  InstDev dev{};
+  void *arena = nullptr;
+  size_t arena_bytes = 0;
+  int64_t eval_iter = 1;    // value of the reference's `iteration` at the next evaluation
+  int64_t acc_at_eval = 0;  // accepted steps at the last evaluation
+  int64_t iter_start = 1;   // `iteration` of the last evaluation
+  bool needs_eval = true;
+  bool done = false;
+  int32_t reason = 0;
+  double start_time = 0.0, basic_time = 0.0;
+  double last_trial = 1.0, pdist = 0.0, ddist = 0.0;
+  pdlp_iteration_stats last_stats{};
+  pdlp_result *res = nullptr;
+  pdlp_stats_cb cb = nullptr;
+  void *user = nullptr;
+  double tolerance_override = -1.0;  // serving: per-request tolerance
+  int64_t tag = -1;                  // serving: request id
+};
+
+class Engine {
+ public:
+  Engine(hipStream_t s, DevicePool *pool, const pdlp_params &prm);
+  ~Engine();
+
+  // Adds an instance (already prepared/rescaled on the host).
+  int add(std::unique_ptr<HostInst> inst, std::string *err);
+  // Runs all instances to termination.  Returns 0 or a negative HIP error.
+  int run(std::string *err);
+  // Runs one round: pending evaluations, then one chunk of attempts.  Returns
+  // 0 while work remains, 1 when all instances are done, < 0 on error.
+  int step_round(std::string *err);
+  size_t count() const { return inst_.size(); }
+  HostInst &inst(size_t i) { return *inst_[i]; }
+  void remove_done();  // drops finished instances (serving)
+
+ private:
+  int upload(HostInst &h, std::string *err);
+  int rebuild_tables(std::string *err);
+  int ensure_step_table(int64_t need, std::string *err);
+  int evaluate(std::string *err);
+  int run_attempts(int64_t count, std::string *err);
+  int finish_instance(size_t q, std::string *err);
+  void host_eval_logic(size_t q, const double *out);
+
+  hipStream_t s_;
+  DevicePool *pool_;
+  pdlp_params prm_;
+  int prec_ = 0;
+  std::vector<std::unique_ptr<HostInst>> inst_;
+  std::vector<Ctl> ctl_;  // host mirror (pinned copy below)
+  Ctl *ctl_pinned_ = nullptr;
+  double *out_pinned_ = nullptr;
+  size_t out_cap_ = 0;
+  // device tables
+  bool tables_dirty_ = true;
+  int np_ = 0, nd_ = 0, nlong_ = 0, nslots_ = 0;
+  BlockDesc *d_pblocks_ = nullptr, *d_dblocks_ = nullptr;
+  LongRow *d_longs_ = nullptr;
+  int32_t *d_long_cnt_ = nullptr;
+  double *d_long_part_ = nullptr;
+  InstDev *d_insts_ = nullptr;
+  Ctl *d_ctl_ = nullptr;
+  double *d_ppart_ = nullptr, *d_dpart_ = nullptr, *d_epn_ = nullptr, *d_epm_ = nullptr,
+         *d_out_ = nullptr;
+  double *d_red_ = nullptr, *d_grow_ = nullptr;
+  int64_t tab_len_ = 0;
+  std::vector<void *> table_allocs_;
+};
+
+double now_sec();
+
+}  // namespace pdlp

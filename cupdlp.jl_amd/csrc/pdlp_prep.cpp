@@ -1,0 +1,256 @@
+// pdlp_prep.cpp — validation, cached norms, host rescaling and CSR layouts.
+#include "pdlp_prep.h"
+
+#include <math.h>
+#include <string.h>
+
+#include <algorithm>
+#include <cstdio>
+
+namespace pdlp {
+
+static inline double jl_max(double a, double b) {
+  if (a != a || b != b) return a + b;
+  if (a > b) return a;
+  if (b > a) return b;
+  return signbit(a) ? b : a;
+}
+
+int validate_lp(const pdlp_lp *lp, std::string *err) {
+  char buf[512];
+  if (!lp) {
+    *err = "null problem";
+    return 1;
+  }
+  if (lp->n < 0 || lp->m < 0 || lp->num_eq < 0 || lp->num_eq > lp->m || lp->nnz < 0) {
+    snprintf(buf, sizeof buf, "invalid sizes n=%lld m=%lld num_eq=%lld nnz=%lld",
+             (long long)lp->n, (long long)lp->m, (long long)lp->num_eq, (long long)lp->nnz);
+    *err = buf;
+    return 1;
+  }
+  if (lp->n >= INT32_MAX || lp->m >= INT32_MAX || lp->nnz >= INT32_MAX) {
+    *err = "problem exceeds the Int32 device index range";
+    return 1;
+  }
+  if (!lp->colptr || (lp->n > 0 && (!lp->c || !lp->lb || !lp->ub)) || (lp->m > 0 && !lp->b) ||
+      (lp->nnz > 0 && (!lp->rowval || !lp->nzval))) {
+    *err = "null array in problem";
+    return 1;
+  }
+  if (lp->colptr[0] != 1 || lp->colptr[lp->n] - 1 != lp->nnz) {
+    *err = "colptr inconsistent with nnz";
+    return 1;
+  }
+  for (int64_t j = 0; j < lp->n; ++j) {
+    if (lp->colptr[j + 1] < lp->colptr[j]) {
+      snprintf(buf, sizeof buf, "colptr not monotone at column %lld", (long long)j + 1);
+      *err = buf;
+      return 1;
+    }
+    for (int64_t k = lp->colptr[j] - 1; k < lp->colptr[j + 1] - 1; ++k) {
+      if (lp->rowval[k] < 1 || lp->rowval[k] > lp->m ||
+          (k > lp->colptr[j] - 1 && lp->rowval[k] <= lp->rowval[k - 1])) {
+        snprintf(buf, sizeof buf, "row index out of range or unsorted in column %lld",
+                 (long long)j + 1);
+        *err = buf;
+        return 1;
+      }
+    }
+  }
+  int bad = 0;
+  for (int64_t j = 0; j < lp->n; ++j) {
+    if (lp->lb[j] == INFINITY) bad |= 1;
+    if (lp->ub[j] == -INFINITY) bad |= 2;
+    if (isnan(lp->lb[j]) || isnan(lp->ub[j])) bad |= 4;
+    if (!isfinite(lp->c[j])) bad |= 16;
+  }
+  for (int64_t i = 0; i < lp->m; ++i)
+    if (!isfinite(lp->b[i])) bad |= 8;
+  for (int64_t k = 0; k < lp->nnz; ++k)
+    if (!isfinite(lp->nzval[k])) bad |= 32;
+  if (bad) {
+    snprintf(buf, sizeof buf,
+             "Error found when validating QuadraticProgrammingProblem (flags %d: 1 lb==Inf, "
+             "2 ub==-Inf, 4 NaN bound, 8 non-finite rhs, 16 non-finite objective, 32 "
+             "non-finite matrix)",
+             bad);
+    *err = buf;
+    return 1;
+  }
+  return 0;
+}
+
+static double norm2(const std::vector<double> &v) {
+  // blocked sum of squares, blocks combined in order
+  const size_t B = 4096;
+  double s = 0.0;
+  for (size_t k = 0; k < v.size(); k += B) {
+    double t = 0.0;
+    const size_t e = std::min(v.size(), k + B);
+    for (size_t i = k; i < e; ++i) t += v[i] * v[i];
+    s += t;
+  }
+  return sqrt(s);
+}
+
+static double norminf(const std::vector<double> &v) {
+  double mx = 0.0;
+  for (double x : v) {
+    const double t = fabs(x);
+    if (t > mx || t != t) mx = t;
+  }
+  return mx;
+}
+
+int load_problem(const pdlp_lp *lp, Prepared *P, std::string *err) {
+  int rc = validate_lp(lp, err);
+  if (rc) return rc;
+  const int64_t n = lp->n, m = lp->m, nnz = lp->nnz;
+  P->n = n; P->m = m; P->neq = lp->num_eq; P->nnz = nnz; P->c0 = lp->c0;
+  P->t_ptr.resize(n + 1);
+  for (int64_t j = 0; j <= n; ++j) P->t_ptr[j] = (int32_t)(lp->colptr[j] - 1);
+  P->t_col.resize(nnz);
+  P->t_val.resize(nnz);
+  for (int64_t k = 0; k < nnz; ++k) {
+    P->t_col[k] = (int32_t)(lp->rowval[k] - 1);
+    P->t_val[k] = lp->nzval[k];
+  }
+  P->oc.assign(lp->c, lp->c + n);
+  P->ol.assign(lp->lb, lp->lb + n);
+  P->ou.assign(lp->ub, lp->ub + n);
+  P->ob.assign(lp->b, lp->b + m);
+  P->c = P->oc; P->l = P->ol; P->u = P->ou; P->b = P->ob;
+  P->E.assign(m, 1.0);
+  P->D.assign(n, 1.0);
+  P->qc.l_inf_c = norminf(P->oc);
+  P->qc.l_inf_b = norminf(P->ob);
+  P->qc.l2_c = norm2(P->oc);
+  P->qc.l2_b = norm2(P->ob);
+  // CSR of A by a counting sort (column indices in each row ascending)
+  P->a_ptr.assign(m + 1, 0);
+  for (int64_t k = 0; k < nnz; ++k) P->a_ptr[P->t_col[k] + 1]++;
+  for (int64_t i = 0; i < m; ++i) P->a_ptr[i + 1] += P->a_ptr[i];
+  P->a_col.resize(nnz);
+  P->a_val.resize(nnz);
+  P->csr_from_csc.resize(nnz);
+  std::vector<int32_t> fill(P->a_ptr.begin(), P->a_ptr.end() - (m >= 0 ? 1 : 0));
+  for (int64_t j = 0; j < n; ++j)
+    for (int32_t k = P->t_ptr[j]; k < P->t_ptr[j + 1]; ++k) {
+      const int32_t i = P->t_col[k];
+      const int32_t p = fill[i]++;
+      P->a_col[p] = (int32_t)j;
+      P->a_val[p] = P->t_val[k];
+      P->csr_from_csc[p] = k;
+    }
+  P->max_abs_scaled = norminf(P->t_val);
+  return 0;
+}
+
+// scale_problem — src/preprocess.jl:541-561, with Diag(1/r) * A * Diag(1/v)
+// evaluated entrywise as ((1/r_i) * a_ij) * (1/v_j).
+static void scale(Prepared *P, const std::vector<double> &r, const std::vector<double> &v) {
+  for (int64_t j = 0; j < P->n; ++j) {
+    P->c[j] /= v[j];
+    P->u[j] *= v[j];
+    P->l[j] *= v[j];
+  }
+  for (int64_t i = 0; i < P->m; ++i) P->b[i] /= r[i];
+  for (int64_t j = 0; j < P->n; ++j) {
+    const double iv = 1.0 / v[j];
+    for (int32_t k = P->t_ptr[j]; k < P->t_ptr[j + 1]; ++k) {
+      const double ir = 1.0 / r[P->t_col[k]];
+      P->t_val[k] = (ir * P->t_val[k]) * iv;
+    }
+  }
+}
+
+void rescale_host(const pdlp_params *prm, Prepared *P) {
+  const int64_t n = P->n, m = P->m;
+  std::vector<double> r(m), v(n);
+  // ruiz_rescaling(p = Inf) — src/preprocess.jl:398-463 (objective matrix zero)
+  for (int64_t it = 0; it < prm->l_inf_ruiz_iterations; ++it) {
+    for (int64_t j = 0; j < n; ++j) {
+      double mx = 0.0;
+      for (int32_t k = P->t_ptr[j]; k < P->t_ptr[j + 1]; ++k) mx = jl_max(mx, fabs(P->t_val[k]));
+      v[j] = sqrt(jl_max(mx, 0.0));
+      if (v[j] == 0.0) v[j] = 1.0;
+    }
+    if (m > 0) {
+      std::fill(r.begin(), r.end(), 0.0);
+      for (int64_t j = 0; j < n; ++j)
+        for (int32_t k = P->t_ptr[j]; k < P->t_ptr[j + 1]; ++k)
+          r[P->t_col[k]] = jl_max(r[P->t_col[k]], fabs(P->t_val[k]));
+      for (int64_t i = 0; i < m; ++i) {
+        r[i] = sqrt(r[i]);
+        if (r[i] == 0.0) r[i] = 1.0;
+      }
+    }
+    scale(P, r, v);
+    for (int64_t i = 0; i < m; ++i) P->E[i] *= r[i];
+    for (int64_t j = 0; j < n; ++j) P->D[j] *= v[j];
+  }
+  // l2_norm_rescaling — src/preprocess.jl:89-103, 344-358
+  if (prm->l2_norm_rescaling) {
+    std::vector<double> rsf(m, 0.0), csf(n, 0.0), rs(m, 0.0), cs(n, 0.0);
+    for (int64_t j = 0; j < n; ++j)
+      for (int32_t k = P->t_ptr[j]; k < P->t_ptr[j + 1]; ++k) {
+        const double a = fabs(P->t_val[k]);
+        rsf[P->t_col[k]] = jl_max(rsf[P->t_col[k]], a);
+        csf[j] = jl_max(csf[j], a);
+      }
+    for (auto &x : rsf) if (x == 0.0) x = 1.0;
+    for (auto &x : csf) if (x == 0.0) x = 1.0;
+    for (int64_t j = 0; j < n; ++j)
+      for (int32_t k = P->t_ptr[j]; k < P->t_ptr[j + 1]; ++k) {
+        const int32_t i = P->t_col[k];
+        const double tr = (1.0 / rsf[i]) * P->t_val[k];
+        const double tc = P->t_val[k] * (1.0 / csf[j]);
+        rs[i] += tr * tr;
+        cs[j] += tc * tc;
+      }
+    for (int64_t i = 0; i < m; ++i) {
+      double nr = rsf[i] * sqrt(rs[i]);
+      if (nr == 0.0) nr = 1.0;
+      r[i] = sqrt(nr);
+    }
+    for (int64_t j = 0; j < n; ++j) {
+      double nc = csf[j] * sqrt(cs[j]);
+      if (nc == 0.0) nc = 1.0;
+      v[j] = sqrt(nc);
+    }
+    scale(P, r, v);
+    for (int64_t i = 0; i < m; ++i) P->E[i] *= r[i];
+    for (int64_t j = 0; j < n; ++j) P->D[j] *= v[j];
+  }
+  // pock_chambolle_rescaling — src/preprocess.jl:494-525
+  if (prm->has_pock_chambolle_alpha) {
+    const double alpha = prm->pock_chambolle_alpha;
+    const double ec = 2.0 - alpha, er = alpha;
+    for (int64_t j = 0; j < n; ++j) {
+      double s = 0.0;
+      if (ec == 0.0)
+        s = (double)m;
+      else
+        for (int32_t k = P->t_ptr[j]; k < P->t_ptr[j + 1]; ++k) s += pow(fabs(P->t_val[k]), ec);
+      v[j] = sqrt(s);
+    }
+    std::fill(r.begin(), r.end(), 0.0);
+    if (er == 0.0) {
+      std::fill(r.begin(), r.end(), (double)n);
+    } else {
+      for (int64_t j = 0; j < n; ++j)
+        for (int32_t k = P->t_ptr[j]; k < P->t_ptr[j + 1]; ++k)
+          r[P->t_col[k]] += pow(fabs(P->t_val[k]), er);
+    }
+    for (int64_t i = 0; i < m; ++i) r[i] = sqrt(r[i]);
+    for (auto &x : v) if (x == 0.0) x = 1.0;
+    for (auto &x : r) if (x == 0.0) x = 1.0;
+    scale(P, r, v);
+    for (int64_t i = 0; i < m; ++i) P->E[i] *= r[i];
+    for (int64_t j = 0; j < n; ++j) P->D[j] *= v[j];
+  }
+  for (int64_t p = 0; p < P->nnz; ++p) P->a_val[p] = P->t_val[P->csr_from_csc[p]];
+  P->max_abs_scaled = norminf(P->t_val);
+}
+
+}  // namespace pdlp

@@ -1,0 +1,49 @@
+// pdlp_prep.h — host-side problem preparation: validation, the cached norms
+// of the original problem, diagonal rescaling and the CSR layouts the
+// kernels read.
+#pragma once
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "pdlp_hip.h"
+
+namespace pdlp {
+
+struct QpCache {  // CachedQuadraticProgramInfo, src/termination.jl:123-137
+  double l_inf_c, l_inf_b, l2_c, l2_b;
+};
+
+// An LP after rescale_problem, in the device layouts (0-based, int32).
+struct Prepared {
+  int64_t n = 0, m = 0, neq = 0, nnz = 0;
+  double c0 = 0.0;
+  // A^T as CSR = A as CSC
+  std::vector<int32_t> t_ptr, t_col;
+  std::vector<double> t_val;
+  // A as CSR
+  std::vector<int32_t> a_ptr, a_col;
+  std::vector<double> a_val;
+  std::vector<int64_t> csr_from_csc;  // a_val[p] = t_val[csr_from_csc[p]]
+  // scaled vectors and the scaling (E rows, D columns)
+  std::vector<double> c, l, u, b, E, D;
+  // original vectors
+  std::vector<double> oc, ol, ou, ob;
+  QpCache qc{};
+  double max_abs_scaled = 0.0;  // norm(scaled A, Inf) (elementwise)
+};
+
+// validate (src/preprocess.jl:8-74) plus index checks of the C view.
+// Returns 0 or a positive code with *err set.
+int validate_lp(const pdlp_lp *lp, std::string *err);
+
+// Loads the original problem and its cached norms; builds both layouts.
+// Rescaling is applied by rescale_host (or on the device).
+int load_problem(const pdlp_lp *lp, Prepared *P, std::string *err);
+
+// rescale_problem (src/preprocess.jl:619-675) on the host: Ruiz-inf,
+// optional L2, optional Pock-Chambolle.  Updates t_val/a_val and the vectors.
+void rescale_host(const pdlp_params *prm, Prepared *P);
+
+}  // namespace pdlp

@@ -1,0 +1,239 @@
+"""``optimize(params, problem)`` over the C ABI of libpdlp_hip.so.
+
+Mirrors cuPDLP.optimize (src/primal_dual_hybrid_gradient_gpu.jl:457-798):
+same arguments, same SaddlePointOutput, same error behaviour (invalid input
+raises, like the reference's ``error()`` from ``validate``).
+"""
+import ctypes as C
+import threading
+
+import numpy as np
+
+from . import _abi
+from .types import (AdaptiveStepsizeParams, ConvergenceInformation, InfeasibilityInformation,
+                    IterationStats, PdhgParameters, PointType, QuadraticProgrammingProblem,
+                    RestartChoice, SaddlePointOutput, TerminationReason,
+                    termination_reason_to_string)
+
+
+class PdlpError(RuntimeError):
+    """Raised for invalid input (code > 0) or a device failure (code < 0)."""
+
+    def __init__(self, code, msg):
+        super().__init__(f"pdlp error {code}: {msg}")
+        self.code = code
+
+
+def _ptr(a, ct):
+    return a.ctypes.data_as(C.POINTER(ct))
+
+
+def to_c_lp(problem: QuadraticProgrammingProblem):
+    """Builds a borrowed pdlp_lp view (1-based CSC, Int64) and the arrays it
+    points to (kept alive by the returned tuple)."""
+    A = problem.constraint_matrix
+    m, n = A.shape
+    colptr = np.ascontiguousarray(A.indptr, dtype=np.int64) + 1
+    rowval = np.ascontiguousarray(A.indices, dtype=np.int64) + 1
+    nzval = np.ascontiguousarray(A.data, dtype=np.float64)
+    c = np.ascontiguousarray(problem.objective_vector, dtype=np.float64)
+    lb = np.ascontiguousarray(problem.variable_lower_bound, dtype=np.float64)
+    ub = np.ascontiguousarray(problem.variable_upper_bound, dtype=np.float64)
+    b = np.ascontiguousarray(problem.right_hand_side, dtype=np.float64)
+    if len(c) != n or len(lb) != n or len(ub) != n or len(b) != m:
+        raise PdlpError(1, "vector lengths do not match the constraint matrix "
+                           "(src/preprocess.jl:8-74)")
+    keep = (colptr, rowval, nzval, c, lb, ub, b)
+    lp = _abi.pdlp_lp(n=n, m=m, num_eq=problem.num_equalities, nnz=int(A.nnz),
+                      colptr=_ptr(colptr, C.c_int64), rowval=_ptr(rowval, C.c_int64),
+                      nzval=_ptr(nzval, C.c_double), c=_ptr(c, C.c_double),
+                      lb=_ptr(lb, C.c_double), ub=_ptr(ub, C.c_double),
+                      b=_ptr(b, C.c_double), c0=problem.objective_constant)
+    return lp, keep
+
+
+def to_c_params(p: PdhgParameters) -> _abi.pdlp_params:
+    cp = _abi.pdlp_params()
+    cp.l_inf_ruiz_iterations = int(p.l_inf_ruiz_iterations)
+    cp.l2_norm_rescaling = int(bool(p.l2_norm_rescaling))
+    cp.has_pock_chambolle_alpha = 0 if p.pock_chambolle_alpha is None else 1
+    cp.pock_chambolle_alpha = 0.0 if p.pock_chambolle_alpha is None else float(p.pock_chambolle_alpha)
+    cp.primal_importance = float(p.primal_importance)
+    cp.scale_invariant_initial_primal_weight = int(bool(p.scale_invariant_initial_primal_weight))
+    cp.record_iteration_stats = int(bool(p.record_iteration_stats))
+    cp.verbosity = int(p.verbosity)
+    cp.termination_evaluation_frequency = int(p.termination_evaluation_frequency)
+    if isinstance(p.step_size_policy_params, AdaptiveStepsizeParams):
+        cp.step_size_policy = 0
+        cp.reduction_exponent = float(p.step_size_policy_params.reduction_exponent)
+        cp.growth_exponent = float(p.step_size_policy_params.growth_exponent)
+    else:
+        cp.step_size_policy = 1
+    tc = p.termination_criteria
+    cp.termination_criteria.optimality_norm = int(tc.optimality_norm)
+    cp.termination_criteria.iteration_limit = int(min(tc.iteration_limit, 2**31 - 1))
+    cp.termination_criteria.eps_optimal_absolute = float(tc.eps_optimal_absolute)
+    cp.termination_criteria.eps_optimal_relative = float(tc.eps_optimal_relative)
+    cp.termination_criteria.eps_primal_infeasible = float(tc.eps_primal_infeasible)
+    cp.termination_criteria.eps_dual_infeasible = float(tc.eps_dual_infeasible)
+    cp.termination_criteria.time_sec_limit = float(tc.time_sec_limit)
+    cp.termination_criteria.kkt_matrix_pass_limit = float(tc.kkt_matrix_pass_limit)
+    rp = p.restart_params
+    cp.restart_params.restart_scheme = int(rp.restart_scheme)
+    cp.restart_params.restart_to_current_metric = int(rp.restart_to_current_metric)
+    cp.restart_params.restart_frequency_if_fixed = int(rp.restart_frequency_if_fixed)
+    cp.restart_params.artificial_restart_threshold = float(rp.artificial_restart_threshold)
+    cp.restart_params.sufficient_reduction_for_restart = float(rp.sufficient_reduction_for_restart)
+    cp.restart_params.necessary_reduction_for_restart = float(rp.necessary_reduction_for_restart)
+    cp.restart_params.primal_weight_update_smoothing = float(rp.primal_weight_update_smoothing)
+    cp.precision = int(p.precision)
+    cp.refine_max_outer = int(p.refine_max_outer)
+    cp.refine_inner_tolerance = float(p.refine_inner_tolerance)
+    cp.refine_max_scale = float(p.refine_max_scale)
+    cp.device_preconditioning = int(bool(p.device_preconditioning))
+    return cp
+
+
+_CI_FIELDS = [f for f, _ in _abi.pdlp_convergence_information._fields_ if f not in ("candidate_type", "pad_")]
+_II_FIELDS = [f for f, _ in _abi.pdlp_infeasibility_information._fields_ if f not in ("candidate_type", "pad_")]
+
+
+def from_c_stats(s: _abi.pdlp_iteration_stats) -> IterationStats:
+    ci = ConvergenceInformation(PointType(s.convergence_information.candidate_type),
+                                *[getattr(s.convergence_information, f) for f in _CI_FIELDS])
+    ii = InfeasibilityInformation(PointType(s.infeasibility_information.candidate_type),
+                                  *[getattr(s.infeasibility_information, f) for f in _II_FIELDS])
+    return IterationStats(
+        iteration_number=s.iteration_number, convergence_information=[ci],
+        infeasibility_information=[ii],
+        cumulative_kkt_matrix_passes=s.cumulative_kkt_matrix_passes,
+        cumulative_rejected_steps=s.cumulative_rejected_steps,
+        cumulative_time_sec=s.cumulative_time_sec, restart_used=RestartChoice(s.restart_used),
+        step_size=s.step_size, primal_weight=s.primal_weight,
+        method_specific_stats={
+            "time_spent_doing_basic_algorithm": s.time_spent_doing_basic_algorithm,
+            "total_attempts": float(s.total_attempts)})
+
+
+def run_solve(call, problem: QuadraticProgrammingProblem, params: PdhgParameters,
+              want_current=False):
+    """Shared marshalling: ``call(lp, params, result, cb, user) -> int``."""
+    lp, keep = to_c_lp(problem)
+    cp = to_c_params(params)
+    n, m = problem.num_variables, problem.num_constraints
+    x = np.zeros(max(n, 1)); y = np.zeros(max(m, 1))
+    xc = np.zeros(max(n, 1)) if want_current else None
+    yc = np.zeros(max(m, 1)) if want_current else None
+    res = _abi.pdlp_result()
+    res.x = _ptr(x, C.c_double); res.y = _ptr(y, C.c_double)
+    if want_current:
+        res.x_current = _ptr(xc, C.c_double); res.y_current = _ptr(yc, C.c_double)
+    stats = []
+
+    def _cb(sp, _user):
+        stats.append(from_c_stats(sp.contents))
+
+    cb = _abi.STATS_CB(_cb)
+    rc = call(C.byref(lp), C.byref(cp), C.byref(res), cb, None)
+    del keep
+    if rc != 0:
+        return rc, None
+    final = from_c_stats(res.final_stats)
+    if not stats or stats[-1].iteration_number != final.iteration_number:
+        stats.append(final)
+    else:
+        stats[-1] = final
+    reason = TerminationReason(res.termination_reason)
+    out = SaddlePointOutput(
+        primal_solution=x[:n].copy(), dual_solution=y[:m].copy(), termination_reason=reason,
+        termination_string=termination_reason_to_string(reason),
+        iteration_count=int(res.iteration_count), iteration_stats=stats,
+        solve_time_sec=res.solve_time_sec, total_attempts=int(res.total_attempts),
+        refinement_rounds=int(res.refinement_rounds),
+        primal_current=None if xc is None else xc[:n].copy(),
+        dual_current=None if yc is None else yc[:m].copy())
+    return 0, out
+
+
+class Context:
+    """One HIP stream + device pool on one GPU (pdlp_ctx)."""
+
+    def __init__(self, device=0):
+        self.lib = _abi.load()
+        h = C.c_void_p()
+        rc = self.lib.pdlp_ctx_create(int(device), C.byref(h))
+        if rc != 0:
+            raise PdlpError(rc, self.lib.pdlp_last_error(None).decode())
+        self.handle = h
+
+    def close(self):
+        if self.handle:
+            self.lib.pdlp_ctx_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def error(self):
+        return self.lib.pdlp_last_error(self.handle).decode()
+
+    def solve(self, problem, params, want_current=False):
+        def call(lp, cp, res, cb, user):
+            return self.lib.pdlp_solve(self.handle, lp, cp, res, cb, user)
+        rc, out = run_solve(call, problem, params, want_current)
+        if rc != 0:
+            raise PdlpError(rc, self.error())
+        return out
+
+    def batch_solve(self, problems, params):
+        k = len(problems)
+        lps = (_abi.pdlp_lp * k)()
+        keeps, outs, results = [], [], (_abi.pdlp_result * k)()
+        for i, p in enumerate(problems):
+            lp, keep = to_c_lp(p)
+            lps[i] = lp
+            n, m = p.num_variables, p.num_constraints
+            x = np.zeros(max(n, 1)); y = np.zeros(max(m, 1))
+            keeps.append((keep, x, y))
+            results[i].x = _ptr(x, C.c_double); results[i].y = _ptr(y, C.c_double)
+        cp = to_c_params(params)
+        rc = self.lib.pdlp_batch_solve(self.handle, k, lps, C.byref(cp), results)
+        if rc != 0:
+            raise PdlpError(rc, self.error())
+        for i, p in enumerate(problems):
+            (_, x, y) = keeps[i]
+            r = results[i]
+            reason = TerminationReason(r.termination_reason)
+            outs.append(SaddlePointOutput(
+                primal_solution=x[:p.num_variables].copy(), dual_solution=y[:p.num_constraints].copy(),
+                termination_reason=reason, termination_string=termination_reason_to_string(reason),
+                iteration_count=int(r.iteration_count), iteration_stats=[from_c_stats(r.final_stats)],
+                solve_time_sec=r.solve_time_sec, total_attempts=int(r.total_attempts),
+                refinement_rounds=int(r.refinement_rounds)))
+        return outs
+
+
+_ctx_lock = threading.Lock()
+_default_ctx = None
+
+
+def default_context() -> Context:
+    global _default_ctx
+    with _ctx_lock:
+        if _default_ctx is None:
+            _default_ctx = Context(0)
+        return _default_ctx
+
+
+def optimize(params: PdhgParameters, original_problem: QuadraticProgrammingProblem,
+             want_current=False) -> SaddlePointOutput:
+    """cuPDLP.optimize on the GPU (src/primal_dual_hybrid_gradient_gpu.jl:457-798)."""
+    return default_context().solve(original_problem, params, want_current)
+
+
+def default_params() -> PdhgParameters:
+    """Defaults of cuPDLP.Optimizer() (src/MOI_wrapper.jl:58-78)."""
+    return PdhgParameters()

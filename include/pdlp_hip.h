@@ -1,0 +1,274 @@
+/*
+ * pdlp_hip.h — C ABI of the MI355X restarted-PDHG LP engine (libpdlp_hip.so).
+ *
+ * This is the drop-in boundary that replaces the body of cuPDLP.jl's
+ *   optimize(params::PdhgParameters, original_problem::QuadraticProgrammingProblem)
+ *       ::SaddlePointOutput            (src/primal_dual_hybrid_gradient_gpu.jl:457-798)
+ * A Julia `cuPDLP` module keeps its public types and `cuPDLP.Optimizer`
+ * (src/MOI_wrapper.jl) and `ccall`s the functions below.  Only plain C types
+ * cross the boundary: no C++ or torch types, no CUDA/HIP types.
+ *
+ * Conventions
+ *  - The LP is given exactly as Julia's SparseMatrixCSC{Float64,Int64} holds
+ *    it: 1-based CSC (colptr[n+1], rowval[nnz], nzval[nnz]).  Host arrays are
+ *    borrowed for the duration of a call only and never retained.
+ *  - Rows 1..num_eq are equalities, rows num_eq+1..m are ">=" rows
+ *    (src/quadratic_programming.jl:6-22).
+ *  - Enum integer values equal the reference's Julia @enum values (first = 0).
+ *  - Return codes: 0 ok; > 0 invalid input (message in pdlp_last_error);
+ *    < 0 HIP / runtime failure (message in pdlp_last_error).
+ */
+#ifndef PDLP_HIP_H
+#define PDLP_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PDLP_ABI_VERSION 1
+
+/* TerminationReason — src/solve_log.jl:323-334 */
+enum {
+  PDLP_TERMINATION_REASON_UNSPECIFIED = 0,
+  PDLP_TERMINATION_REASON_OPTIMAL = 1,
+  PDLP_TERMINATION_REASON_PRIMAL_INFEASIBLE = 2,
+  PDLP_TERMINATION_REASON_DUAL_INFEASIBLE = 3,
+  PDLP_TERMINATION_REASON_TIME_LIMIT = 4,
+  PDLP_TERMINATION_REASON_ITERATION_LIMIT = 5,
+  PDLP_TERMINATION_REASON_KKT_MATRIX_PASS_LIMIT = 6,
+  PDLP_TERMINATION_REASON_NUMERICAL_ERROR = 7,
+  PDLP_TERMINATION_REASON_INVALID_PROBLEM = 8,
+  PDLP_TERMINATION_REASON_OTHER = 9
+};
+
+/* RestartChoice — src/solve_log.jl:19-24 */
+enum {
+  PDLP_RESTART_CHOICE_UNSPECIFIED = 0,
+  PDLP_RESTART_CHOICE_NO_RESTART = 1,
+  PDLP_RESTART_CHOICE_WEIGHTED_AVERAGE_RESET = 2,
+  PDLP_RESTART_CHOICE_RESTART_TO_AVERAGE = 3
+};
+
+/* PointType — src/solve_log.jl:39-45 */
+enum {
+  PDLP_POINT_TYPE_UNSPECIFIED = 0,
+  PDLP_POINT_TYPE_CURRENT_ITERATE = 1,
+  PDLP_POINT_TYPE_ITERATE_DIFFERENCE = 2,
+  PDLP_POINT_TYPE_AVERAGE_ITERATE = 3,
+  PDLP_POINT_TYPE_NONE = 4
+};
+
+/* RestartScheme, RestartToCurrentMetric — src/saddle_point_gpu.jl:343,350 */
+enum { PDLP_NO_RESTARTS = 0, PDLP_FIXED_FREQUENCY = 1, PDLP_ADAPTIVE_KKT = 2 };
+enum { PDLP_NO_RESTART_TO_CURRENT = 0, PDLP_KKT_GREEDY = 1 };
+
+/* OptimalityNorm — src/termination.jl:6 */
+enum { PDLP_L_INF = 0, PDLP_L2 = 1 };
+
+/* step_size_policy_params variant — src/primal_dual_hybrid_gradient_gpu.jl:2-7 */
+enum { PDLP_STEP_ADAPTIVE = 0, PDLP_STEP_CONSTANT = 1 };
+
+/* Arithmetic of the bulk iteration (extension, no reference counterpart). */
+enum {
+  PDLP_PRECISION_FP64 = 0,        /* the reference's arithmetic */
+  PDLP_PRECISION_FP32_REFINE = 1  /* FP32 matrix/iterates + FP64 iterative refinement */
+};
+
+/* Borrowed view of an LP in the reference's host layout
+ * (QuadraticProgrammingProblem, src/quadratic_programming.jl:25-67, with the
+ * objective matrix absent: the reference's GPU path ignores it,
+ * src/cpu_to_gpu.jl:1-14). */
+typedef struct pdlp_lp {
+  int64_t n;            /* number of variables */
+  int64_t m;            /* number of constraint rows */
+  int64_t num_eq;       /* rows [0, num_eq) are equalities */
+  int64_t nnz;          /* colptr[n] - 1 */
+  const int64_t *colptr;  /* n+1, 1-based */
+  const int64_t *rowval;  /* nnz, 1-based row indices, sorted within a column */
+  const double *nzval;    /* nnz */
+  const double *c;        /* objective_vector, n */
+  const double *lb;       /* variable_lower_bound, n (may hold -Inf) */
+  const double *ub;       /* variable_upper_bound, n (may hold +Inf) */
+  const double *b;        /* right_hand_side, m */
+  double c0;              /* objective_constant */
+} pdlp_lp;
+
+/* TerminationCriteria — src/termination.jl:9-77 (same field order). */
+typedef struct pdlp_termination_criteria {
+  int32_t optimality_norm;          /* PDLP_L_INF / PDLP_L2 */
+  int32_t iteration_limit;          /* Int32 in the reference */
+  double eps_optimal_absolute;
+  double eps_optimal_relative;
+  double eps_primal_infeasible;
+  double eps_dual_infeasible;
+  double time_sec_limit;
+  double kkt_matrix_pass_limit;
+} pdlp_termination_criteria;
+
+/* RestartParameters — src/saddle_point_gpu.jl:353-383 (same field order). */
+typedef struct pdlp_restart_params {
+  int32_t restart_scheme;
+  int32_t restart_to_current_metric;
+  int64_t restart_frequency_if_fixed;
+  double artificial_restart_threshold;
+  double sufficient_reduction_for_restart;
+  double necessary_reduction_for_restart;
+  double primal_weight_update_smoothing;
+} pdlp_restart_params;
+
+/* PdhgParameters — src/primal_dual_hybrid_gradient_gpu.jl:9-24, field for
+ * field, followed by extensions that have no reference counterpart. */
+typedef struct pdlp_params {
+  int64_t l_inf_ruiz_iterations;
+  int32_t l2_norm_rescaling;
+  int32_t has_pock_chambolle_alpha;  /* 0 encodes `nothing` */
+  double pock_chambolle_alpha;
+  double primal_importance;
+  int32_t scale_invariant_initial_primal_weight;
+  int32_t record_iteration_stats;
+  int64_t verbosity;
+  int32_t termination_evaluation_frequency;
+  int32_t step_size_policy;          /* PDLP_STEP_ADAPTIVE / PDLP_STEP_CONSTANT */
+  double reduction_exponent;         /* AdaptiveStepsizeParams */
+  double growth_exponent;
+  pdlp_termination_criteria termination_criteria;
+  pdlp_restart_params restart_params;
+  /* ---- extensions ---- */
+  int32_t precision;                 /* PDLP_PRECISION_* */
+  int32_t refine_max_outer;          /* FP32_REFINE: max refinement rounds */
+  double refine_inner_tolerance;     /* FP32_REFINE: tolerance of each FP32 solve */
+  double refine_max_scale;           /* FP32_REFINE: cap of the per-round scale-up */
+  int32_t device_preconditioning;    /* 1: Ruiz/Pock-Chambolle on the GPU */
+  int32_t reserved[7];
+} pdlp_params;
+
+/* ConvergenceInformation — src/solve_log.jl:51-155 (same field order). */
+typedef struct pdlp_convergence_information {
+  int32_t candidate_type;
+  int32_t pad_;
+  double primal_objective;
+  double dual_objective;
+  double corrected_dual_objective;
+  double l_inf_primal_residual;
+  double l2_primal_residual;
+  double l_inf_dual_residual;
+  double l2_dual_residual;
+  double relative_l_inf_primal_residual;
+  double relative_l2_primal_residual;
+  double relative_l_inf_dual_residual;
+  double relative_l2_dual_residual;
+  double relative_optimality_gap;
+  double l_inf_primal_variable;
+  double l2_primal_variable;
+  double l_inf_dual_variable;
+  double l2_dual_variable;
+} pdlp_convergence_information;
+
+/* InfeasibilityInformation — src/solve_log.jl:161-212 (same field order). */
+typedef struct pdlp_infeasibility_information {
+  int32_t candidate_type;
+  int32_t pad_;
+  double max_primal_ray_infeasibility;
+  double primal_ray_linear_objective;
+  double max_dual_ray_infeasibility;
+  double dual_ray_objective;
+} pdlp_infeasibility_information;
+
+/* IterationStats — src/solve_log.jl:219-302.  Exactly one convergence and
+ * one infeasibility record, as the reference produces
+ * (src/iteration_stats_utils_gpu.jl:405-429). */
+typedef struct pdlp_iteration_stats {
+  int32_t iteration_number;
+  int32_t cumulative_rejected_steps;
+  pdlp_convergence_information convergence_information;
+  pdlp_infeasibility_information infeasibility_information;
+  double cumulative_kkt_matrix_passes;
+  double cumulative_time_sec;
+  int32_t restart_used;
+  int32_t pad_;
+  double step_size;
+  double primal_weight;
+  double time_spent_doing_basic_algorithm;  /* method_specific_stats entry */
+  int64_t total_attempts;                   /* extension: step attempts so far */
+} pdlp_iteration_stats;
+
+/* Output of one solve — SaddlePointOutput (src/saddle_point_gpu.jl:3-34)
+ * with caller-allocated vectors.  x (n) and y (m) receive the unscaled
+ * average iterate the reference returns (src/saddle_point_gpu.jl:40-61). */
+typedef struct pdlp_result {
+  double *x;                 /* caller-allocated, n */
+  double *y;                 /* caller-allocated, m */
+  double *x_current;         /* optional (NULL): unscaled current iterate, n */
+  double *y_current;         /* optional (NULL): unscaled current iterate, m */
+  int32_t termination_reason;
+  int32_t iteration_count;
+  pdlp_iteration_stats final_stats;
+  double solve_time_sec;         /* wall time of the whole call */
+  double preprocess_time_sec;    /* validation + rescaling + upload */
+  int64_t total_attempts;        /* total_number_iterations of the reference */
+  int32_t refinement_rounds;     /* FP32_REFINE: outer rounds performed */
+  int32_t pad_;
+} pdlp_result;
+
+/* Receives each recorded IterationStats (replaces Vector{IterationStats}). */
+typedef void (*pdlp_stats_cb)(const pdlp_iteration_stats *stats, void *user);
+
+/* Opaque per-device context: one HIP stream and its device buffers.  One
+ * solve at a time per context; separate contexts may be used from separate
+ * host threads. */
+typedef struct pdlp_ctx pdlp_ctx;
+
+/* Creates a context on HIP device `device`. */
+int pdlp_ctx_create(int device, pdlp_ctx **out);
+void pdlp_ctx_destroy(pdlp_ctx *ctx);
+
+/* Last error message of this context (or of the process if ctx is NULL). */
+const char *pdlp_last_error(const pdlp_ctx *ctx);
+
+/* Fills `p` with the defaults of cuPDLP.Optimizer()
+ * (src/MOI_wrapper.jl:52-80, construct_termination_criteria
+ * src/termination.jl:79-99). */
+void pdlp_default_params(pdlp_params *p);
+
+/* Replaces optimize(params, problem) — src/primal_dual_hybrid_gradient_gpu.jl:457-798. */
+int pdlp_solve(pdlp_ctx *ctx, const pdlp_lp *lp, const pdlp_params *params,
+               pdlp_result *result, pdlp_stats_cb cb, void *user);
+
+/* Solves `count` independent LPs together, packed block-diagonally into the
+ * same kernel launches with per-instance step sizes, restarts and
+ * termination (extension; each instance follows optimize() exactly). */
+int pdlp_batch_solve(pdlp_ctx *ctx, int32_t count, const pdlp_lp *lps,
+                     const pdlp_params *params, pdlp_result *results);
+
+/* Continuous-batching request loop (extension). */
+typedef struct pdlp_server pdlp_server;
+int pdlp_serve_open(pdlp_ctx *ctx, const pdlp_params *params,
+                    double hbm_budget_bytes, int32_t max_slots,
+                    pdlp_server **out);
+/* Copies the LP to the device queue; returns a request id >= 0 in *id. */
+int pdlp_serve_submit(pdlp_server *srv, const pdlp_lp *lp, double tolerance,
+                      int64_t *id);
+/* Advances the device work; returns the number of requests finished since
+ * the last poll in *finished and copies their ids to ids[0..*finished). */
+int pdlp_serve_poll(pdlp_server *srv, int64_t *ids, int32_t max_ids,
+                    int32_t *finished);
+/* Copies the result of a finished request (x: n, y: m caller-allocated). */
+int pdlp_serve_result(pdlp_server *srv, int64_t id, pdlp_result *result);
+int pdlp_serve_close(pdlp_server *srv);
+
+/* Library identification. */
+int pdlp_abi_version(void);
+
+/* Writes sizeof() of pdlp_lp, pdlp_termination_criteria, pdlp_restart_params,
+ * pdlp_params, pdlp_convergence_information, pdlp_infeasibility_information,
+ * pdlp_iteration_stats and pdlp_result (in that order) to out[0..max) and
+ * returns the count, so bindings can check their mirrors (host only). */
+int32_t pdlp_abi_layout(int64_t *out, int32_t max);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PDLP_HIP_H */

@@ -1,0 +1,56 @@
+"""Small LPs with hand-derived answers, shared by the oracle and GPU tests."""
+import numpy as np
+import scipy.sparse as sp
+
+from cupdlp import QuadraticProgrammingProblem
+
+INF = np.inf
+
+
+def lp(A, b, c, lb, ub, neq, c0=0.0):
+    return QuadraticProgrammingProblem(np.asarray(lb, float), np.asarray(ub, float), None,
+                                       np.asarray(c, float), c0, sp.csc_matrix(np.asarray(A, float)),
+                                       np.asarray(b, float), neq)
+
+
+def tiny_optimal():
+    """min -x  s.t. -x >= -1, 0 <= x: optimum -1 at x = 1, y = 1."""
+    return lp([[-1.0]], [-1.0], [-1.0], [0.0], [INF], 0), -1.0
+
+
+def tiny_two_var():
+    """min x1 + 2 x2 s.t. x1 + x2 = 2, x1 - x2 >= -1, x >= 0  -> x = (2, 0), obj 2."""
+    return lp([[1, 1], [1, -1]], [2, -1], [1, 2], [0, 0], [INF, INF], 1), 2.0
+
+
+def primal_infeasible():
+    """x >= 1 (row) with x <= 0 (bound)."""
+    return lp([[1.0]], [1.0], [1.0], [-INF], [0.0], 0)
+
+
+def dual_infeasible():
+    """min -x s.t. x >= 0 (row), x >= 0: unbounded."""
+    return lp([[1.0]], [0.0], [-1.0], [0.0], [INF], 0)
+
+
+def no_rows():
+    """m = 0: min x1 - x2 with 0 <= x1 <= 1, -1 <= x2 <= 2 -> x = (0, 2), obj -2."""
+    A = sp.csc_matrix((0, 2))
+    return QuadraticProgrammingProblem(np.array([0.0, -1.0]), np.array([1.0, 2.0]), None,
+                                       np.array([1.0, -1.0]), 0.0, A, np.zeros(0), 0), -2.0
+
+
+def long_row_lp(n=6000, seed=7):
+    """A random LP with one dense row of n entries (longer than a row block)
+    plus sparse rows; certified by construction."""
+    from cupdlp.generators import random_lp
+    prob, cert = random_lp(n=n, m=n // 4, num_eq=n // 8, nnz_per_col=3, seed=seed)
+    A = prob.constraint_matrix.tolil()
+    A[0, :] = np.random.default_rng(seed).standard_normal(n)
+    A = sp.csc_matrix(A)
+    b = prob.right_hand_side.copy()
+    b[0] = A[0, :] @ cert.x  # equality row 0 stays tight; its y* stays valid
+    c = A.T @ cert.y + cert.r
+    p = QuadraticProgrammingProblem(prob.variable_lower_bound, prob.variable_upper_bound, None,
+                                    c, 0.0, A, b, prob.num_equalities)
+    return p, float(c @ cert.x)

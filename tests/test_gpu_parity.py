@@ -1,0 +1,188 @@
+"""GPU parity: libpdlp_hip.so (through the C ABI) against the CPU oracle.
+
+Tolerances (FP64 path): per-evaluation step size, primal weight, objectives
+and residual norms agree to 1e-9 relative over the first 1,000 iterations,
+restart decisions and termination reasons are identical, and the returned
+iterates agree to 1e-9 relative to their norm.  The device and the oracle sum
+SpMV rows and reductions in different orders, so agreement is to rounding,
+not bitwise.
+"""
+import math
+import sys
+
+import numpy as np
+import pytest
+
+import lp_cases
+import oracle as O
+from cupdlp import PdhgParameters, TerminationReason as TR, construct_termination_criteria
+from cupdlp.generators import config_lp, random_lp, transportation_lp
+
+pytestmark = pytest.mark.gpu
+
+REL = 1e-9
+
+
+def params(tol=1e-4, iteration_limit=2**31 - 1, **kw):
+    tc = construct_termination_criteria(eps_optimal_absolute=tol, eps_optimal_relative=tol,
+                                        iteration_limit=iteration_limit)
+    return PdhgParameters(termination_criteria=tc, **kw)
+
+
+def rel(a, b):
+    return abs(a - b) / max(abs(a), abs(b), 1e-300)
+
+
+def vec_rel(a, b):
+    na = max(np.linalg.norm(a), np.linalg.norm(b), 1e-300)
+    return np.linalg.norm(a - b) / na
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def compare_runs(g, o, rel_tol=REL, vec_tol=REL, check_all_stats=True):
+    assert g.termination_reason == o.termination_reason, (g.termination_reason, o.termination_reason)
+    assert g.iteration_count == o.iteration_count
+    assert g.total_attempts == o.total_attempts
+    if check_all_stats:
+        assert len(g.iteration_stats) == len(o.iteration_stats)
+        for sg, so in zip(g.iteration_stats, o.iteration_stats):
+            assert sg.iteration_number == so.iteration_number
+            assert sg.restart_used == so.restart_used, (sg.iteration_number, sg.restart_used, so.restart_used)
+            assert rel(sg.step_size, so.step_size) <= rel_tol, (sg.iteration_number, sg.step_size, so.step_size)
+            assert rel(sg.primal_weight, so.primal_weight) <= rel_tol
+            cg, co = sg.convergence_information[0], so.convergence_information[0]
+            scale = abs(co.primal_objective) + abs(co.dual_objective) + 1.0
+            assert abs(cg.primal_objective - co.primal_objective) <= rel_tol * scale
+            assert abs(cg.dual_objective - co.dual_objective) <= rel_tol * scale
+            for f in ("l2_primal_residual", "l2_dual_residual", "l_inf_primal_residual",
+                      "l_inf_dual_residual", "l2_primal_variable", "l2_dual_variable"):
+                a, b = getattr(cg, f), getattr(co, f)
+                assert abs(a - b) <= 1e-7 * max(abs(b), 1e-12) + 1e-13 * scale, (f, a, b)
+    assert vec_rel(g.primal_solution, o.primal_solution) <= vec_tol
+    assert vec_rel(g.dual_solution, o.dual_solution) <= vec_tol
+
+
+def test_c0_first_1000_iterations(gpu_ctx):
+    say("case: n=2000 LP seed 0, iteration_limit=1000, FP64")
+    prob, cert = config_lp(0)
+    p = params(tol=1e-12, iteration_limit=1000)
+    g = gpu_ctx.solve(prob, p, want_current=True)
+    o = O.solve(prob, p, want_current=True)
+    assert g.termination_reason == TR.TERMINATION_REASON_ITERATION_LIMIT
+    compare_runs(g, o)
+    assert vec_rel(g.primal_current, o.primal_current) <= REL
+    assert vec_rel(g.dual_current, o.dual_current) <= REL
+
+
+@pytest.mark.parametrize("tol", [1e-4, 1e-8])
+def test_c0_to_tolerance(gpu_ctx, tol):
+    say(f"case: n=2000 LP seed 0 to {tol}")
+    prob, cert = config_lp(0)
+    p = params(tol=tol)
+    g = gpu_ctx.solve(prob, p)
+    o = O.solve(prob, p)
+    compare_runs(g, o, rel_tol=1e-7, vec_tol=1e-7)
+    assert g.termination_reason == TR.TERMINATION_REASON_OPTIMAL
+    ci = g.iteration_stats[-1].convergence_information[0]
+    # reference optimality test (src/termination.jl:142-170) holds on the output
+    assert abs(ci.primal_objective - cert.objective) <= 50 * tol * (1 + abs(cert.objective))
+
+
+def test_closed_forms(gpu_ctx):
+    say("case: closed-form tiny LPs")
+    prob, opt = lp_cases.tiny_optimal()
+    g = gpu_ctx.solve(prob, params(1e-8))
+    assert g.termination_reason == TR.TERMINATION_REASON_OPTIMAL
+    assert abs(g.primal_solution[0] - 1.0) < 1e-6
+    assert abs(g.iteration_stats[-1].convergence_information[0].primal_objective - opt) < 1e-6
+    prob, opt = lp_cases.tiny_two_var()
+    g = gpu_ctx.solve(prob, params(1e-8))
+    o = O.solve(prob, params(1e-8))
+    compare_runs(g, o)
+    assert abs(g.iteration_stats[-1].convergence_information[0].primal_objective - opt) < 1e-6
+
+
+def test_infeasible_and_unbounded(gpu_ctx):
+    say("case: primal infeasible / dual infeasible")
+    g = gpu_ctx.solve(lp_cases.primal_infeasible(), params(1e-6))
+    o = O.solve(lp_cases.primal_infeasible(), params(1e-6))
+    assert g.termination_reason == o.termination_reason == TR.TERMINATION_REASON_PRIMAL_INFEASIBLE
+    assert g.iteration_count == o.iteration_count >= 9  # detection allowed from iteration 10
+    g = gpu_ctx.solve(lp_cases.dual_infeasible(), params(1e-6))
+    o = O.solve(lp_cases.dual_infeasible(), params(1e-6))
+    assert g.termination_reason == o.termination_reason == TR.TERMINATION_REASON_DUAL_INFEASIBLE
+    assert g.iteration_count == o.iteration_count
+
+
+def test_no_rows(gpu_ctx):
+    say("case: m = 0")
+    prob, opt = lp_cases.no_rows()
+    g = gpu_ctx.solve(prob, params(1e-8))
+    o = O.solve(prob, params(1e-8))
+    assert g.termination_reason == o.termination_reason == TR.TERMINATION_REASON_OPTIMAL
+    np.testing.assert_allclose(g.primal_solution, [0.0, 2.0], atol=1e-6)
+    assert abs(g.iteration_stats[-1].convergence_information[0].primal_objective - opt) < 1e-6
+
+
+def test_iteration_limit_semantics(gpu_ctx):
+    say("case: iteration_limit = 37")
+    prob, _ = config_lp(0, scale=0.25)
+    g = gpu_ctx.solve(prob, params(1e-12, iteration_limit=37))
+    o = O.solve(prob, params(1e-12, iteration_limit=37))
+    assert g.termination_reason == TR.TERMINATION_REASON_ITERATION_LIMIT
+    assert g.iteration_count == o.iteration_count == 37
+    compare_runs(g, o)
+
+
+def test_long_rows(gpu_ctx):
+    say("case: LP with a 6000-entry row (chunked long-row path)")
+    prob, opt = lp_cases.long_row_lp(6000)
+    p = params(1e-12, iteration_limit=500)
+    g = gpu_ctx.solve(prob, p)
+    o = O.solve(prob, p)
+    compare_runs(g, o, rel_tol=1e-8, vec_tol=1e-8)
+
+
+def test_transportation_small(gpu_ctx):
+    say("case: 60 x 60 transportation LP to 1e-6")
+    prob = transportation_lp(60, 60, seed=5)
+    p = params(1e-6)
+    g = gpu_ctx.solve(prob, p)
+    o = O.solve(prob, p)
+    compare_runs(g, o, rel_tol=1e-7, vec_tol=1e-7)
+
+
+def test_scaled_zipf_lp(gpu_ctx):
+    say("case: n=20000 Zipf-degree scaled LP, 640 iterations")
+    prob, cert = config_lp(1, scale=0.02)
+    p = params(1e-12, iteration_limit=640)
+    g = gpu_ctx.solve(prob, p)
+    o = O.solve(prob, p)
+    compare_runs(g, o, rel_tol=1e-8, vec_tol=1e-8)
+
+
+def test_batch_alone_vs_mixed_bitwise(gpu_ctx):
+    say("case: batch of 4 vs alone")
+    probs = [random_lp(n=n, m=n // 2, seed=s)[0] for n, s in ((300, 11), (1200, 12), (5000, 13), (700, 14))]
+    p = params(1e-6)
+    alone = [gpu_ctx.solve(q, p) for q in probs]
+    mixed = gpu_ctx.batch_solve(probs, p)
+    for a, b in zip(alone, mixed):
+        assert a.termination_reason == b.termination_reason
+        assert a.iteration_count == b.iteration_count
+        assert np.array_equal(a.primal_solution, b.primal_solution)
+        assert np.array_equal(a.dual_solution, b.dual_solution)
+
+
+def test_invalid_input_raises(gpu_ctx):
+    say("case: NaN objective")
+    from cupdlp.solver import PdlpError
+    prob, _ = lp_cases.tiny_optimal()
+    prob.objective_vector[0] = math.nan
+    with pytest.raises(PdlpError) as ei:
+        gpu_ctx.solve(prob, params())
+    assert ei.value.code > 0
