@@ -1,0 +1,227 @@
+#!/usr/bin/env python3
+"""Benchmark of the MI355X restarted-PDHG engine (one GPU).
+
+Default workload (the batched configuration of the headline metric): 512 LPs
+from the random-LP generator (seeds 0..511, n log-uniform in [500, 200,000],
+m = n/2, 8 nonzeros per column), solved together by the batched engine to
+1e-8 relative KKT (the reference's own optimality test, eps_abs = eps_rel).
+A "step" is one solve of the whole batch; `value` is solves/s over the K
+timed steps.  Inputs are generated and uploaded outside the timed region,
+per the reference's warm-up convention (scripts/solve.jl:158-161); the timed
+region covers rescaling, transfer, the solves and the result download.
+
+Reported beside it:
+  * plain FP64 one-at-a-time solves/s on the same GPU (a bounded prefix of the
+    batch), and the speed-up over it;
+  * p50/p99 latency of the instances within a batch step;
+  * `roofline`: the dominant iteration kernel's algorithmic bytes per launch
+    over its HIP-event duration (events on the library's own stream);
+  * `cpu_baseline`: the CPU oracle (same restarted PDHG, OpenMP) on a bounded
+    sample of the same LPs on this host's cores.
+
+Other workloads: --workload c1 (n = 1,000,000 LP, fixed iteration count, for
+the FP64 SpMV roofline), --workload c0 (n = 2,000 LP).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(REPO, "cupdlp.jl_amd"))
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def make_params(tol, precision, iteration_limit=2**31 - 1, time_limit=float("inf")):
+    from cupdlp import PdhgParameters, Precision, construct_termination_criteria
+    tc = construct_termination_criteria(eps_optimal_absolute=tol, eps_optimal_relative=tol,
+                                        iteration_limit=iteration_limit, time_sec_limit=time_limit)
+    return PdhgParameters(termination_criteria=tc, record_iteration_stats=False,
+                          precision=Precision.FP32_REFINE if precision == "fp32" else Precision.FP64)
+
+
+def batch_problems(count):
+    from cupdlp.generators import batch_sizes, random_lp
+    sizes = batch_sizes(count)
+    return [random_lp(n=int(n), m=int(n) // 2, seed=i)[0] for i, n in enumerate(sizes)]
+
+
+def profile_roofline(prof, kernel_names, note):
+    pm, dm = prof.primal_ms, prof.dual_ms
+    if prof.primal_launches == 0:
+        return None
+    if dm >= pm:
+        name, ms, nbytes, launches = kernel_names[1], dm, prof.dual_bytes, prof.dual_launches
+    else:
+        name, ms, nbytes, launches = kernel_names[0], pm, prof.primal_bytes, prof.primal_launches
+    avg_ms = ms / launches
+    bytes_per_launch = nbytes / launches
+    achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
+    return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": name,
+            "avg_launch_us": round(avg_ms * 1e3, 2), "bytes_per_launch": bytes_per_launch,
+            "launches": launches, "note": note,
+            "other_kernel": {"primal_avg_us": round(pm / max(prof.primal_launches, 1) * 1e3, 2),
+                             "dual_avg_us": round(dm / max(prof.dual_launches, 1) * 1e3, 2),
+                             "primal_GBs": round(prof.primal_bytes / max(pm, 1e-12) / 1e6, 1),
+                             "dual_GBs": round(prof.dual_bytes / max(dm, 1e-12) / 1e6, 1)}}
+
+
+def cpu_baseline_batch(problems, tol, budget_s=20.0):
+    """The CPU oracle on a bounded sample: instances in batch order until the
+    time budget is spent."""
+    sys.path.insert(0, os.path.join(REPO, "oracle"))
+    import oracle as O
+    O.build()
+    threads = O.lib().orc_num_threads()
+    p = make_params(tol, "fp64")
+    t0 = time.time()
+    done = 0
+    for prob in problems:
+        O.solve(prob, p)
+        done += 1
+        if time.time() - t0 > budget_s:
+            break
+    dt = time.time() - t0
+    return {"value": round(done / dt, 4), "unit": "solves/s", "cores": threads, "kind": "port",
+            "sample": f"first {done} of the {len(problems)} batch LPs, one at a time, "
+                      f"tol {tol:g}, {dt:.1f} s"}
+
+
+def run_batch(args, ctx, lib):
+    import ctypes as C
+    from cupdlp import _abi
+    log(f"generating {args.count} LPs ...")
+    t = time.time()
+    problems = batch_problems(args.count)
+    nnz = sum(p.constraint_matrix.nnz for p in problems)
+    log(f"  {time.time() - t:.1f} s, total nnz {nnz}, total n {sum(p.num_variables for p in problems)}")
+    params = make_params(args.tol, args.precision)
+    for w in range(args.warmup):
+        log(f"warmup {w}")
+        ctx.batch_solve(problems, params)
+    lib.pdlp_get_profile(ctx.handle, C.byref(_abi.pdlp_kernel_profile()))
+    lib.pdlp_set_profiling(ctx.handle, 1)
+    lat, iters, bad = [], [], 0
+    t0 = time.time()
+    for k in range(args.steps):
+        outs = ctx.batch_solve(problems, params)
+        for o in outs:
+            lat.append(o.solve_time_sec)
+            iters.append(o.iteration_count)
+            bad += int(o.termination_string != "OPTIMAL")
+        log(f"step {k}: {time.time() - t0:.2f} s cumulative")
+    dt = time.time() - t0
+    lib.pdlp_set_profiling(ctx.handle, 0)
+    prof = _abi.pdlp_kernel_profile()
+    lib.pdlp_get_profile(ctx.handle, C.byref(prof))
+    value = args.steps * len(problems) / dt
+    # plain FP64 one at a time on a prefix of the batch
+    p64 = make_params(args.tol, "fp64")
+    t1 = time.time()
+    nser = 0
+    for prob in problems:
+        ctx.solve(prob, p64)
+        nser += 1
+        if time.time() - t1 > args.serial_budget:
+            break
+    ser = nser / (time.time() - t1)
+    extra = {
+        "p50_latency_s": round(float(np.percentile(lat, 50)), 4),
+        "p99_latency_s": round(float(np.percentile(lat, 99)), 4),
+        "not_optimal": bad,
+        "median_iterations": int(np.median(iters)),
+        "fp64_serial_solves_per_s": round(ser, 3),
+        "fp64_serial_sample": nser,
+        "speedup_vs_fp64_serial": round(value / ser, 3),
+    }
+    roof = profile_roofline(prof, ("k_primal", "k_dual"),
+                            f"{args.count}-LP batch, {args.precision}, tol {args.tol:g}")
+    cfg = {"workload": f"batch of {args.count} random LPs (seeds 0..{args.count - 1}, n log-uniform "
+                       f"500..200000, m=n/2, 8 nnz/col)",
+           "tolerance": args.tol, "precision": args.precision, "total_nnz": nnz}
+    return value, dt, cfg, extra, roof, problems
+
+
+def run_single(args, ctx, lib, which):
+    import ctypes as C
+    from cupdlp import _abi
+    from cupdlp.generators import config_lp
+    log(f"generating config {which} ...")
+    prob, cert = config_lp(which)
+    params = make_params(args.tol, args.precision, iteration_limit=args.iters)
+    for w in range(args.warmup):
+        ctx.solve(prob, params)
+    lib.pdlp_set_profiling(ctx.handle, 1)
+    t0 = time.time()
+    its = 0
+    for k in range(args.steps):
+        out = ctx.solve(prob, params)
+        its += out.iteration_count
+    dt = time.time() - t0
+    lib.pdlp_set_profiling(ctx.handle, 0)
+    prof = _abi.pdlp_kernel_profile()
+    lib.pdlp_get_profile(ctx.handle, C.byref(prof))
+    roof = profile_roofline(prof, ("k_primal", "k_dual"), f"config {which}, {args.precision}")
+    n, m, nnz = prob.num_variables, prob.num_constraints, prob.constraint_matrix.nnz
+    cfg = {"workload": f"single LP n={n} m={m} nnz={nnz}", "iteration_limit": args.iters,
+           "precision": args.precision}
+    extra = {"iterations_per_s": round(its / dt, 1), "iterations": its}
+    return args.steps / dt, dt, cfg, extra, roof, [prob]
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", choices=("batch", "c1", "c0"), default="batch")
+    ap.add_argument("--precision", choices=("fp64", "fp32"), default="fp64")
+    ap.add_argument("--tol", type=float, default=1e-8)
+    ap.add_argument("--count", type=int, default=512)
+    ap.add_argument("--iters", type=int, default=1000)
+    ap.add_argument("--serial-budget", type=float, default=20.0)
+    ap.add_argument("--cpu-budget", type=float, default=20.0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
+              "(the engine runs on a single device)", file=sys.stderr)
+        sys.exit(2)
+    from cupdlp.solver import Context
+    ctx = Context(0)
+    lib = ctx.lib
+    if args.workload == "batch":
+        value, dt, cfg, extra, roof, problems = run_batch(args, ctx, lib)
+        metric_unit = "solves/s"
+    else:
+        value, dt, cfg, extra, roof, problems = run_single(args, ctx, lib, 1 if args.workload == "c1" else 0)
+        metric_unit = "solves/s"
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline_batch(problems, args.tol, args.cpu_budget)
+    with open(os.path.join(REPO, "BASELINE.json")) as f:
+        metric = json.load(f)["metric"]
+    line = {
+        "metric": metric, "value": round(value, 4), "unit": metric_unit, "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(dt / args.steps * 1e3, 2), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64" if args.precision == "fp64" else "f32+f64",
+        "data": "synthetic (seeded generators with certified optima)", "config": cfg,
+        "roofline": roof, "cpu_baseline": cpu,
+    }
+    line.update(extra)
+    ctx.close()
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

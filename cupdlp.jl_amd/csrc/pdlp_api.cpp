@@ -20,6 +20,7 @@ struct pdlp_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
   DevicePool pool;
+  Profiler prof;
   std::string err;
 };
 
@@ -121,6 +122,7 @@ void pdlp_ctx_destroy(pdlp_ctx *ctx) {
   if (!ctx) return;
   (void)hipSetDevice(ctx->device);
   (void)hipStreamSynchronize(ctx->stream);
+  for (hipEvent_t ev : ctx->prof.events) (void)hipEventDestroy(ev);
   ctx->pool.release_all();
   (void)hipStreamDestroy(ctx->stream);
   delete ctx;
@@ -130,6 +132,19 @@ const char *pdlp_last_error(const pdlp_ctx *ctx) {
   if (ctx) return ctx->err.c_str();
   std::lock_guard<std::mutex> lk(g_err_mu);
   return g_err.c_str();
+}
+
+int pdlp_set_profiling(pdlp_ctx *ctx, int32_t enable) {
+  if (!ctx) return 1;
+  ctx->prof.enabled = enable != 0;
+  return 0;
+}
+
+int pdlp_get_profile(pdlp_ctx *ctx, pdlp_kernel_profile *out) {
+  if (!ctx || !out) return 1;
+  *out = ctx->prof.acc;
+  memset(&ctx->prof.acc, 0, sizeof ctx->prof.acc);
+  return 0;
 }
 
 }  // extern "C"
@@ -194,9 +209,10 @@ extern "C" int pdlp_solve(pdlp_ctx *ctx, const pdlp_lp *lp, const pdlp_params *p
   rc = prepare_instance(lp, params, h.get(), &ctx->err);
   if (rc) return rc;
   h->res = result;
+  h->t_submit = t0;
   h->cb = cb;
   h->user = user;
-  Engine eng(ctx->stream, &ctx->pool, *params);
+  Engine eng(ctx->stream, &ctx->pool, *params, &ctx->prof);
   rc = eng.add(std::move(h), &ctx->err);
   if (rc) return rc;
   result->preprocess_time_sec = now_sec() - t0;
@@ -219,7 +235,7 @@ extern "C" int pdlp_batch_solve(pdlp_ctx *ctx, int32_t count, const pdlp_lp *lps
   int rc = check_params(params, &ctx->err);
   if (rc) return rc;
   (void)hipSetDevice(ctx->device);
-  Engine eng(ctx->stream, &ctx->pool, *params);
+  Engine eng(ctx->stream, &ctx->pool, *params, &ctx->prof);
   for (int32_t i = 0; i < count; ++i) {
     if ((lps[i].n > 0 && !results[i].x) || (lps[i].m > 0 && !results[i].y)) {
       ctx->err = "result vectors must be allocated by the caller (instance " + std::to_string(i) + ")";
@@ -232,17 +248,14 @@ extern "C" int pdlp_batch_solve(pdlp_ctx *ctx, int32_t count, const pdlp_lp *lps
       return rc;
     }
     h->res = &results[i];
+    h->t_submit = t0;
     rc = eng.add(std::move(h), &ctx->err);
     if (rc) return rc;
   }
   const double t1 = now_sec();
   rc = eng.run(&ctx->err);
   if (rc) return rc;
-  const double t2 = now_sec();
-  for (int32_t i = 0; i < count; ++i) {
-    results[i].preprocess_time_sec = t1 - t0;
-    results[i].solve_time_sec = t2 - t0;
-  }
+  for (int32_t i = 0; i < count; ++i) results[i].preprocess_time_sec = t1 - t0;
   return 0;
 }
 
@@ -302,6 +315,7 @@ extern "C" int pdlp_serve_submit(pdlp_server *s, const pdlp_lp *lp, double toler
   int rc = pdlp::prepare_instance(lp, &s->prm, h.get(), &s->ctx->err);
   if (rc) return rc;
   h->tolerance_override = tolerance;
+  h->t_submit = now_sec();
   ServeReq r;
   r.id = s->next_id++;
   r.tol = tolerance;

@@ -26,7 +26,8 @@ constexpr int kBlockRows = 1024; // rows per row block
 // Evaluation accumulators (see eval kernel): n-side and m-side.
 constexpr int kEvalN = 25;
 constexpr int kEvalM = 18;
-constexpr int kEvalOut = kEvalN + kEvalM;
+constexpr int kEvalOut = kEvalN + kEvalM;  // index of the primal ray objective
+constexpr int kEvalStride = kEvalOut + 1;   // doubles per instance in the output
 
 struct BlockDesc {
   int32_t inst;   // instance index

@@ -9,6 +9,7 @@
 #include <algorithm>
 #include <cstdio>
 
+#include "pdlp_hostsum.h"
 #include "pdlp_kernels.h"
 
 namespace pdlp {
@@ -89,8 +90,8 @@ void DevicePool::release_all() {
 }
 
 // ---------------------------------------------------------------- engine
-Engine::Engine(hipStream_t s, DevicePool *pool, const pdlp_params &prm)
-    : s_(s), pool_(pool), prm_(prm) {
+Engine::Engine(hipStream_t s, DevicePool *pool, const pdlp_params &prm, Profiler *prof)
+    : s_(s), pool_(pool), prof_(prof) , prm_(prm) {
   prec_ = prm.precision == PDLP_PRECISION_FP32_REFINE ? 1 : 0;
 }
 
@@ -211,17 +212,7 @@ int Engine::add(std::unique_ptr<HostInst> h, std::string *err) {
   double omega = prm_.primal_importance;
   if (prm_.scale_invariant_initial_primal_weight) {
     // select_initial_primal_weight — src/saddle_point_gpu.jl:767-785
-    auto nrm = [](const std::vector<double> &v) {
-      double s = 0.0;
-      for (size_t k = 0; k < v.size(); k += 4096) {
-        double t = 0.0;
-        const size_t e = std::min(v.size(), k + 4096);
-        for (size_t i = k; i < e; ++i) t += v[i] * v[i];
-        s += t;
-      }
-      return sqrt(s);
-    };
-    const double rhs = sqrt(1.0) * nrm(P.b), obj = sqrt(1.0) * nrm(P.c);
+    const double rhs = sqrt(1.0) * dd_norm2(P.b), obj = sqrt(1.0) * dd_norm2(P.c);
     if (obj > 0.0 && rhs > 0.0) omega = prm_.primal_importance * (obj / rhs);
   }
   c.omega = omega;
@@ -298,16 +289,18 @@ int Engine::rebuild_tables(std::string *err) {
   d_dblocks_ = (BlockDesc *)alloc(sizeof(BlockDesc) * db.size());
   d_longs_ = (LongRow *)alloc(sizeof(LongRow) * longs.size());
   d_long_cnt_ = (int32_t *)alloc(sizeof(int32_t) * longs.size());
-  d_long_part_ = (double *)alloc(sizeof(double) * nslots);
+  d_long_part_ = (double *)alloc(2 * sizeof(double) * nslots);
   d_insts_ = (InstDev *)alloc(sizeof(InstDev) * ninst);
   d_ctl_ = (Ctl *)alloc(sizeof(Ctl) * ninst);
-  d_ppart_ = (double *)alloc(sizeof(double) * pb.size());
-  d_dpart_ = (double *)alloc(sizeof(double) * 2 * db.size());
-  d_epn_ = (double *)alloc(sizeof(double) * kEvalN * pb.size());
-  d_epm_ = (double *)alloc(sizeof(double) * kEvalM * db.size());
-  d_out_ = (double *)alloc(sizeof(double) * kEvalOut * ninst);
+  // compensated partial sums: (s, e) pairs
+  d_ppart_ = (double *)alloc(sizeof(double) * 2 * pb.size());
+  d_dpart_ = (double *)alloc(sizeof(double) * 4 * db.size());
+  d_epn_ = (double *)alloc(sizeof(double) * 2 * kEvalN * pb.size());
+  d_epm_ = (double *)alloc(sizeof(double) * 2 * kEvalM * db.size());
+  d_epr_ = (double *)alloc(sizeof(double) * 2 * pb.size());
+  d_out_ = (double *)alloc(sizeof(double) * kEvalStride * ninst);
   if (!d_pblocks_ || !d_dblocks_ || !d_longs_ || !d_long_cnt_ || !d_long_part_ || !d_insts_ ||
-      !d_ctl_ || !d_ppart_ || !d_dpart_ || !d_epn_ || !d_epm_ || !d_out_) {
+      !d_ctl_ || !d_ppart_ || !d_dpart_ || !d_epn_ || !d_epm_ || !d_epr_ || !d_out_) {
     if (err) *err = "device allocation of block tables failed";
     return -2;
   }
@@ -328,7 +321,7 @@ int Engine::rebuild_tables(std::string *err) {
   if (out_cap_ < ninst) {
     if (out_pinned_) (void)hipHostFree(out_pinned_);
     out_pinned_ = nullptr;
-    HIPCHK(hipHostMalloc((void **)&out_pinned_, sizeof(double) * kEvalOut * std::max<size_t>(ninst, 1), 0));
+    HIPCHK(hipHostMalloc((void **)&out_pinned_, sizeof(double) * kEvalStride * std::max<size_t>(ninst, 1), 0));
     out_cap_ = ninst;
   }
   memcpy(ctl_pinned_, ctl_.data(), sizeof(Ctl) * ninst);
@@ -370,16 +363,41 @@ int Engine::run_attempts(int64_t count, std::string *err) {
   for (size_t q = 0; q < inst_.size(); ++q) max_att = std::max(max_att, ctl_[q].attempts);
   int rc = ensure_step_table(max_att + count + 2, err);
   if (rc) return rc;
+  const bool prof = prof_ && prof_->enabled;
+  // Algorithmic bytes per launch of each instance (FP64: s_A = s_v = 8):
+  // N(s_A+4) + 4(rows+1) + s_v*gathered + 9*s_v*rows.
+  std::vector<double> pbytes, dbytes;
+  std::vector<int64_t> remaining;
+  if (prof) {
+    const double sv = prec_ ? 4.0 : 8.0;
+    const size_t need = 4 * (size_t)count;
+    while (prof_->events.size() < need) {
+      hipEvent_t ev;
+      HIPCHK(hipEventCreate(&ev));
+      prof_->events.push_back(ev);
+    }
+    for (size_t q = 0; q < inst_.size(); ++q) {
+      const Prepared &P = inst_[q]->P;
+      const double N = (double)P.nnz, n = (double)P.n, m = (double)P.m;
+      pbytes.push_back(N * (sv + 4) + 4 * (n + 1) + sv * m + 9 * sv * n);
+      dbytes.push_back(N * (sv + 4) + 4 * (m + 1) + sv * n + 9 * sv * m);
+      remaining.push_back(inst_[q]->done ? 0 : ctl_[q].target - ctl_[q].accepted);
+    }
+  }
   const double t0 = now_sec();
   for (int64_t a = 0; a < count; ++a) {
+    if (prof) HIPCHK(hipEventRecord(prof_->events[4 * a + 0], s_));
     int e = launch_primal(s_, prec_, false, np_, d_pblocks_, d_longs_, d_long_cnt_, d_long_part_,
                           d_insts_, d_ctl_, d_ppart_);
+    if (prof) HIPCHK(hipEventRecord(prof_->events[4 * a + 1], s_));
     if (!e)
       e = launch_dual(s_, prec_, false, nd_, d_dblocks_, d_longs_, d_long_cnt_, d_long_part_,
                       d_insts_, d_ctl_, d_dpart_);
+    if (prof) HIPCHK(hipEventRecord(prof_->events[4 * a + 2], s_));
     if (!e)
       e = launch_step(s_, (int)inst_.size(), d_insts_, d_ctl_, d_ppart_, d_dpart_, d_red_, d_grow_,
                       tab_len_, prm_.reduction_exponent, prm_.growth_exponent);
+    if (prof) HIPCHK(hipEventRecord(prof_->events[4 * a + 3], s_));
     if (e) {
       if (err) *err = std::string("kernel launch failed: ") + hipGetErrorString((hipError_t)e);
       return -e;
@@ -390,6 +408,25 @@ int Engine::run_attempts(int64_t count, std::string *err) {
   HIPCHK(hipStreamSynchronize(s_));
   memcpy(ctl_.data(), ctl_pinned_, sizeof(Ctl) * ninst);
   const double dt = now_sec() - t0;
+  if (prof) {
+    for (int64_t a = 0; a < count; ++a) {
+      float ms_p = 0.f, ms_d = 0.f;
+      HIPCHK(hipEventElapsedTime(&ms_p, prof_->events[4 * a + 0], prof_->events[4 * a + 1]));
+      HIPCHK(hipEventElapsedTime(&ms_d, prof_->events[4 * a + 1], prof_->events[4 * a + 2]));
+      double bp = 0.0, bd = 0.0;
+      for (size_t q = 0; q < ninst; ++q)
+        if (a < remaining[q]) {
+          bp += pbytes[q];
+          bd += dbytes[q];
+        }
+      prof_->acc.primal_launches += 1;
+      prof_->acc.dual_launches += 1;
+      prof_->acc.primal_ms += ms_p;
+      prof_->acc.dual_ms += ms_d;
+      prof_->acc.primal_bytes += bp;
+      prof_->acc.dual_bytes += bd;
+    }
+  }
   for (size_t q = 0; q < ninst; ++q) {
     HostInst &h = *inst_[q];
     if (h.done) continue;
@@ -489,7 +526,7 @@ void Engine::host_eval_logic(size_t q, const double *o) {
     const double s = N[3];
     const double bmax = jl_max(M[7], N[11]);
     ii.max_primal_ray_infeasibility = s != 0.0 ? bmax / s : bmax;
-    ii.primal_ray_linear_objective = s != 0.0 ? N[2] / s : N[2];
+    ii.primal_ray_linear_objective = s != 0.0 ? o[kEvalOut] : N[2];
     const double dri2 = jl_max(M[3], N[8]);
     const double dobj2 = (M[4] + c0) + N[9];
     const double scaling = jl_max(M[5], N[10]);
@@ -617,16 +654,16 @@ int Engine::evaluate(std::string *err) {
                     d_insts_, d_ctl_, d_dpart_);
   if (!e)
     e = launch_eval(s_, prec_, np_, d_pblocks_, nd_, d_dblocks_, d_insts_, d_ctl_, d_epn_, d_epm_,
-                    (int)ninst, d_out_);
+                    d_epr_, (int)ninst, d_out_);
   if (e) {
     if (err) *err = std::string("evaluation launch failed: ") + hipGetErrorString((hipError_t)e);
     return -e;
   }
-  HIPCHK(hipMemcpyAsync(out_pinned_, d_out_, sizeof(double) * kEvalOut * ninst,
+  HIPCHK(hipMemcpyAsync(out_pinned_, d_out_, sizeof(double) * kEvalStride * ninst,
                         hipMemcpyDeviceToHost, s_));
   HIPCHK(hipStreamSynchronize(s_));
   for (size_t q = 0; q < ninst; ++q)
-    if (ctl_[q].eval_flag) host_eval_logic(q, out_pinned_ + q * kEvalOut);
+    if (ctl_[q].eval_flag) host_eval_logic(q, out_pinned_ + q * kEvalStride);
   memcpy(ctl_pinned_, ctl_.data(), sizeof(Ctl) * ninst);
   HIPCHK(hipMemcpyAsync(d_ctl_, ctl_pinned_, sizeof(Ctl) * ninst, hipMemcpyHostToDevice, s_));
   e = launch_restart(s_, prec_, np_, d_pblocks_, nd_, d_dblocks_, d_insts_, d_ctl_);
@@ -673,6 +710,7 @@ int Engine::finish_instance(size_t q, std::string *err) {
       for (size_t i = 0; i < m; ++i) r->y_current[i] = val(by, i) / h.P.E[i];
   }
   r->termination_reason = h.reason;
+  r->solve_time_sec = now_sec() - h.t_submit;
   r->iteration_count = (int32_t)(h.eval_iter - 1);
   r->final_stats = h.last_stats;
   r->total_attempts = C.attempts;

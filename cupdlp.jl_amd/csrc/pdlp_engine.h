@@ -60,13 +60,21 @@ struct HostInst {
   pdlp_result *res = nullptr;
   pdlp_stats_cb cb = nullptr;
   void *user = nullptr;
+  double t_submit = 0.0;             // wall time the request entered the library
   double tolerance_override = -1.0;  // serving: per-request tolerance
   int64_t tag = -1;                  // serving: request id
 };
 
+// Event-based timing of the iteration kernels (see pdlp_kernel_profile).
+struct Profiler {
+  bool enabled = false;
+  pdlp_kernel_profile acc{};
+  std::vector<hipEvent_t> events;  // reusable pool
+};
+
 class Engine {
  public:
-  Engine(hipStream_t s, DevicePool *pool, const pdlp_params &prm);
+  Engine(hipStream_t s, DevicePool *pool, const pdlp_params &prm, Profiler *prof = nullptr);
   ~Engine();
 
   // Adds an instance (already prepared/rescaled on the host).
@@ -91,6 +99,7 @@ class Engine {
 
   hipStream_t s_;
   DevicePool *pool_;
+  Profiler *prof_ = nullptr;
   pdlp_params prm_;
   int prec_ = 0;
   std::vector<std::unique_ptr<HostInst>> inst_;
@@ -108,7 +117,7 @@ class Engine {
   InstDev *d_insts_ = nullptr;
   Ctl *d_ctl_ = nullptr;
   double *d_ppart_ = nullptr, *d_dpart_ = nullptr, *d_epn_ = nullptr, *d_epm_ = nullptr,
-         *d_out_ = nullptr;
+         *d_epr_ = nullptr, *d_out_ = nullptr;
   double *d_red_ = nullptr, *d_grow_ = nullptr;
   int64_t tab_len_ = 0;
   std::vector<void *> table_allocs_;

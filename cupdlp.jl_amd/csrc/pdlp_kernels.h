@@ -20,7 +20,7 @@ int launch_step(hipStream_t s, int ninst, const InstDev *insts, Ctl *ctl, const 
                 int64_t tab_len, double red_exp, double grow_exp);
 int launch_eval(hipStream_t s, int prec, int np, const BlockDesc *pblocks, int nd,
                 const BlockDesc *dblocks, const InstDev *insts, const Ctl *ctl, double *epn,
-                double *epm, int ninst, double *out);
+                double *epm, double *epr, int ninst, double *out);
 int launch_restart(hipStream_t s, int prec, int np, const BlockDesc *pblocks, int nd,
                    const BlockDesc *dblocks, const InstDev *insts, const Ctl *ctl);
 

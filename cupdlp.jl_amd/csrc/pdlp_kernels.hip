@@ -14,10 +14,14 @@
 // Candidates are written to the non-current buffers, so a rejected attempt
 // costs nothing to undo and the host needs no synchronisation per attempt.
 //
-// Every reduction is deterministic: per-thread sums in a fixed stride order,
-// wave butterflies read from lane 0, waves combined in order, and per-block
-// partials combined in block order, so results do not depend on timing or on
-// which other instances share the launch.
+// Every sum (SpMV rows and all reductions) in the FP64 path is a compensated
+// sum of error-free products (TwoProd with FMA, TwoSum), so the rounded
+// result is the FP64 value nearest the exact sum whatever the summation order.
+// It is also deterministic: per-thread sums in a fixed stride order, wave
+// butterflies read from lane 0, waves and blocks combined in order.  Results
+// therefore match a CPU evaluation of the same formulas bit for bit (except
+// for exact sums lying within ~1e-30 relative of a rounding midpoint) and do
+// not depend on which other instances share the launch.
 //
 // Floating-point contraction is disabled for this file (-ffp-contract=off) so
 // each elementwise formula rounds exactly as the reference's written form.
@@ -45,31 +49,68 @@ __device__ __forceinline__ double jl_min(double a, double b) {
   return __builtin_signbit(a) ? a : b;
 }
 
-__device__ __forceinline__ double wave_sum(double v) {
+// Compensated accumulator: value s + e, with e collecting the rounding errors.
+struct DD {
+  double s, e;
+};
+__device__ __forceinline__ DD dd_zero() { return DD{0.0, 0.0}; }
+// s += v, error into e (TwoSum)
+__device__ __forceinline__ void dd_add(DD &a, double v) {
+  const double s = a.s + v;
+  const double bb = s - a.s;
+  const double t = (a.s - (s - bb)) + (v - bb);
+  a.s = s;
+  a.e += t;
+}
+// s += x*y exactly (TwoProd via FMA, then TwoSum)
+__device__ __forceinline__ void dd_add_prod(DD &a, double x, double y) {
+  const double p = x * y;
+  const double q = __builtin_fma(x, y, -p);
+  dd_add(a, p);
+  a.e += q;
+}
+__device__ __forceinline__ void dd_merge(DD &a, const DD &b) {
+  const double s = a.s + b.s;
+  const double bb = s - a.s;
+  const double t = (a.s - (s - bb)) + (b.s - bb);
+  a.s = s;
+  a.e = (a.e + b.e) + t;
+}
+__device__ __forceinline__ double dd_val(const DD &a) { return a.s + a.e; }
+
+__device__ __forceinline__ DD wave_dd(DD v) {
 #pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  for (int o = 32; o > 0; o >>= 1) {
+    DD w{__shfl_xor(v.s, o, 64), __shfl_xor(v.e, o, 64)};
+    dd_merge(v, w);
+  }
   return v;
 }
+
+// Compensated sum over the work-group; valid in thread 0.  `scratch` holds
+// 2*kThreads/64 doubles.  Contains barriers.
+__device__ __forceinline__ DD block_dd(DD v, double *scratch) {
+  v = wave_dd(v);
+  const int w = threadIdx.x >> 6;
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) {
+    scratch[2 * w] = v.s;
+    scratch[2 * w + 1] = v.e;
+  }
+  __syncthreads();
+  DD r = dd_zero();
+  if (threadIdx.x == 0) {
+    r = DD{scratch[0], scratch[1]};
+#pragma unroll
+    for (int k = 1; k < kThreads / 64; ++k) dd_merge(r, DD{scratch[2 * k], scratch[2 * k + 1]});
+  }
+  return r;
+}
+
 __device__ __forceinline__ double wave_max(double v) {
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) v = jl_max(v, __shfl_xor(v, o, 64));
   return v;
-}
-
-// Sum over the work-group; the result is valid in thread 0.  `scratch`
-// holds kThreads/64 doubles.  Contains barriers: call from uniform control flow.
-__device__ __forceinline__ double block_sum(double v, double *scratch) {
-  v = wave_sum(v);
-  const int w = threadIdx.x >> 6;
-  __syncthreads();
-  if ((threadIdx.x & 63) == 0) scratch[w] = v;
-  __syncthreads();
-  double s = 0.0;
-  if (threadIdx.x == 0) {
-#pragma unroll
-    for (int k = 0; k < kThreads / 64; ++k) s += scratch[k];
-  }
-  return s;
 }
 
 template <typename T>
@@ -81,22 +122,28 @@ __device__ __forceinline__ void st(void *p, int i, double v) {
   ((T *)p)[i] = (T)v;
 }
 
-// Products of one row block staged in LDS, then per-row sums by groups of
-// `tpr` lanes (a power of two, at most a wave).  epi(row, sum) runs on the
-// first lane of each group.  With do_spmv false no matrix entry is read and
-// epi receives 0.
+// Products of one row block staged in LDS as exact (hi, lo) pairs, then
+// per-row compensated sums by groups of `tpr` lanes (a power of two, at most a
+// wave).  epi(row, sum) runs on the first lane of each group.  With do_spmv
+// false no matrix entry is read and epi receives 0.
 template <typename T, typename V, typename Epi>
 __device__ __forceinline__ void rowblock_spmv(const int32_t *__restrict__ ptr,
                                               const int32_t *__restrict__ col,
                                               const T *__restrict__ val,
                                               const V *__restrict__ xv, int row0, int nrows,
-                                              double *lds, bool do_spmv, Epi &&epi) {
+                                              double *lds_p, double *lds_q, bool do_spmv,
+                                              Epi &&epi) {
   const int k0 = ptr[row0];
   int tpr = 1;
   if (do_spmv) {
     const int nz = ptr[row0 + nrows] - k0;
-    for (int k = threadIdx.x; k < nz; k += kThreads)
-      lds[k] = (double)val[k0 + k] * (double)xv[col[k0 + k]];
+    for (int k = threadIdx.x; k < nz; k += kThreads) {
+      const double a = (double)val[k0 + k];
+      const double x = (double)xv[col[k0 + k]];
+      const double pr = a * x;
+      lds_p[k] = pr;
+      lds_q[k] = __builtin_fma(a, x, -pr);
+    }
     __syncthreads();
     while (tpr < 64 && tpr * 2 * nrows <= kThreads) tpr <<= 1;
   }
@@ -104,50 +151,66 @@ __device__ __forceinline__ void rowblock_spmv(const int32_t *__restrict__ ptr,
   const int grp = threadIdx.x / tpr;
   const int ngrp = kThreads / tpr;
   for (int r = grp; r < nrows; r += ngrp) {
-    double s = 0.0;
+    DD acc = dd_zero();
     if (do_spmv) {
       const int rs = ptr[row0 + r] - k0, re = ptr[row0 + r + 1] - k0;
-      for (int k = rs + lane; k < re; k += tpr) s += lds[k];
-      for (int o = tpr >> 1; o > 0; o >>= 1) s += __shfl_xor(s, o, tpr);
+      for (int k = rs + lane; k < re; k += tpr) {
+        dd_add(acc, lds_p[k]);
+        acc.e += lds_q[k];
+      }
+      for (int o = tpr >> 1; o > 0; o >>= 1) {
+        const DD w{__shfl_xor(acc.s, o, tpr), __shfl_xor(acc.e, o, tpr)};
+        dd_merge(acc, w);
+      }
     }
-    if (lane == 0) epi(row0 + r, s);
+    if (lane == 0) epi(row0 + r, dd_val(acc));
   }
 }
 
-// Partial sum of one kBlockNnz chunk of a long row; valid in thread 0.
+// Compensated partial sum of one kBlockNnz chunk of a long row; valid in thread 0.
 template <typename T, typename V>
-__device__ __forceinline__ double longrow_chunk(const int32_t *__restrict__ ptr,
-                                                const int32_t *__restrict__ col,
-                                                const T *__restrict__ val,
-                                                const V *__restrict__ xv, int row, int chunk,
-                                                double *scratch) {
+__device__ __forceinline__ DD longrow_chunk(const int32_t *__restrict__ ptr,
+                                            const int32_t *__restrict__ col,
+                                            const T *__restrict__ val,
+                                            const V *__restrict__ xv, int row, int chunk,
+                                            double *scratch) {
   const int kb = ptr[row] + chunk * kBlockNnz;
   const int ke = min(kb + kBlockNnz, ptr[row + 1]);
-  double s = 0.0;
-  for (int k = kb + (int)threadIdx.x; k < ke; k += kThreads) s += (double)val[k] * (double)xv[col[k]];
-  return block_sum(s, scratch);
+  DD acc = dd_zero();
+  for (int k = kb + (int)threadIdx.x; k < ke; k += kThreads)
+    dd_add_prod(acc, (double)val[k], (double)xv[col[k]]);
+  return block_dd(acc, scratch);
 }
 
 // Thread 0 of a chunk block publishes its partial; returns true in the last
 // chunk to finish, with *total the chunk sums combined in chunk order.
 __device__ __forceinline__ bool longrow_publish(const LongRow lr, int chunk, int lid,
-                                                int32_t *long_cnt, double *long_part,
-                                                double s, double *total) {
-  __hip_atomic_store(&long_part[lr.slot + chunk], s, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                                                int32_t *long_cnt, double *long_part, DD s,
+                                                double *total) {
+  __hip_atomic_store(&long_part[2 * (lr.slot + chunk)], s.s, __ATOMIC_RELAXED,
+                     __HIP_MEMORY_SCOPE_AGENT);
+  __hip_atomic_store(&long_part[2 * (lr.slot + chunk) + 1], s.e, __ATOMIC_RELAXED,
+                     __HIP_MEMORY_SCOPE_AGENT);
   const int prev = __hip_atomic_fetch_add(&long_cnt[lid], 1, __ATOMIC_ACQ_REL,
                                           __HIP_MEMORY_SCOPE_AGENT);
   if (prev != lr.nchunks - 1) return false;
-  double t = 0.0;
-  for (int c = 0; c < lr.nchunks; ++c)
-    t += __hip_atomic_load(&long_part[lr.slot + c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  DD t = dd_zero();
+  for (int c = 0; c < lr.nchunks; ++c) {
+    const DD v{__hip_atomic_load(&long_part[2 * (lr.slot + c)], __ATOMIC_RELAXED,
+                                 __HIP_MEMORY_SCOPE_AGENT),
+               __hip_atomic_load(&long_part[2 * (lr.slot + c) + 1], __ATOMIC_RELAXED,
+                                 __HIP_MEMORY_SCOPE_AGENT)};
+    dd_merge(t, v);
+  }
   __hip_atomic_store(&long_cnt[lid], 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  *total = t;
+  *total = dd_val(t);
   return true;
 }
 
 // ---------------------------------------------------------------------------
 // Primal half-step over the rows of A^T.
 // FLUSH: only the pending weighted sums and A^T y (evaluation preparation).
+// part[2b..2b+1]: compensated sum of dx^2 of block b.
 template <typename T, bool FLUSH>
 __global__ __launch_bounds__(kThreads) void k_primal(const BlockDesc *__restrict__ blocks,
                                                      const LongRow *__restrict__ longs,
@@ -156,8 +219,9 @@ __global__ __launch_bounds__(kThreads) void k_primal(const BlockDesc *__restrict
                                                      const InstDev *__restrict__ insts,
                                                      const Ctl *__restrict__ ctl,
                                                      double *__restrict__ part) {
-  __shared__ double lds[kBlockNnz];
-  __shared__ double scratch[kThreads / 64];
+  __shared__ double lds_p[kBlockNnz];
+  __shared__ double lds_q[kBlockNnz];
+  __shared__ double scratch[2 * kThreads / 64];
   const BlockDesc bd = blocks[blockIdx.x];
   const Ctl *C = &ctl[bd.inst];
   const bool run = FLUSH ? (C->eval_flag != 0) : (C->active != 0);
@@ -178,7 +242,7 @@ __global__ __launch_bounds__(kThreads) void k_primal(const BlockDesc *__restrict
   const T *__restrict__ cv = (const T *)I.c;
   const T *__restrict__ lv = (const T *)I.l;
   const T *__restrict__ uv = (const T *)I.u;
-  double acc = 0.0;
+  DD acc = dd_zero();
   auto epi = [&](int j, double s) {
     const double a = need ? s : (double)aty[j];
     const double xj = (double)xv[j];
@@ -193,32 +257,36 @@ __global__ __launch_bounds__(kThreads) void k_primal(const BlockDesc *__restrict
       const double d = t - xj;
       dx[j] = (T)d;
       xn[j] = (T)(xj + d);
-      acc += d * d;
+      dd_add_prod(acc, d, d);
     }
   };
   if (bd.nrows > 0) {
-    rowblock_spmv(I.t_ptr, I.t_col, (const T *)I.t_val, yv, bd.row0, bd.nrows, lds, need, epi);
+    rowblock_spmv(I.t_ptr, I.t_col, (const T *)I.t_val, yv, bd.row0, bd.nrows, lds_p, lds_q, need,
+                  epi);
   } else {
     const int chunk = -bd.nrows - 1;
     if (!need) {
       if (chunk == 0 && threadIdx.x == 0) epi(bd.row0, 0.0);
     } else {
-      const double s = longrow_chunk(I.t_ptr, I.t_col, (const T *)I.t_val, yv, bd.row0, chunk,
-                                     scratch);
+      const DD sm = longrow_chunk(I.t_ptr, I.t_col, (const T *)I.t_val, yv, bd.row0, chunk, scratch);
       if (threadIdx.x == 0) {
         double tot;
-        if (longrow_publish(longs[bd.aux], chunk, bd.aux, long_cnt, long_part, s, &tot))
+        if (longrow_publish(longs[bd.aux], chunk, bd.aux, long_cnt, long_part, sm, &tot))
           epi(bd.row0, tot);
       }
     }
   }
   if (!FLUSH) {
-    const double t = block_sum(acc, scratch);
-    if (threadIdx.x == 0) part[blockIdx.x] = t;
+    const DD t = block_dd(acc, scratch);
+    if (threadIdx.x == 0) {
+      part[2 * blockIdx.x] = t.s;
+      part[2 * blockIdx.x + 1] = t.e;
+    }
   }
 }
 
 // Dual half-step over the rows of A.  FLUSH: only the pending weighted sums.
+// part[4b..4b+3]: compensated sums of dy^2 and Adx*dy of block b.
 template <typename T, bool FLUSH>
 __global__ __launch_bounds__(kThreads) void k_dual(const BlockDesc *__restrict__ blocks,
                                                    const LongRow *__restrict__ longs,
@@ -227,8 +295,9 @@ __global__ __launch_bounds__(kThreads) void k_dual(const BlockDesc *__restrict__
                                                    const InstDev *__restrict__ insts,
                                                    const Ctl *__restrict__ ctl,
                                                    double *__restrict__ part) {
-  __shared__ double lds[kBlockNnz];
-  __shared__ double scratch[kThreads / 64];
+  __shared__ double lds_p[kBlockNnz];
+  __shared__ double lds_q[kBlockNnz];
+  __shared__ double scratch[2 * kThreads / 64];
   const BlockDesc bd = blocks[blockIdx.x];
   const Ctl *C = &ctl[bd.inst];
   const bool run = FLUSH ? (C->eval_flag != 0) : (C->active != 0);
@@ -247,7 +316,7 @@ __global__ __launch_bounds__(kThreads) void k_dual(const BlockDesc *__restrict__
   T *__restrict__ sax = (T *)I.sax;
   const T *__restrict__ bv = (const T *)I.b;
   const T *__restrict__ dxv = (const T *)I.dx;
-  double acc_dy = 0.0, acc_int = 0.0;
+  DD acc_dy = dd_zero(), acc_int = dd_zero();
   auto epi = [&](int i, double adx) {
     const double yi = (double)yv[i];
     const double axi = (double)axv[i];
@@ -261,32 +330,34 @@ __global__ __launch_bounds__(kThreads) void k_dual(const BlockDesc *__restrict__
       const double d = t - yi;
       yn[i] = (T)(yi + d);
       axn[i] = (T)(axi + adx);
-      acc_dy += d * d;
-      acc_int += adx * d;
+      dd_add_prod(acc_dy, d, d);
+      dd_add_prod(acc_int, adx, d);
     }
   };
   if (bd.nrows > 0) {
-    rowblock_spmv(I.a_ptr, I.a_col, (const T *)I.a_val, dxv, bd.row0, bd.nrows, lds, !FLUSH, epi);
+    rowblock_spmv(I.a_ptr, I.a_col, (const T *)I.a_val, dxv, bd.row0, bd.nrows, lds_p, lds_q,
+                  !FLUSH, epi);
   } else {
     const int chunk = -bd.nrows - 1;
     if (FLUSH) {
       if (chunk == 0 && threadIdx.x == 0) epi(bd.row0, 0.0);
     } else {
-      const double s = longrow_chunk(I.a_ptr, I.a_col, (const T *)I.a_val, dxv, bd.row0, chunk,
-                                     scratch);
+      const DD sm = longrow_chunk(I.a_ptr, I.a_col, (const T *)I.a_val, dxv, bd.row0, chunk, scratch);
       if (threadIdx.x == 0) {
         double tot;
-        if (longrow_publish(longs[bd.aux], chunk, bd.aux, long_cnt, long_part, s, &tot))
+        if (longrow_publish(longs[bd.aux], chunk, bd.aux, long_cnt, long_part, sm, &tot))
           epi(bd.row0, tot);
       }
     }
   }
   if (!FLUSH) {
-    const double a = block_sum(acc_dy, scratch);
-    const double b = block_sum(acc_int, scratch);
+    const DD a = block_dd(acc_dy, scratch);
+    const DD b = block_dd(acc_int, scratch);
     if (threadIdx.x == 0) {
-      part[2 * blockIdx.x] = a;
-      part[2 * blockIdx.x + 1] = b;
+      part[4 * blockIdx.x] = a.s;
+      part[4 * blockIdx.x + 1] = a.e;
+      part[4 * blockIdx.x + 2] = b.s;
+      part[4 * blockIdx.x + 3] = b.e;
     }
   }
 }
@@ -301,29 +372,30 @@ __global__ __launch_bounds__(kThreads) void k_step(const InstDev *__restrict__ i
                                                    const double *__restrict__ grow_tab,
                                                    int64_t tab_len, double red_exp,
                                                    double grow_exp) {
-  __shared__ double scratch[kThreads / 64];
+  __shared__ double scratch[2 * kThreads / 64];
   const int q = blockIdx.x;
   Ctl *C = &ctl[q];
   const int active = C->active;
   if (!active) return;
   const InstDev &I = insts[q];
-  double sx = 0.0, sy = 0.0, sxy = 0.0;
-  for (int b = I.pb0 + (int)threadIdx.x; b < I.pb1; b += kThreads) sx += ppart[b];
+  DD sx = dd_zero(), sy = dd_zero(), sxy = dd_zero();
+  for (int b = I.pb0 + (int)threadIdx.x; b < I.pb1; b += kThreads)
+    dd_merge(sx, DD{ppart[2 * b], ppart[2 * b + 1]});
   for (int b = I.db0 + (int)threadIdx.x; b < I.db1; b += kThreads) {
-    sy += dpart[2 * b];
-    sxy += dpart[2 * b + 1];
+    dd_merge(sy, DD{dpart[4 * b], dpart[4 * b + 1]});
+    dd_merge(sxy, DD{dpart[4 * b + 2], dpart[4 * b + 3]});
   }
-  sx = block_sum(sx, scratch);
-  sy = block_sum(sy, scratch);
-  sxy = block_sum(sxy, scratch);
+  sx = block_dd(sx, scratch);
+  sy = block_dd(sy, scratch);
+  sxy = block_dd(sxy, scratch);
   if (threadIdx.x != 0) return;
   const double omega = C->omega;
   double eta = C->eta;
   const int64_t k = C->attempts + 1;
   C->attempts = k;
   C->kkt_passes += 1.0;
-  const double interaction = fabs(sxy);
-  const double nx = sqrt(sx), ny = sqrt(sy);
+  const double interaction = fabs(dd_val(sxy));
+  const double nx = sqrt(dd_val(sx)), ny = sqrt(dd_val(sy));
   const double movement = 0.5 * omega * (nx * nx) + (0.5 / omega) * (ny * ny);
   C->last_interaction = interaction;
   C->last_movement = movement;
@@ -372,6 +444,8 @@ __global__ __launch_bounds__(kThreads) void k_step(const InstDev *__restrict__ i
 // candidates of the restart scheme (src/saddle_point_gpu.jl:250-278, 479-622)
 // and the restart distances (:673-682), all in one pass.  Blocks [0, np) walk
 // the A^T row blocks (variables), the rest walk the A row blocks (rows).
+// Each accumulator is a compensated sum or an exact maximum; partials hold
+// (s, e) pairs per block.
 
 __device__ __forceinline__ bool eval_is_max_n(int k) {
   return k == 1 || k == 3 || k == 6 || k == 8 || k == 10 || k == 11;
@@ -385,22 +459,37 @@ __device__ __forceinline__ double contrib(double l, double u, double r) {
 }
 
 template <int K, bool M>
-__device__ __forceinline__ void block_reduce_acc(double (&acc)[K], double *lds, double *out) {
+__device__ __forceinline__ void block_reduce_acc(DD (&acc)[K], double *lds, double *out) {
   const int w = threadIdx.x >> 6;
+  constexpr int NW = kThreads / 64;
 #pragma unroll
   for (int k = 0; k < K; ++k) {
     const bool ismax = M ? eval_is_max_m(k) : eval_is_max_n(k);
-    const double v = ismax ? wave_max(acc[k]) : wave_sum(acc[k]);
-    if ((threadIdx.x & 63) == 0) lds[k * (kThreads / 64) + w] = v;
+    DD v;
+    if (ismax) {
+      v = DD{wave_max(acc[k].s), 0.0};
+    } else {
+      v = wave_dd(acc[k]);
+    }
+    if ((threadIdx.x & 63) == 0) {
+      lds[2 * (k * NW + w)] = v.s;
+      lds[2 * (k * NW + w) + 1] = v.e;
+    }
   }
   __syncthreads();
   if (threadIdx.x < K) {
     const int k = threadIdx.x;
     const bool ismax = M ? eval_is_max_m(k) : eval_is_max_n(k);
-    double s = lds[k * (kThreads / 64)];
-    for (int q = 1; q < kThreads / 64; ++q)
-      s = ismax ? jl_max(s, lds[k * (kThreads / 64) + q]) : s + lds[k * (kThreads / 64) + q];
-    out[k] = s;
+    DD r{lds[2 * (k * NW)], lds[2 * (k * NW) + 1]};
+    for (int q = 1; q < NW; ++q) {
+      const DD v{lds[2 * (k * NW + q)], lds[2 * (k * NW + q) + 1]};
+      if (ismax)
+        r.s = jl_max(r.s, v.s);
+      else
+        dd_merge(r, v);
+    }
+    out[2 * k] = r.s;
+    out[2 * k + 1] = r.e;
   }
 }
 
@@ -423,7 +512,7 @@ __global__ __launch_bounds__(kThreads) void k_eval(const BlockDesc *__restrict__
                                                    const Ctl *__restrict__ ctl,
                                                    double *__restrict__ epn,
                                                    double *__restrict__ epm) {
-  __shared__ double lds[kEvalN * (kThreads / 64)];
+  __shared__ double lds[2 * kEvalN * (kThreads / 64)];
   const bool nside = (int)blockIdx.x < np;
   const int blk = nside ? (int)blockIdx.x : (int)blockIdx.x - np;
   const BlockDesc bd = nside ? pblocks[blk] : dblocks[blk];
@@ -436,9 +525,9 @@ __global__ __launch_bounds__(kThreads) void k_eval(const BlockDesc *__restrict__
   int r0, r1;
   block_rows(bd, &r0, &r1);
   if (nside) {
-    double acc[kEvalN];
+    DD acc[kEvalN];
 #pragma unroll
-    for (int k = 0; k < kEvalN; ++k) acc[k] = 0.0;
+    for (int k = 0; k < kEvalN; ++k) acc[k] = dd_zero();
     const void *xcur = cur ? I.x1 : I.x0;
     for (int j = r0 + (int)threadIdx.x; j < r1; j += kThreads) {
       const double xc = ld<T>(xcur, j);
@@ -455,25 +544,25 @@ __global__ __launch_bounds__(kThreads) void k_eval(const BlockDesc *__restrict__
       const double ol = I.ol[j], ou = I.ou[j], oc = I.oc[j];
       const double fl = isfinite(ol) ? 1.0 : 0.0, fu = isfinite(ou) ? 1.0 : 0.0;
       const double lvv = jl_max(ol - ox, 0.0), uvv = jl_max(ox - ou, 0.0);
-      acc[0] += lvv * lvv;
-      acc[0] += uvv * uvv;
-      acc[1] = jl_max(acc[1], jl_max(lvv, uvv));
-      acc[2] += oc * ox;
-      acc[3] = jl_max(acc[3], fabs(ox));
-      acc[4] += ox * ox;
+      dd_add_prod(acc[0], lvv, lvv);
+      dd_add_prod(acc[0], uvv, uvv);
+      acc[1].s = jl_max(acc[1].s, jl_max(lvv, uvv));
+      dd_add_prod(acc[2], oc, ox);
+      acc[3].s = jl_max(acc[3].s, fabs(ox));
+      dd_add_prod(acc[4], ox, ox);
       const double r = jl_max(og, 0.0) * fl + jl_min(og, 0.0) * fu;
       const double rv = og - r;
-      acc[5] += rv * rv;
-      acc[6] = jl_max(acc[6], fabs(rv));
-      acc[7] += contrib(ol, ou, r);
+      dd_add_prod(acc[5], rv, rv);
+      acc[6].s = jl_max(acc[6].s, fabs(rv));
+      dd_add(acc[7], contrib(ol, ou, r));
       const double g2 = og - oc;
       const double r2 = jl_max(g2, 0.0) * fl + jl_min(g2, 0.0) * fu;
       const double rv2 = g2 - r2;
-      acc[8] = jl_max(acc[8], fabs(rv2));
-      acc[9] += contrib(ol, ou, r2);
-      acc[10] = jl_max(acc[10], fabs(r2));
-      acc[11] = jl_max(acc[11], jl_max(fl != 0.0 ? jl_max(-ox, 0.0) : 0.0,
-                                        fu != 0.0 ? jl_max(ox, 0.0) : 0.0));
+      acc[8].s = jl_max(acc[8].s, fabs(rv2));
+      dd_add(acc[9], contrib(ol, ou, r2));
+      acc[10].s = jl_max(acc[10].s, fabs(r2));
+      acc[11].s = jl_max(acc[11].s, jl_max(fl != 0.0 ? jl_max(-ox, 0.0) : 0.0,
+                                            fu != 0.0 ? jl_max(ox, 0.0) : 0.0));
       // restart candidates in the scaled space: current, average, last restart
       const double xs3[3] = {xc, ax, ld<T>(I.lx, j)};
       const double gs3[3] = {gnew, ag, ld<T>(I.lg, j)};
@@ -482,22 +571,22 @@ __global__ __launch_bounds__(kThreads) void k_eval(const BlockDesc *__restrict__
       for (int q = 0; q < 3; ++q) {
         const double xs = xs3[q], gs = gs3[q];
         const double a = jl_max(ls - xs, 0.0), b = jl_max(xs - us, 0.0);
-        acc[12 + 4 * q] += a * a;
-        acc[12 + 4 * q] += b * b;
-        acc[13 + 4 * q] += cs * xs;
+        dd_add_prod(acc[12 + 4 * q], a, a);
+        dd_add_prod(acc[12 + 4 * q], b, b);
+        dd_add_prod(acc[13 + 4 * q], cs, xs);
         const double rr = jl_max(gs, 0.0) * sfl + jl_min(gs, 0.0) * sfu;
         const double rvv = gs - rr;
-        acc[14 + 4 * q] += rvv * rvv;
-        acc[15 + 4 * q] += contrib(ls, us, rr);
+        dd_add_prod(acc[14 + 4 * q], rvv, rvv);
+        dd_add(acc[15 + 4 * q], contrib(ls, us, rr));
       }
       const double dxl = ax - xs3[2];
-      acc[24] += dxl * dxl;
+      dd_add_prod(acc[24], dxl, dxl);
     }
-    block_reduce_acc<kEvalN, false>(acc, lds, epn + (size_t)blk * kEvalN);
+    block_reduce_acc<kEvalN, false>(acc, lds, epn + (size_t)blk * 2 * kEvalN);
   } else {
-    double acc[kEvalM];
+    DD acc[kEvalM];
 #pragma unroll
-    for (int k = 0; k < kEvalM; ++k) acc[k] = 0.0;
+    for (int k = 0; k < kEvalM; ++k) acc[k] = dd_zero();
     const void *ycur = cur ? I.y1 : I.y0;
     const void *axcur = cur ? I.ax1 : I.ax0;
     const int neq = I.neq;
@@ -512,17 +601,17 @@ __global__ __launch_bounds__(kThreads) void k_eval(const BlockDesc *__restrict__
       const double ob = I.ob[i];
       const double t = ob - oax;
       const double cvv = i < neq ? t : jl_max(t, 0.0);
-      acc[0] += cvv * cvv;
-      acc[1] = jl_max(acc[1], fabs(cvv));
+      dd_add_prod(acc[0], cvv, cvv);
+      acc[1].s = jl_max(acc[1].s, fabs(cvv));
       if (i >= neq) {
         const double dr = jl_max(-oy, 0.0);
-        acc[2] += dr * dr;
-        acc[3] = jl_max(acc[3], dr);
+        dd_add_prod(acc[2], dr, dr);
+        acc[3].s = jl_max(acc[3].s, dr);
       }
-      acc[4] += ob * oy;
-      acc[5] = jl_max(acc[5], fabs(oy));
-      acc[6] += oy * oy;
-      acc[7] = jl_max(acc[7], i < neq ? fabs(oax) : jl_max(-oax, 0.0));
+      dd_add_prod(acc[4], ob, oy);
+      acc[5].s = jl_max(acc[5].s, fabs(oy));
+      dd_add_prod(acc[6], oy, oy);
+      acc[7].s = jl_max(acc[7].s, i < neq ? fabs(oax) : jl_max(-oax, 0.0));
       const double bs = ld<T>(I.b, i);
       const double ys3[3] = {yc, ay, ld<T>(I.ly, i)};
       const double as3[3] = {axc, aax, ld<T>(I.lax, i)};
@@ -530,17 +619,17 @@ __global__ __launch_bounds__(kThreads) void k_eval(const BlockDesc *__restrict__
       for (int q = 0; q < 3; ++q) {
         const double tt = bs - as3[q];
         const double cc = i < neq ? tt : jl_max(tt, 0.0);
-        acc[8 + 3 * q] += cc * cc;
+        dd_add_prod(acc[8 + 3 * q], cc, cc);
         if (i >= neq) {
           const double dr = jl_max(-ys3[q], 0.0);
-          acc[9 + 3 * q] += dr * dr;
+          dd_add_prod(acc[9 + 3 * q], dr, dr);
         }
-        acc[10 + 3 * q] += bs * ys3[q];
+        dd_add_prod(acc[10 + 3 * q], bs, ys3[q]);
       }
       const double dyl = ay - ys3[2];
-      acc[17] += dyl * dyl;
+      dd_add_prod(acc[17], dyl, dyl);
     }
-    block_reduce_acc<kEvalM, true>(acc, lds, epm + (size_t)blk * kEvalM);
+    block_reduce_acc<kEvalM, true>(acc, lds, epm + (size_t)blk * 2 * kEvalM);
   }
 }
 
@@ -551,7 +640,7 @@ __global__ __launch_bounds__(kThreads) void k_eval_finish(const InstDev *__restr
                                                           const double *__restrict__ epn,
                                                           const double *__restrict__ epm,
                                                           double *__restrict__ out) {
-  __shared__ double scratch[kThreads / 64];
+  __shared__ double scratch[2 * kThreads / 64];
   const int q = blockIdx.x;
   const int k = blockIdx.y;
   if (!ctl[q].eval_flag) return;
@@ -562,20 +651,70 @@ __global__ __launch_bounds__(kThreads) void k_eval_finish(const InstDev *__restr
   const bool ismax = nside ? eval_is_max_n(kk) : eval_is_max_m(kk);
   const double *ep = nside ? epn : epm;
   const int b0 = nside ? I.pb0 : I.db0, b1 = nside ? I.pb1 : I.db1;
-  double s = 0.0;
-  for (int b = b0 + (int)threadIdx.x; b < b1; b += kThreads) {
-    const double v = ep[(size_t)b * K + kk];
-    s = ismax ? jl_max(s, v) : s + v;
+  if (ismax) {
+    double s = 0.0;
+    for (int b = b0 + (int)threadIdx.x; b < b1; b += kThreads)
+      s = jl_max(s, ep[((size_t)b * K + kk) * 2]);
+    s = wave_max(s);
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) scratch[w] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double t = scratch[0];
+      for (int i = 1; i < kThreads / 64; ++i) t = jl_max(t, scratch[i]);
+      out[(size_t)q * kEvalStride + k] = t;
+    }
+  } else {
+    DD s = dd_zero();
+    for (int b = b0 + (int)threadIdx.x; b < b1; b += kThreads)
+      dd_merge(s, DD{ep[((size_t)b * K + kk) * 2], ep[((size_t)b * K + kk) * 2 + 1]});
+    s = block_dd(s, scratch);
+    if (threadIdx.x == 0) out[(size_t)q * kEvalStride + k] = dd_val(s);
   }
-  s = ismax ? wave_max(s) : wave_sum(s);
-  const int w = threadIdx.x >> 6;
-  if ((threadIdx.x & 63) == 0) scratch[w] = s;
-  __syncthreads();
+}
+
+// Second pass for the primal ray objective dot(c, x / norm(x, Inf))
+// (src/iteration_stats_utils_gpu.jl:326-343), which needs the norm first.
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_eval_ray(const BlockDesc *__restrict__ pblocks,
+                                                       const InstDev *__restrict__ insts,
+                                                       const Ctl *__restrict__ ctl,
+                                                       const double *__restrict__ out,
+                                                       double *__restrict__ epr) {
+  __shared__ double scratch[2 * kThreads / 64];
+  const BlockDesc bd = pblocks[blockIdx.x];
+  if (!ctl[bd.inst].eval_flag) return;
+  const InstDev &I = insts[bd.inst];
+  const double sinf = out[(size_t)bd.inst * kEvalStride + 3];
+  int r0, r1;
+  block_rows(bd, &r0, &r1);
+  DD acc = dd_zero();
+  if (sinf != 0.0) {
+    for (int j = r0 + (int)threadIdx.x; j < r1; j += kThreads) {
+      const double ox = ld<T>(I.avg_x, j) / I.D[j];
+      dd_add_prod(acc, I.oc[j], ox / sinf);
+    }
+  }
+  acc = block_dd(acc, scratch);
   if (threadIdx.x == 0) {
-    double t = scratch[0];
-    for (int i = 1; i < kThreads / 64; ++i) t = ismax ? jl_max(t, scratch[i]) : t + scratch[i];
-    out[(size_t)q * kEvalOut + k] = t;
+    epr[2 * blockIdx.x] = acc.s;
+    epr[2 * blockIdx.x + 1] = acc.e;
   }
+}
+
+__global__ __launch_bounds__(kThreads) void k_eval_ray_finish(const InstDev *__restrict__ insts,
+                                                              const Ctl *__restrict__ ctl,
+                                                              const double *__restrict__ epr,
+                                                              double *__restrict__ out) {
+  __shared__ double scratch[2 * kThreads / 64];
+  const int q = blockIdx.x;
+  if (!ctl[q].eval_flag) return;
+  const InstDev &I = insts[q];
+  DD s = dd_zero();
+  for (int b = I.pb0 + (int)threadIdx.x; b < I.pb1; b += kThreads)
+    dd_merge(s, DD{epr[2 * b], epr[2 * b + 1]});
+  s = block_dd(s, scratch);
+  if (threadIdx.x == 0) out[(size_t)q * kEvalStride + kEvalOut] = dd_val(s);
 }
 
 // Restart actions (src/saddle_point_gpu.jl:572-622, 659-692): 1 = reset the
@@ -682,7 +821,7 @@ int launch_step(hipStream_t s, int ninst, const InstDev *insts, Ctl *ctl, const 
 
 int launch_eval(hipStream_t s, int prec, int np, const BlockDesc *pblocks, int nd,
                 const BlockDesc *dblocks, const InstDev *insts, const Ctl *ctl, double *epn,
-                double *epm, int ninst, double *out) {
+                double *epm, double *epr, int ninst, double *out) {
   if (np + nd > 0) {
     if (prec == 0)
       hipLaunchKernelGGL(k_eval<double>, dim3(grid_of(np + nd)), dim3(kThreads), 0, s, pblocks, np,
@@ -694,6 +833,17 @@ int launch_eval(hipStream_t s, int prec, int np, const BlockDesc *pblocks, int n
   if (ninst > 0)
     hipLaunchKernelGGL(k_eval_finish, dim3(grid_of(ninst), kEvalOut), dim3(kThreads), 0, s, insts,
                        ctl, epn, epm, out);
+  if (np > 0) {
+    if (prec == 0)
+      hipLaunchKernelGGL(k_eval_ray<double>, dim3(grid_of(np)), dim3(kThreads), 0, s, pblocks,
+                         insts, ctl, out, epr);
+    else
+      hipLaunchKernelGGL(k_eval_ray<float>, dim3(grid_of(np)), dim3(kThreads), 0, s, pblocks,
+                         insts, ctl, out, epr);
+  }
+  if (ninst > 0)
+    hipLaunchKernelGGL(k_eval_ray_finish, dim3(grid_of(ninst)), dim3(kThreads), 0, s, insts, ctl,
+                       epr, out);
   PDLP_CHECK_LAUNCH();
 }
 

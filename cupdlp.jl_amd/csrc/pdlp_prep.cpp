@@ -1,6 +1,8 @@
 // pdlp_prep.cpp — validation, cached norms, host rescaling and CSR layouts.
 #include "pdlp_prep.h"
 
+#include "pdlp_hostsum.h"
+
 #include <math.h>
 #include <string.h>
 
@@ -80,18 +82,7 @@ int validate_lp(const pdlp_lp *lp, std::string *err) {
   return 0;
 }
 
-static double norm2(const std::vector<double> &v) {
-  // blocked sum of squares, blocks combined in order
-  const size_t B = 4096;
-  double s = 0.0;
-  for (size_t k = 0; k < v.size(); k += B) {
-    double t = 0.0;
-    const size_t e = std::min(v.size(), k + B);
-    for (size_t i = k; i < e; ++i) t += v[i] * v[i];
-    s += t;
-  }
-  return sqrt(s);
-}
+static double norm2(const std::vector<double> &v) { return dd_norm2(v); }
 
 static double norminf(const std::vector<double> &v) {
   double mx = 0.0;
@@ -226,23 +217,27 @@ void rescale_host(const pdlp_params *prm, Prepared *P) {
   if (prm->has_pock_chambolle_alpha) {
     const double alpha = prm->pock_chambolle_alpha;
     const double ec = 2.0 - alpha, er = alpha;
+    // column and row sums as compensated sums (order independent)
     for (int64_t j = 0; j < n; ++j) {
       double s = 0.0;
-      if (ec == 0.0)
+      if (ec == 0.0) {
         s = (double)m;
-      else
-        for (int32_t k = P->t_ptr[j]; k < P->t_ptr[j + 1]; ++k) s += pow(fabs(P->t_val[k]), ec);
+      } else {
+        HostDD acc;
+        for (int32_t k = P->t_ptr[j]; k < P->t_ptr[j + 1]; ++k) acc.add(pow(fabs(P->t_val[k]), ec));
+        s = acc.value();
+      }
       v[j] = sqrt(s);
     }
-    std::fill(r.begin(), r.end(), 0.0);
     if (er == 0.0) {
-      std::fill(r.begin(), r.end(), (double)n);
+      std::fill(r.begin(), r.end(), sqrt((double)n));
     } else {
+      std::vector<HostDD> racc(m);
       for (int64_t j = 0; j < n; ++j)
         for (int32_t k = P->t_ptr[j]; k < P->t_ptr[j + 1]; ++k)
-          r[P->t_col[k]] += pow(fabs(P->t_val[k]), er);
+          racc[P->t_col[k]].add(pow(fabs(P->t_val[k]), er));
+      for (int64_t i = 0; i < m; ++i) r[i] = sqrt(racc[i].value());
     }
-    for (int64_t i = 0; i < m; ++i) r[i] = sqrt(r[i]);
     for (auto &x : v) if (x == 0.0) x = 1.0;
     for (auto &x : r) if (x == 0.0) x = 1.0;
     scale(P, r, v);

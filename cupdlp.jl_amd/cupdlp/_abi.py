@@ -118,6 +118,12 @@ class pdlp_result(C.Structure):
     ]
 
 
+class pdlp_kernel_profile(C.Structure):
+    _fields_ = [("primal_launches", C.c_int64), ("dual_launches", C.c_int64),
+                ("primal_ms", C.c_double), ("dual_ms", C.c_double),
+                ("primal_bytes", C.c_double), ("dual_bytes", C.c_double)]
+
+
 STATS_CB = C.CFUNCTYPE(None, C.POINTER(pdlp_iteration_stats), C.c_void_p)
 
 # Order of the sizes reported by pdlp_abi_layout().
@@ -130,7 +136,7 @@ EXPORTED = (
     "pdlp_ctx_create", "pdlp_ctx_destroy", "pdlp_last_error", "pdlp_default_params",
     "pdlp_solve", "pdlp_batch_solve", "pdlp_serve_open", "pdlp_serve_submit",
     "pdlp_serve_poll", "pdlp_serve_result", "pdlp_serve_close", "pdlp_abi_version",
-    "pdlp_abi_layout",
+    "pdlp_abi_layout", "pdlp_set_profiling", "pdlp_get_profile",
 )
 
 
@@ -165,6 +171,10 @@ def declare(lib):
     lib.pdlp_serve_close.restype = C.c_int
     lib.pdlp_abi_version.argtypes = []
     lib.pdlp_abi_version.restype = C.c_int
+    lib.pdlp_set_profiling.argtypes = [C.c_void_p, C.c_int32]
+    lib.pdlp_set_profiling.restype = C.c_int
+    lib.pdlp_get_profile.argtypes = [C.c_void_p, C.POINTER(pdlp_kernel_profile)]
+    lib.pdlp_get_profile.restype = C.c_int
     lib.pdlp_abi_layout.argtypes = [C.POINTER(C.c_int64), C.c_int32]
     lib.pdlp_abi_layout.restype = C.c_int32
     return lib

@@ -258,6 +258,23 @@ int pdlp_serve_poll(pdlp_server *srv, int64_t *ids, int32_t max_ids,
 int pdlp_serve_result(pdlp_server *srv, int64_t id, pdlp_result *result);
 int pdlp_serve_close(pdlp_server *srv);
 
+/* Timing of the two iteration kernels (extension, for benchmarks): when
+ * enabled, every k_primal / k_dual launch of pdlp_solve / pdlp_batch_solve is
+ * bracketed by HIP events on the context's stream.  Bytes are the algorithmic
+ * bytes of each launch: N(s_A+4) + 4(rows+1) + s_v*(gathered length) + 9*s_v*rows
+ * summed over the instances active in that launch. */
+typedef struct pdlp_kernel_profile {
+  int64_t primal_launches;
+  int64_t dual_launches;
+  double primal_ms;     /* summed event-to-event durations */
+  double dual_ms;
+  double primal_bytes;  /* summed algorithmic bytes */
+  double dual_bytes;
+} pdlp_kernel_profile;
+int pdlp_set_profiling(pdlp_ctx *ctx, int32_t enable);
+/* Copies the accumulated profile and resets it. */
+int pdlp_get_profile(pdlp_ctx *ctx, pdlp_kernel_profile *out);
+
 /* Library identification. */
 int pdlp_abi_version(void);
 

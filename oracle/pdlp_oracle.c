@@ -25,11 +25,11 @@
  *   check_termination_criteria    src/termination.jl:142-246
  *
  * Choices where the reference leaves the result open (documented):
- *  - SpMV sums each CSR row (A) or CSC column (A^T) left to right; cuSPARSE's
- *    order is unspecified.  Dot products and 2-norms are sums over fixed
- *    blocks of 4096 elements combined left to right (independent of the
- *    thread count); cuBLAS's order is unspecified.  2-norms are sqrt(sum x^2)
- *    without cuBLAS's overflow guard.
+ *  - Every sum (SpMV rows, dot products, 2-norms, the Pock-Chambolle row and
+ *    column sums) is a compensated sum of error-free products rounded once,
+ *    i.e. the FP64 value nearest the exact sum, independent of the order;
+ *    cuSPARSE's and cuBLAS's orders are unspecified.  2-norms are
+ *    sqrt(sum x^2) without cuBLAS's overflow guard.
  *  - The triple product Diag(1/r) * A * Diag(1/v) of scale_problem is taken
  *    as ((1/r_i) * a_ij) * (1/v_j).
  *  - ConstantStepsizeParams: the power method starts from a SplitMix64/
@@ -84,49 +84,83 @@ static inline double jl_min(double a, double b) {
   return signbit(a) ? a : b;
 }
 
-/* ---------------- deterministic blocked reductions ---------------- */
+/* ---------------- sums ----------------
+ * Exact mode (default): every sum is a compensated sum of error-free
+ * products (TwoProd with fma, TwoSum), rounded once at the end, i.e. the
+ * FP64 value nearest the exact sum whatever the summation order.  Plain mode
+ * (orc_set_exact(0)) uses ordinary FP64 accumulation and exists only to time
+ * an ordinary CPU implementation. */
 
-static double red_combine(double *part, int64_t nb) {
-  double s = 0.0;
-  for (int64_t k = 0; k < nb; ++k) s += part[k];
-  return s;
+static int g_exact = 1;
+
+void orc_set_exact(int on) { g_exact = on != 0; }
+
+typedef struct {
+  double s, e;
+} dd_t;
+
+static inline void dd_add(dd_t *a, double v) {
+  double s = a->s + v;
+  double bb = s - a->s;
+  double t = (a->s - (s - bb)) + (v - bb);
+  a->s = s;
+  a->e += t;
 }
+static inline void dd_add_prod(dd_t *a, double x, double y) {
+  double p = x * y;
+  double q = fma(x, y, -p);
+  dd_add(a, p);
+  a->e += q;
+}
+static inline void dd_merge(dd_t *a, dd_t b) {
+  double s = a->s + b.s;
+  double bb = s - a->s;
+  double t = (a->s - (s - bb)) + (b.s - bb);
+  a->s = s;
+  a->e = (a->e + b.e) + t;
+}
+static inline double dd_val(dd_t a) { return a.s + a.e; }
 
-static double vdot(const double *a, const double *b, int64_t len) {
+/* mode 0: sum a[i]*b[i]; mode 1: sum a[i]; mode 2: sum a[i]^2 + b[i]^2 */
+static double blocked(const double *a, const double *b, int64_t len, int mode) {
   if (len <= 0) return 0.0;
   int64_t nb = (len + ORC_BLK - 1) / ORC_BLK;
-  double *part = (double *)malloc(sizeof(double) * nb);
+  dd_t *part = (dd_t *)malloc(sizeof(dd_t) * nb);
 #pragma omp parallel for schedule(static)
   for (int64_t k = 0; k < nb; ++k) {
     int64_t e = (k + 1) * ORC_BLK < len ? (k + 1) * ORC_BLK : len;
-    double s = 0.0;
-    for (int64_t i = k * ORC_BLK; i < e; ++i) s += a[i] * b[i];
-    part[k] = s;
+    dd_t acc = {0.0, 0.0};
+    if (g_exact) {
+      for (int64_t i = k * ORC_BLK; i < e; ++i) {
+        if (mode == 0) dd_add_prod(&acc, a[i], b[i]);
+        else if (mode == 1) dd_add(&acc, a[i]);
+        else { dd_add_prod(&acc, a[i], a[i]); dd_add_prod(&acc, b[i], b[i]); }
+      }
+    } else {
+      double t = 0.0;
+      for (int64_t i = k * ORC_BLK; i < e; ++i) {
+        if (mode == 0) t += a[i] * b[i];
+        else if (mode == 1) t += a[i];
+        else t += a[i] * a[i] + b[i] * b[i];
+      }
+      acc.s = t;
+    }
+    part[k] = acc;
   }
-  double s = red_combine(part, nb);
+  dd_t s = {0.0, 0.0};
+  for (int64_t k = 0; k < nb; ++k) {
+    if (g_exact) dd_merge(&s, part[k]);
+    else s.s += part[k].s;
+  }
   free(part);
-  return s;
+  return g_exact ? dd_val(s) : s.s;
 }
 
-static double vsumsq(const double *a, int64_t len) { return vdot(a, a, len); }
-
+static double vdot(const double *a, const double *b, int64_t len) { return blocked(a, b, len, 0); }
+static double vsumsq(const double *a, int64_t len) { return blocked(a, a, len, 0); }
+static double vsumsq_pair(const double *a, const double *b, int64_t len) { return blocked(a, b, len, 2); }
 static double vnorm2(const double *a, int64_t len) { return sqrt(vsumsq(a, len)); }
-
-static double vsum(const double *a, int64_t len) {
-  if (len <= 0) return 0.0;
-  int64_t nb = (len + ORC_BLK - 1) / ORC_BLK;
-  double *part = (double *)malloc(sizeof(double) * nb);
-#pragma omp parallel for schedule(static)
-  for (int64_t k = 0; k < nb; ++k) {
-    int64_t e = (k + 1) * ORC_BLK < len ? (k + 1) * ORC_BLK : len;
-    double s = 0.0;
-    for (int64_t i = k * ORC_BLK; i < e; ++i) s += a[i];
-    part[k] = s;
-  }
-  double s = red_combine(part, nb);
-  free(part);
-  return s;
-}
+static double vsum(const double *a, int64_t len) { return blocked(a, a, len, 1); }
 
 /* norm(v, Inf) = maximum(abs, v); 0 for an empty vector. */
 static double vmaxabs(const double *a, int64_t len) {
@@ -203,9 +237,16 @@ static void build_csr(oprob *P) {
 static void spmv_A(const oprob *P, const double *x, double *y) {
 #pragma omp parallel for schedule(dynamic, 1024)
   for (int64_t i = 0; i < P->m; ++i) {
-    double s = 0.0;
-    for (int64_t k = P->rowptr[i]; k < P->rowptr[i + 1]; ++k) s += P->rval[k] * x[P->colidx[k]];
-    y[i] = s;
+    if (g_exact) {
+      dd_t acc = {0.0, 0.0};
+      for (int64_t k = P->rowptr[i]; k < P->rowptr[i + 1]; ++k)
+        dd_add_prod(&acc, P->rval[k], x[P->colidx[k]]);
+      y[i] = dd_val(acc);
+    } else {
+      double s = 0.0;
+      for (int64_t k = P->rowptr[i]; k < P->rowptr[i + 1]; ++k) s += P->rval[k] * x[P->colidx[k]];
+      y[i] = s;
+    }
   }
 }
 
@@ -213,9 +254,16 @@ static void spmv_A(const oprob *P, const double *x, double *y) {
 static void spmv_AT(const oprob *P, const double *x, double *y) {
 #pragma omp parallel for schedule(dynamic, 1024)
   for (int64_t j = 0; j < P->n; ++j) {
-    double s = 0.0;
-    for (int64_t k = P->colptr[j]; k < P->colptr[j + 1]; ++k) s += P->cval[k] * x[P->rowidx[k]];
-    y[j] = s;
+    if (g_exact) {
+      dd_t acc = {0.0, 0.0};
+      for (int64_t k = P->colptr[j]; k < P->colptr[j + 1]; ++k)
+        dd_add_prod(&acc, P->cval[k], x[P->rowidx[k]]);
+      y[j] = dd_val(acc);
+    } else {
+      double s = 0.0;
+      for (int64_t k = P->colptr[j]; k < P->colptr[j + 1]; ++k) s += P->cval[k] * x[P->rowidx[k]];
+      y[j] = s;
+    }
   }
 }
 
@@ -381,24 +429,28 @@ static void l2_rescale(oprob *P, double *r_out, double *v_out) {
 static void pc_rescale(oprob *P, double alpha, double *r_out, double *v_out) {
   int64_t m = P->m, n = P->n;
   double ec = 2.0 - alpha, er = alpha;
+  /* column and row sums as compensated sums (order independent) */
   for (int64_t j = 0; j < n; ++j) {
     double s = 0.0;
     if (ec == 0.0) {
       s = (double)m;
     } else {
-      for (int64_t k = P->colptr[j]; k < P->colptr[j + 1]; ++k) s += pow(fabs(P->cval[k]), ec);
+      dd_t acc = {0.0, 0.0};
+      for (int64_t k = P->colptr[j]; k < P->colptr[j + 1]; ++k) dd_add(&acc, pow(fabs(P->cval[k]), ec));
+      s = dd_val(acc);
     }
     v_out[j] = sqrt(s);
   }
-  for (int64_t i = 0; i < m; ++i) r_out[i] = 0.0;
   if (er == 0.0) {
-    for (int64_t i = 0; i < m; ++i) r_out[i] = (double)n;
+    for (int64_t i = 0; i < m; ++i) r_out[i] = sqrt((double)n);
   } else {
+    dd_t *racc = (dd_t *)calloc((size_t)(m > 0 ? m : 1), sizeof(dd_t));
     for (int64_t j = 0; j < n; ++j)
       for (int64_t k = P->colptr[j]; k < P->colptr[j + 1]; ++k)
-        r_out[P->rowidx[k]] += pow(fabs(P->cval[k]), er);
+        dd_add(&racc[P->rowidx[k]], pow(fabs(P->cval[k]), er));
+    for (int64_t i = 0; i < m; ++i) r_out[i] = sqrt(dd_val(racc[i]));
+    free(racc);
   }
-  for (int64_t i = 0; i < m; ++i) r_out[i] = sqrt(r_out[i]);
   for (int64_t j = 0; j < n; ++j) if (v_out[j] == 0.0) v_out[j] = 1.0;
   for (int64_t i = 0; i < m; ++i) if (r_out[i] == 0.0) r_out[i] = 1.0;
   scale_problem(P, r_out, v_out);
@@ -486,7 +538,7 @@ static void primal_residual(int64_t n, int64_t m, int64_t neq, const double *l, 
 }
 
 static double primal_res_l2(int64_t n, int64_t m, const kkt_buf *K) {
-  return sqrt(vsumsq(K->cv, m) + vsumsq(K->lv, n) + vsumsq(K->uv, n));
+  return sqrt(vsumsq(K->cv, m) + vsumsq_pair(K->lv, K->uv, n));
 }
 static double primal_res_inf(int64_t n, int64_t m, const kkt_buf *K) {
   return jl_max(jl_max(vmaxabs(K->cv, m), vmaxabs(K->lv, n)), vmaxabs(K->uv, n));

@@ -49,7 +49,7 @@ def long_row_lp(n=6000, seed=7):
     A[0, :] = np.random.default_rng(seed).standard_normal(n)
     A = sp.csc_matrix(A)
     b = prob.right_hand_side.copy()
-    b[0] = A[0, :] @ cert.x  # equality row 0 stays tight; its y* stays valid
+    b[0] = float((A[0, :] @ cert.x)[0])  # equality row 0 stays tight; its y* stays valid
     c = A.T @ cert.y + cert.r
     p = QuadraticProgrammingProblem(prob.variable_lower_bound, prob.variable_upper_bound, None,
                                     c, 0.0, A, b, prob.num_equalities)

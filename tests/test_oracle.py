@@ -1,0 +1,259 @@
+"""Pins the CPU oracle (oracle/pdlp_oracle.c) before it is trusted as the
+checker of the GPU path.  Each check is independent of the oracle's own
+formulas: closed-form LPs, the generator's optimality certificates, HiGHS
+objectives (tests/golden/highs_objectives.json), dense linear algebra,
+the Pock-Chambolle operator-norm bound, and hand-derived step-size values.
+"""
+import json
+import math
+import os
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+import lp_cases
+import oracle as O
+from cupdlp import (PdhgParameters, RestartChoice, TerminationReason as TR,
+                   construct_restart_parameters, construct_termination_criteria)
+from cupdlp.generators import config_lp, random_lp, transportation_lp
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+with open(os.path.join(HERE, "golden", "highs_objectives.json")) as f:
+    HIGHS = json.load(f)["objectives"]
+
+
+def params(tol=1e-8, iteration_limit=2**31 - 1, **kw):
+    tc = construct_termination_criteria(eps_optimal_absolute=tol, eps_optimal_relative=tol,
+                                        iteration_limit=iteration_limit)
+    return PdhgParameters(termination_criteria=tc, **kw)
+
+
+def obj(out):
+    return out.iteration_stats[-1].convergence_information[0].primal_objective
+
+
+# ---------------------------------------------------------------- end to end
+
+def test_closed_form_optimal():
+    prob, opt = lp_cases.tiny_optimal()
+    out = O.solve(prob, params())
+    assert out.termination_reason == TR.TERMINATION_REASON_OPTIMAL
+    assert abs(out.primal_solution[0] - 1.0) < 1e-7
+    assert abs(out.dual_solution[0] - 1.0) < 1e-7  # y* = 1 for the row -x >= -1
+    assert abs(obj(out) - opt) < 1e-7
+
+
+def test_closed_form_two_variables():
+    prob, opt = lp_cases.tiny_two_var()
+    out = O.solve(prob, params())
+    assert out.termination_reason == TR.TERMINATION_REASON_OPTIMAL
+    np.testing.assert_allclose(out.primal_solution, [2.0, 0.0], atol=1e-6)
+    assert abs(obj(out) - opt) < 1e-6
+
+
+def test_primal_infeasible_detected_after_iteration_10():
+    out = O.solve(lp_cases.primal_infeasible(), params(1e-6))
+    assert out.termination_reason == TR.TERMINATION_REASON_PRIMAL_INFEASIBLE
+    assert out.iteration_count >= 9  # suppressed while iteration < 10 (pdhg:688-690)
+
+
+def test_dual_infeasible_detected():
+    out = O.solve(lp_cases.dual_infeasible(), params(1e-6))
+    assert out.termination_reason == TR.TERMINATION_REASON_DUAL_INFEASIBLE
+    assert out.iteration_count >= 9
+
+
+def test_no_rows_defined():
+    prob, opt = lp_cases.no_rows()
+    out = O.solve(prob, params())
+    assert out.termination_reason == TR.TERMINATION_REASON_OPTIMAL
+    np.testing.assert_allclose(out.primal_solution, [0.0, 2.0], atol=1e-9)
+
+
+@pytest.mark.parametrize("name,make,tol", [
+    ("c0_seed0", lambda: config_lp(0), 1e-8),
+    ("random_n300_s11", lambda: random_lp(n=300, m=150, seed=11), 1e-8),
+    ("random_n1200_s12", lambda: random_lp(n=1200, m=600, seed=12), 1e-8),
+    ("long_row_6000", lambda: lp_cases.long_row_lp(6000), 1e-8),
+])
+def test_generator_certificate_and_highs(name, make, tol):
+    prob, cert_obj = make()
+    cert = cert_obj if isinstance(cert_obj, float) else cert_obj.objective
+    out = O.solve(prob, params(tol))
+    assert out.termination_reason == TR.TERMINATION_REASON_OPTIMAL
+    h = HIGHS[name]["objective"]
+    assert abs(cert - h) <= 1e-9 * (1 + abs(h))      # certificate is optimal
+    assert abs(obj(out) - h) <= 20 * tol * (1 + abs(h))
+    # the returned point satisfies the reference optimality test in the original space
+    ci, _ = O.point_stats(prob, out.primal_solution, out.dual_solution, tol, tol)
+    assert ci.relative_l2_primal_residual < 5 * tol
+    assert ci.relative_l2_dual_residual < 5 * tol
+
+
+def test_transportation_against_highs():
+    prob = transportation_lp(60, 60, seed=5)
+    out = O.solve(prob, params(1e-8))
+    assert out.termination_reason == TR.TERMINATION_REASON_OPTIMAL
+    h = HIGHS["transport_60x60_s5"]["objective"]
+    assert abs(obj(out) - h) <= 1e-6 * abs(h)
+
+
+def test_iteration_limit_semantics():
+    prob, _ = config_lp(0, scale=0.25)
+    out = O.solve(prob, params(1e-12, iteration_limit=37))
+    assert out.termination_reason == TR.TERMINATION_REASON_ITERATION_LIMIT
+    assert out.iteration_count == 37
+    # evaluations at iterations 1..10 and every 64 (pdhg:627-630), numbered iteration-1
+    nums = [s.iteration_number for s in out.iteration_stats]
+    assert nums[:10] == list(range(10)) and nums[-1] == 37
+
+
+def test_restart_schedule_early_artificial_restarts():
+    # count >= 0.36 * (iteration - 1) fires at iterations 2 and 3 (count = 1)
+    prob, _ = config_lp(0)
+    out = O.solve(prob, params(1e-12, iteration_limit=12))
+    used = [s.restart_used for s in out.iteration_stats]
+    assert used[0] == RestartChoice.RESTART_CHOICE_NO_RESTART   # iteration 1, count 0
+    assert used[1] != RestartChoice.RESTART_CHOICE_NO_RESTART
+    assert used[2] != RestartChoice.RESTART_CHOICE_NO_RESTART
+    assert used[3] == RestartChoice.RESTART_CHOICE_NO_RESTART   # 1 < 0.36 * 3
+
+
+def test_fixed_frequency_and_no_restarts_run():
+    prob, _ = config_lp(0, scale=0.25)
+    rp = construct_restart_parameters(1, 1, 64, 1.0, 0.2, 0.8, 0.5)
+    out = O.solve(prob, params(1e-6, restart_params=rp))
+    assert out.termination_reason == TR.TERMINATION_REASON_OPTIMAL
+    rp = construct_restart_parameters(0, 1, 1000, 1.0, 0.2, 0.8, 0.5)
+    out = O.solve(prob, params(1e-4, restart_params=rp))
+    assert out.termination_reason == TR.TERMINATION_REASON_OPTIMAL
+
+
+def test_invalid_inputs_raise():
+    prob, _ = lp_cases.tiny_optimal()
+    prob.objective_vector[0] = math.nan
+    with pytest.raises(O.OracleError):
+        O.solve(prob, params())
+    prob, _ = lp_cases.tiny_optimal()
+    prob.variable_lower_bound[0] = math.inf
+    with pytest.raises(O.OracleError):
+        O.solve(prob, params())
+    prob, _ = lp_cases.tiny_optimal()
+    with pytest.raises(O.OracleError):
+        O.solve(prob, params(primal_importance=-1.0))
+
+
+# ---------------------------------------------------------------- rescaling
+
+def _dense_scaled(prob, vals):
+    A = prob.constraint_matrix.copy()
+    A.data = vals.copy()
+    return A.toarray()
+
+
+def test_ruiz_diagonal_closed_form():
+    # diag(4, 9): one Ruiz pass divides by sqrt(4)*sqrt(4), sqrt(9)*sqrt(9) -> identity;
+    # Pock-Chambolle then sees unit rows/columns.  E = D = (2, 3).
+    prob = lp_cases.lp([[4.0, 0.0], [0.0, 9.0]], [1.0, 1.0], [1.0, 1.0], [0, 0], [10, 10], 2)
+    vals, c, l, u, b, E, D = O.rescale(prob, PdhgParameters())
+    np.testing.assert_array_equal(vals, [1.0, 1.0])
+    np.testing.assert_array_equal(E, [2.0, 3.0])
+    np.testing.assert_array_equal(D, [2.0, 3.0])
+    np.testing.assert_array_equal(c, [0.5, 1.0 / 3.0])
+    np.testing.assert_array_equal(u, [20.0, 30.0])
+    np.testing.assert_array_equal(b, [0.5, 1.0 / 3.0])
+
+
+def test_rescaling_is_a_diagonal_similarity():
+    prob, _ = random_lp(n=80, m=40, seed=3, row_scale_decades=2.0, col_scale_decades=2.0)
+    vals, c, l, u, b, E, D = O.rescale(prob, PdhgParameters())
+    A = prob.constraint_matrix.toarray()
+    At = _dense_scaled(prob, vals)
+    np.testing.assert_allclose(At, A / E[:, None] / D[None, :], rtol=1e-13)
+    np.testing.assert_allclose(c, prob.objective_vector / D, rtol=1e-14)
+    np.testing.assert_allclose(b, prob.right_hand_side / E, rtol=1e-14)
+    fin = np.isfinite(prob.variable_upper_bound)
+    np.testing.assert_allclose(u[fin], prob.variable_upper_bound[fin] * D[fin], rtol=1e-14)
+
+
+def test_pock_chambolle_operator_norm_bound():
+    # Pock & Chambolle (2011), Lemma 2: with alpha = 1 the scaled matrix has ||A||_2 <= 1.
+    prob, _ = random_lp(n=120, m=60, seed=4, row_scale_decades=3.0, col_scale_decades=2.0)
+    for ruiz in (0, 10):
+        vals, *_ = O.rescale(prob, PdhgParameters(l_inf_ruiz_iterations=ruiz))
+        s = np.linalg.norm(_dense_scaled(prob, vals), 2)
+        assert s <= 1.0 + 1e-12
+
+
+def test_ruiz_equilibrates_infinity_norms():
+    prob, _ = random_lp(n=60, m=30, seed=5, row_scale_decades=3.0, col_scale_decades=2.0)
+    vals, *_ = O.rescale(prob, PdhgParameters(l_inf_ruiz_iterations=40, pock_chambolle_alpha=None))
+    At = np.abs(_dense_scaled(prob, vals))
+    np.testing.assert_allclose(At.max(axis=0), 1.0, atol=1e-6)
+    np.testing.assert_allclose(At.max(axis=1), 1.0, atol=1e-6)
+
+
+# ---------------------------------------------------------------- KKT statistics
+
+def test_kkt_stats_vanish_at_certificate():
+    prob, cert = config_lp(0)
+    ci, ii = O.point_stats(prob, cert.x, cert.y)
+    scale = 1 + abs(cert.objective)
+    assert ci.l2_primal_residual < 1e-10 * (1 + np.linalg.norm(prob.right_hand_side))
+    assert ci.l2_dual_residual < 1e-10 * (1 + np.linalg.norm(prob.objective_vector))
+    assert abs(ci.primal_objective - ci.dual_objective) < 1e-10 * scale
+    assert abs(ci.primal_objective - cert.objective) < 1e-12 * scale
+
+
+def test_kkt_stats_closed_form_perturbations():
+    # A = I (2 rows: one equality, one >=), bounds [0, 1]; b = (0.5, 0.25), c = (1, 1).
+    prob = lp_cases.lp(np.eye(2), [0.5, 0.25], [1.0, 1.0], [0, 0], [1, 1], 1)
+    x = np.array([0.7, 1.5])          # eq row residual 0.5-0.7=-0.2; >= row satisfied; x2 above u by 0.5
+    y = np.array([2.0, -0.3])         # >= row dual negative: dual residual 0.3
+    ci, ii = O.point_stats(prob, x, y, 1e-6, 1e-6)
+    assert math.isclose(ci.l_inf_primal_residual, 0.5, rel_tol=0, abs_tol=1e-15)
+    assert math.isclose(ci.l2_primal_residual, math.hypot(0.2, 0.5), rel_tol=1e-15)
+    assert math.isclose(ci.primal_objective, 2.2, rel_tol=1e-15)
+    # gradient g = c - A^T y = (-1, 1.3): reduced costs r = (min(g1,0), max(g2,0)) = (-1, 1.3),
+    # no violation; dual residual only from y2 < 0.
+    assert math.isclose(ci.l2_dual_residual, 0.3, rel_tol=1e-15)
+    # dual objective = b'y + sum(l r+ + u r-) = (1 - 0.075) + (0*1.3 + 1*(-1)) = -0.075
+    assert math.isclose(ci.dual_objective, -0.075, rel_tol=1e-12)
+    assert math.isclose(ci.l_inf_primal_variable, 1.5, rel_tol=0)
+
+
+# ---------------------------------------------------------------- step-size rule
+
+def test_step_rule_reject_and_shrink():
+    # eta = 1 > limit = movement/interaction = 0.5 -> reject; attempt k = 1:
+    # new eta = min((1 - 1/2^0.3) * 0.5, (1 + 1/2^0.6) * 1), written as the reference
+    # does (pdhg:405-409: 1/(k+1)^r, not (k+1)^-r, which rounds differently)
+    eta, acc, ne = O.step_rule(1.0, -1.0, 0.5, 1)
+    assert not acc and not ne
+    assert eta == min((1 - 1 / 2.0 ** 0.3) * 0.5, (1 + 1 / 2.0 ** 0.6) * 1.0)
+
+
+def test_step_rule_accept_and_grow():
+    eta, acc, ne = O.step_rule(0.1, 1.0, 0.5, 9)
+    assert acc and not ne
+    assert eta == min((1 - 1 / 10.0 ** 0.3) * 0.5, (1 + 1 / 10.0 ** 0.6) * 0.1)
+
+
+def test_step_rule_zero_interaction_and_numerical_error():
+    eta, acc, ne = O.step_rule(0.25, 0.0, 3.0, 4)
+    assert acc and eta == (1 + 1 / 5.0 ** 0.6) * 0.25      # limit = Inf
+    eta, acc, ne = O.step_rule(0.25, 1e-3, 0.0, 4)
+    assert ne and not acc and eta == 0.25
+
+
+# ---------------------------------------------------------------- SpMV
+
+def test_spmv_matches_dense():
+    rng = np.random.default_rng(9)
+    A = sp.random(37, 53, density=0.2, random_state=3, format="csc")
+    prob = lp_cases.lp(A.toarray(), np.zeros(37), np.zeros(53), np.zeros(53), np.ones(53), 10)
+    x, y = rng.standard_normal(53), rng.standard_normal(37)
+    Ax, ATy = O.spmv(prob, x, y)
+    np.testing.assert_allclose(Ax, A @ x, rtol=1e-13, atol=1e-14)
+    np.testing.assert_allclose(ATy, A.T @ y, rtol=1e-13, atol=1e-14)
