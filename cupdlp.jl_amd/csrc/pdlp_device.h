@@ -20,8 +20,12 @@
 namespace pdlp {
 
 constexpr int kThreads = 256;    // work-group size of every kernel (4 waves)
-constexpr int kBlockNnz = 2048;  // nonzeros staged in LDS per row block
-constexpr int kBlockRows = 1024; // rows per row block
+constexpr int kBlockNnz = 1024;  // nonzeros staged in LDS per row block
+constexpr int kBlockRows = 512;  // rows per row block
+// A row longer than kBlockNnz gets a work-group of its own; a row longer than
+// kLongChunk is split into kLongChunk-entry chunks combined by the last chunk
+// (the only cross-work-group handoff in the iteration kernels).
+constexpr int kLongChunk = 65536;
 
 // Evaluation accumulators (see eval kernel): n-side and m-side.
 constexpr int kEvalN = 25;

@@ -230,11 +230,15 @@ int Engine::add(std::unique_ptr<HostInst> h, std::string *err) {
 static void partition_rows(const std::vector<int32_t> &ptr, int64_t R, int inst,
                            std::vector<BlockDesc> &blocks, std::vector<LongRow> &longs,
                            int *nslots) {
+  if (R == 0) {  // one empty block, so every instance takes part in each launch
+    blocks.push_back({inst, 0, 0, 0});
+    return;
+  }
   int64_t r = 0;
   while (r < R) {
     const int64_t len = ptr[r + 1] - ptr[r];
     if (len > kBlockNnz) {
-      const int nch = (int)((len + kBlockNnz - 1) / kBlockNnz);
+      const int nch = (int)((len + kLongChunk - 1) / kLongChunk);
       const int lid = (int)longs.size();
       longs.push_back({*nslots, nch});
       *nslots += nch;
@@ -299,8 +303,10 @@ int Engine::rebuild_tables(std::string *err) {
   d_epm_ = (double *)alloc(sizeof(double) * 2 * kEvalM * db.size());
   d_epr_ = (double *)alloc(sizeof(double) * 2 * pb.size());
   d_out_ = (double *)alloc(sizeof(double) * kEvalStride * ninst);
+  d_step_cnt_ = (int32_t *)alloc(sizeof(int32_t) * ninst);
   if (!d_pblocks_ || !d_dblocks_ || !d_longs_ || !d_long_cnt_ || !d_long_part_ || !d_insts_ ||
-      !d_ctl_ || !d_ppart_ || !d_dpart_ || !d_epn_ || !d_epm_ || !d_epr_ || !d_out_) {
+      !d_ctl_ || !d_ppart_ || !d_dpart_ || !d_epn_ || !d_epm_ || !d_epr_ || !d_out_ ||
+      !d_step_cnt_) {
     if (err) *err = "device allocation of block tables failed";
     return -2;
   }
@@ -312,6 +318,7 @@ int Engine::rebuild_tables(std::string *err) {
     HIPCHK(hipMemcpyAsync(d_longs_, longs.data(), sizeof(LongRow) * longs.size(), hipMemcpyHostToDevice, s_));
     HIPCHK(hipMemsetAsync(d_long_cnt_, 0, sizeof(int32_t) * longs.size(), s_));
   }
+  HIPCHK(hipMemsetAsync(d_step_cnt_, 0, sizeof(int32_t) * std::max<size_t>(ninst, 1), s_));
   std::vector<InstDev> devs(ninst);
   for (size_t q = 0; q < ninst; ++q) devs[q] = inst_[q]->dev;
   HIPCHK(hipMemcpyAsync(d_insts_, devs.data(), sizeof(InstDev) * ninst, hipMemcpyHostToDevice, s_));
@@ -370,7 +377,7 @@ int Engine::run_attempts(int64_t count, std::string *err) {
   std::vector<int64_t> remaining;
   if (prof) {
     const double sv = prec_ ? 4.0 : 8.0;
-    const size_t need = 4 * (size_t)count;
+    const size_t need = 3 * (size_t)count;
     while (prof_->events.size() < need) {
       hipEvent_t ev;
       HIPCHK(hipEventCreate(&ev));
@@ -386,18 +393,17 @@ int Engine::run_attempts(int64_t count, std::string *err) {
   }
   const double t0 = now_sec();
   for (int64_t a = 0; a < count; ++a) {
-    if (prof) HIPCHK(hipEventRecord(prof_->events[4 * a + 0], s_));
+    if (prof) HIPCHK(hipEventRecord(prof_->events[3 * a + 0], s_));
     int e = launch_primal(s_, prec_, false, np_, d_pblocks_, d_longs_, d_long_cnt_, d_long_part_,
                           d_insts_, d_ctl_, d_ppart_);
-    if (prof) HIPCHK(hipEventRecord(prof_->events[4 * a + 1], s_));
+    if (prof) HIPCHK(hipEventRecord(prof_->events[3 * a + 1], s_));
     if (!e)
       e = launch_dual(s_, prec_, false, nd_, d_dblocks_, d_longs_, d_long_cnt_, d_long_part_,
                       d_insts_, d_ctl_, d_dpart_);
-    if (prof) HIPCHK(hipEventRecord(prof_->events[4 * a + 2], s_));
+    if (prof) HIPCHK(hipEventRecord(prof_->events[3 * a + 2], s_));
     if (!e)
       e = launch_step(s_, (int)inst_.size(), d_insts_, d_ctl_, d_ppart_, d_dpart_, d_red_, d_grow_,
                       tab_len_, prm_.reduction_exponent, prm_.growth_exponent);
-    if (prof) HIPCHK(hipEventRecord(prof_->events[4 * a + 3], s_));
     if (e) {
       if (err) *err = std::string("kernel launch failed: ") + hipGetErrorString((hipError_t)e);
       return -e;
@@ -411,8 +417,8 @@ int Engine::run_attempts(int64_t count, std::string *err) {
   if (prof) {
     for (int64_t a = 0; a < count; ++a) {
       float ms_p = 0.f, ms_d = 0.f;
-      HIPCHK(hipEventElapsedTime(&ms_p, prof_->events[4 * a + 0], prof_->events[4 * a + 1]));
-      HIPCHK(hipEventElapsedTime(&ms_d, prof_->events[4 * a + 1], prof_->events[4 * a + 2]));
+      HIPCHK(hipEventElapsedTime(&ms_p, prof_->events[3 * a + 0], prof_->events[3 * a + 1]));
+      HIPCHK(hipEventElapsedTime(&ms_d, prof_->events[3 * a + 1], prof_->events[3 * a + 2]));
       double bp = 0.0, bd = 0.0;
       for (size_t q = 0; q < ninst; ++q)
         if (a < remaining[q]) {

@@ -122,49 +122,48 @@ __device__ __forceinline__ void st(void *p, int i, double v) {
   ((T *)p)[i] = (T)v;
 }
 
-// Products of one row block staged in LDS as exact (hi, lo) pairs, then
-// per-row compensated sums by groups of `tpr` lanes (a power of two, at most a
-// wave).  epi(row, sum) runs on the first lane of each group.  With do_spmv
-// false no matrix entry is read and epi receives 0.
-template <typename T, typename V, typename Epi>
-__device__ __forceinline__ void rowblock_spmv(const int32_t *__restrict__ ptr,
+// Row sums of one row block.  Phase 1: the block's nonzeros are read as one
+// flat, fully coalesced range (non-temporal loads for the streamed matrix, so
+// the gathered vector keeps the caches), each exact product (hi, lo) is
+// staged in LDS.  Phase 2: groups of `tpr` lanes (a power of two, at most a
+// wave) form each row's compensated sum into rowsum[r].  The caller then runs
+// the elementwise epilogue with consecutive threads on consecutive rows.
+template <typename T, typename V>
+__device__ __forceinline__ void rowblock_sums(const int32_t *__restrict__ ptr,
                                               const int32_t *__restrict__ col,
                                               const T *__restrict__ val,
                                               const V *__restrict__ xv, int row0, int nrows,
-                                              double *lds_p, double *lds_q, bool do_spmv,
-                                              Epi &&epi) {
+                                              double *lds_p, double *lds_q, double *rowsum) {
   const int k0 = ptr[row0];
-  int tpr = 1;
-  if (do_spmv) {
-    const int nz = ptr[row0 + nrows] - k0;
-    for (int k = threadIdx.x; k < nz; k += kThreads) {
-      const double a = (double)val[k0 + k];
-      const double x = (double)xv[col[k0 + k]];
-      const double pr = a * x;
-      lds_p[k] = pr;
-      lds_q[k] = __builtin_fma(a, x, -pr);
-    }
-    __syncthreads();
-    while (tpr < 64 && tpr * 2 * nrows <= kThreads) tpr <<= 1;
+  const int nz = ptr[row0 + nrows] - k0;
+#pragma unroll 4
+  for (int k = threadIdx.x; k < nz; k += kThreads) {
+    const double a = (double)__builtin_nontemporal_load(&val[k0 + k]);
+    const double x = (double)xv[__builtin_nontemporal_load(&col[k0 + k])];
+    const double pr = a * x;
+    lds_p[k] = pr;
+    lds_q[k] = __builtin_fma(a, x, -pr);
   }
+  __syncthreads();
+  int tpr = 1;
+  while (tpr < 64 && tpr * 2 * nrows <= kThreads) tpr <<= 1;
   const int lane = threadIdx.x & (tpr - 1);
   const int grp = threadIdx.x / tpr;
   const int ngrp = kThreads / tpr;
   for (int r = grp; r < nrows; r += ngrp) {
+    const int rs = ptr[row0 + r] - k0, re = ptr[row0 + r + 1] - k0;
     DD acc = dd_zero();
-    if (do_spmv) {
-      const int rs = ptr[row0 + r] - k0, re = ptr[row0 + r + 1] - k0;
-      for (int k = rs + lane; k < re; k += tpr) {
-        dd_add(acc, lds_p[k]);
-        acc.e += lds_q[k];
-      }
-      for (int o = tpr >> 1; o > 0; o >>= 1) {
-        const DD w{__shfl_xor(acc.s, o, tpr), __shfl_xor(acc.e, o, tpr)};
-        dd_merge(acc, w);
-      }
+    for (int k = rs + lane; k < re; k += tpr) {
+      dd_add(acc, lds_p[k]);
+      acc.e += lds_q[k];
     }
-    if (lane == 0) epi(row0 + r, dd_val(acc));
+    for (int o = tpr >> 1; o > 0; o >>= 1) {
+      const DD w{__shfl_xor(acc.s, o, tpr), __shfl_xor(acc.e, o, tpr)};
+      dd_merge(acc, w);
+    }
+    if (lane == 0) rowsum[r] = dd_val(acc);
   }
+  __syncthreads();
 }
 
 // Compensated partial sum of one kBlockNnz chunk of a long row; valid in thread 0.
@@ -174,11 +173,12 @@ __device__ __forceinline__ DD longrow_chunk(const int32_t *__restrict__ ptr,
                                             const T *__restrict__ val,
                                             const V *__restrict__ xv, int row, int chunk,
                                             double *scratch) {
-  const int kb = ptr[row] + chunk * kBlockNnz;
-  const int ke = min(kb + kBlockNnz, ptr[row + 1]);
+  const int kb = ptr[row] + chunk * kLongChunk;
+  const int ke = min(kb + kLongChunk, ptr[row + 1]);
   DD acc = dd_zero();
   for (int k = kb + (int)threadIdx.x; k < ke; k += kThreads)
-    dd_add_prod(acc, (double)val[k], (double)xv[col[k]]);
+    dd_add_prod(acc, (double)__builtin_nontemporal_load(&val[k]),
+                (double)xv[__builtin_nontemporal_load(&col[k])]);
   return block_dd(acc, scratch);
 }
 
@@ -187,6 +187,10 @@ __device__ __forceinline__ DD longrow_chunk(const int32_t *__restrict__ ptr,
 __device__ __forceinline__ bool longrow_publish(const LongRow lr, int chunk, int lid,
                                                 int32_t *long_cnt, double *long_part, DD s,
                                                 double *total) {
+  if (lr.nchunks == 1) {  // whole row in this work-group: no handoff
+    *total = dd_val(s);
+    return true;
+  }
   __hip_atomic_store(&long_part[2 * (lr.slot + chunk)], s.s, __ATOMIC_RELAXED,
                      __HIP_MEMORY_SCOPE_AGENT);
   __hip_atomic_store(&long_part[2 * (lr.slot + chunk) + 1], s.e, __ATOMIC_RELAXED,
@@ -221,6 +225,7 @@ __global__ __launch_bounds__(kThreads) void k_primal(const BlockDesc *__restrict
                                                      double *__restrict__ part) {
   __shared__ double lds_p[kBlockNnz];
   __shared__ double lds_q[kBlockNnz];
+  __shared__ double rowsum[kBlockRows];
   __shared__ double scratch[2 * kThreads / 64];
   const BlockDesc bd = blocks[blockIdx.x];
   const Ctl *C = &ctl[bd.inst];
@@ -260,9 +265,11 @@ __global__ __launch_bounds__(kThreads) void k_primal(const BlockDesc *__restrict
       dd_add_prod(acc, d, d);
     }
   };
-  if (bd.nrows > 0) {
-    rowblock_spmv(I.t_ptr, I.t_col, (const T *)I.t_val, yv, bd.row0, bd.nrows, lds_p, lds_q, need,
-                  epi);
+  if (bd.nrows >= 0) {
+    if (need)
+      rowblock_sums(I.t_ptr, I.t_col, (const T *)I.t_val, yv, bd.row0, bd.nrows, lds_p, lds_q,
+                    rowsum);
+    for (int r = threadIdx.x; r < bd.nrows; r += kThreads) epi(bd.row0 + r, need ? rowsum[r] : 0.0);
   } else {
     const int chunk = -bd.nrows - 1;
     if (!need) {
@@ -285,117 +292,23 @@ __global__ __launch_bounds__(kThreads) void k_primal(const BlockDesc *__restrict
   }
 }
 
-// Dual half-step over the rows of A.  FLUSH: only the pending weighted sums.
-// part[4b..4b+3]: compensated sums of dy^2 and Adx*dy of block b.
-template <typename T, bool FLUSH>
-__global__ __launch_bounds__(kThreads) void k_dual(const BlockDesc *__restrict__ blocks,
-                                                   const LongRow *__restrict__ longs,
-                                                   int32_t *__restrict__ long_cnt,
-                                                   double *__restrict__ long_part,
-                                                   const InstDev *__restrict__ insts,
-                                                   const Ctl *__restrict__ ctl,
-                                                   double *__restrict__ part) {
-  __shared__ double lds_p[kBlockNnz];
-  __shared__ double lds_q[kBlockNnz];
-  __shared__ double scratch[2 * kThreads / 64];
-  const BlockDesc bd = blocks[blockIdx.x];
-  const Ctl *C = &ctl[bd.inst];
-  const bool run = FLUSH ? (C->eval_flag != 0) : (C->active != 0);
-  if (!run) return;
-  const InstDev &I = insts[bd.inst];
-  const int cur = C->cur;
-  const bool pending = C->pending != 0;
-  const double w = C->w_pending;
-  const double sig = C->omega * C->eta;
-  const int neq = I.neq;
-  const T *__restrict__ yv = (const T *)(cur ? I.y1 : I.y0);
-  T *__restrict__ yn = (T *)(cur ? I.y0 : I.y1);
-  const T *__restrict__ axv = (const T *)(cur ? I.ax1 : I.ax0);
-  T *__restrict__ axn = (T *)(cur ? I.ax0 : I.ax1);
-  T *__restrict__ sy = (T *)I.sy;
-  T *__restrict__ sax = (T *)I.sax;
-  const T *__restrict__ bv = (const T *)I.b;
-  const T *__restrict__ dxv = (const T *)I.dx;
-  DD acc_dy = dd_zero(), acc_int = dd_zero();
-  auto epi = [&](int i, double adx) {
-    const double yi = (double)yv[i];
-    const double axi = (double)axv[i];
-    if (pending) {
-      sy[i] = (T)((double)sy[i] + yi * w);
-      sax[i] = (T)((double)sax[i] + axi * w);
-    }
-    if (!FLUSH) {
-      double t = yi + sig * (((double)bv[i] - 2.0 * adx) - axi);
-      if (i >= neq) t = jl_max(t, 0.0);
-      const double d = t - yi;
-      yn[i] = (T)(yi + d);
-      axn[i] = (T)(axi + adx);
-      dd_add_prod(acc_dy, d, d);
-      dd_add_prod(acc_int, adx, d);
-    }
-  };
-  if (bd.nrows > 0) {
-    rowblock_spmv(I.a_ptr, I.a_col, (const T *)I.a_val, dxv, bd.row0, bd.nrows, lds_p, lds_q,
-                  !FLUSH, epi);
-  } else {
-    const int chunk = -bd.nrows - 1;
-    if (FLUSH) {
-      if (chunk == 0 && threadIdx.x == 0) epi(bd.row0, 0.0);
-    } else {
-      const DD sm = longrow_chunk(I.a_ptr, I.a_col, (const T *)I.a_val, dxv, bd.row0, chunk, scratch);
-      if (threadIdx.x == 0) {
-        double tot;
-        if (longrow_publish(longs[bd.aux], chunk, bd.aux, long_cnt, long_part, sm, &tot))
-          epi(bd.row0, tot);
-      }
-    }
-  }
-  if (!FLUSH) {
-    const DD a = block_dd(acc_dy, scratch);
-    const DD b = block_dd(acc_int, scratch);
-    if (threadIdx.x == 0) {
-      part[4 * blockIdx.x] = a.s;
-      part[4 * blockIdx.x + 1] = a.e;
-      part[4 * blockIdx.x + 2] = b.s;
-      part[4 * blockIdx.x + 3] = b.e;
-    }
-  }
-}
-
-// Reduction of the step partials and the adaptive step-size rule, one
-// work-group per instance (src/primal_dual_hybrid_gradient_gpu.jl:324-412).
-__global__ __launch_bounds__(kThreads) void k_step(const InstDev *__restrict__ insts,
-                                                   Ctl *__restrict__ ctl,
-                                                   const double *__restrict__ ppart,
-                                                   const double *__restrict__ dpart,
-                                                   const double *__restrict__ red_tab,
-                                                   const double *__restrict__ grow_tab,
-                                                   int64_t tab_len, double red_exp,
-                                                   double grow_exp) {
-  __shared__ double scratch[2 * kThreads / 64];
-  const int q = blockIdx.x;
-  Ctl *C = &ctl[q];
-  const int active = C->active;
-  if (!active) return;
-  const InstDev &I = insts[q];
-  DD sx = dd_zero(), sy = dd_zero(), sxy = dd_zero();
-  for (int b = I.pb0 + (int)threadIdx.x; b < I.pb1; b += kThreads)
-    dd_merge(sx, DD{ppart[2 * b], ppart[2 * b + 1]});
-  for (int b = I.db0 + (int)threadIdx.x; b < I.db1; b += kThreads) {
-    dd_merge(sy, DD{dpart[4 * b], dpart[4 * b + 1]});
-    dd_merge(sxy, DD{dpart[4 * b + 2], dpart[4 * b + 3]});
-  }
-  sx = block_dd(sx, scratch);
-  sy = block_dd(sy, scratch);
-  sxy = block_dd(sxy, scratch);
-  if (threadIdx.x != 0) return;
+// The adaptive step rule of take_step! (src/primal_dual_hybrid_gradient_gpu.jl:324-412)
+// applied to the reduced sums of one attempt; runs on one thread.
+__device__ __forceinline__ void step_rule(Ctl *C, double ssx, double ssy, double sxy,
+                                          const double *__restrict__ red_tab,
+                                          const double *__restrict__ grow_tab, int64_t tab_len,
+                                          double red_exp, double grow_exp) {
   const double omega = C->omega;
   double eta = C->eta;
   const int64_t k = C->attempts + 1;
   C->attempts = k;
   C->kkt_passes += 1.0;
-  const double interaction = fabs(dd_val(sxy));
-  const double nx = sqrt(dd_val(sx)), ny = sqrt(dd_val(sy));
+  // This attempt's passes added the pending weighted sums and left A^T y of
+  // the current y in aty.
+  C->pending = 0;
+  C->aty_valid = 1;
+  const double interaction = fabs(sxy);
+  const double nx = sqrt(ssx), ny = sqrt(ssy);
   const double movement = 0.5 * omega * (nx * nx) + (0.5 / omega) * (ny * ny);
   C->last_interaction = interaction;
   C->last_movement = movement;
@@ -436,6 +349,129 @@ __global__ __launch_bounds__(kThreads) void k_step(const InstDev *__restrict__ i
     C->eta_entry = eta;
     if (C->accepted >= C->target) C->active = 0;
   }
+}
+
+// Dual half-step over the rows of A.  FLUSH: only the pending weighted sums.
+// part[4b..4b+3]: compensated sums of dy^2 and Adx*dy of block b.
+template <typename T, bool FLUSH>
+__global__ __launch_bounds__(kThreads) void k_dual(const BlockDesc *__restrict__ blocks,
+                                                   const LongRow *__restrict__ longs,
+                                                   int32_t *__restrict__ long_cnt,
+                                                   double *__restrict__ long_part,
+                                                   const InstDev *__restrict__ insts,
+                                                   const Ctl *__restrict__ ctl,
+                                                   double *__restrict__ part) {
+  __shared__ double lds_p[kBlockNnz];
+  __shared__ double lds_q[kBlockNnz];
+  __shared__ double rowsum[kBlockRows];
+  __shared__ double scratch[2 * kThreads / 64];
+  const BlockDesc bd = blocks[blockIdx.x];
+  const Ctl *C = &ctl[bd.inst];
+  const bool run = FLUSH ? (C->eval_flag != 0) : (C->active != 0);
+  if (!run) return;
+  const InstDev &I = insts[bd.inst];
+  const int cur = C->cur;
+  const bool pending = C->pending != 0;
+  const double w = C->w_pending;
+  const double sig = C->omega * C->eta;
+  const int neq = I.neq;
+  const T *__restrict__ yv = (const T *)(cur ? I.y1 : I.y0);
+  T *__restrict__ yn = (T *)(cur ? I.y0 : I.y1);
+  const T *__restrict__ axv = (const T *)(cur ? I.ax1 : I.ax0);
+  T *__restrict__ axn = (T *)(cur ? I.ax0 : I.ax1);
+  T *__restrict__ sy = (T *)I.sy;
+  T *__restrict__ sax = (T *)I.sax;
+  const T *__restrict__ bv = (const T *)I.b;
+  const T *__restrict__ dxv = (const T *)I.dx;
+  DD acc_dy = dd_zero(), acc_int = dd_zero();
+  auto epi = [&](int i, double adx) {
+    const double yi = (double)yv[i];
+    const double axi = (double)axv[i];
+    if (pending) {
+      sy[i] = (T)((double)sy[i] + yi * w);
+      sax[i] = (T)((double)sax[i] + axi * w);
+    }
+    if (!FLUSH) {
+      double t = yi + sig * (((double)bv[i] - 2.0 * adx) - axi);
+      if (i >= neq) t = jl_max(t, 0.0);
+      const double d = t - yi;
+      yn[i] = (T)(yi + d);
+      axn[i] = (T)(axi + adx);
+      dd_add_prod(acc_dy, d, d);
+      dd_add_prod(acc_int, adx, d);
+    }
+  };
+  if (bd.nrows >= 0) {
+    if (!FLUSH)
+      rowblock_sums(I.a_ptr, I.a_col, (const T *)I.a_val, dxv, bd.row0, bd.nrows, lds_p, lds_q,
+                    rowsum);
+    for (int r = threadIdx.x; r < bd.nrows; r += kThreads) epi(bd.row0 + r, FLUSH ? 0.0 : rowsum[r]);
+  } else {
+    const int chunk = -bd.nrows - 1;
+    if (FLUSH) {
+      if (chunk == 0 && threadIdx.x == 0) epi(bd.row0, 0.0);
+    } else {
+      const DD sm = longrow_chunk(I.a_ptr, I.a_col, (const T *)I.a_val, dxv, bd.row0, chunk, scratch);
+      if (threadIdx.x == 0) {
+        double tot;
+        if (longrow_publish(longs[bd.aux], chunk, bd.aux, long_cnt, long_part, sm, &tot))
+          epi(bd.row0, tot);
+      }
+    }
+  }
+  if (FLUSH) return;
+  const DD a = block_dd(acc_dy, scratch);
+  const DD b = block_dd(acc_int, scratch);
+  if (threadIdx.x == 0) {
+    part[4 * blockIdx.x] = a.s;
+    part[4 * blockIdx.x + 1] = a.e;
+    part[4 * blockIdx.x + 2] = b.s;
+    part[4 * blockIdx.x + 3] = b.e;
+  }
+}
+
+// Reduction of the step partials in a fixed order and the step rule; one
+// work-group of kStepThreads per instance.
+constexpr int kStepThreads = 1024;
+
+__global__ __launch_bounds__(kStepThreads) void k_step(const InstDev *__restrict__ insts,
+                                                       Ctl *__restrict__ ctl,
+                                                       const double *__restrict__ ppart,
+                                                       const double *__restrict__ dpart,
+                                                       const double *__restrict__ red_tab,
+                                                       const double *__restrict__ grow_tab,
+                                                       int64_t tab_len, double red_exp,
+                                                       double grow_exp) {
+  __shared__ double sc[3][2 * kStepThreads / 64];
+  const int q = blockIdx.x;
+  Ctl *C = &ctl[q];
+  if (!C->active) return;
+  const InstDev &I = insts[q];
+  DD sx = dd_zero(), sy = dd_zero(), sxy = dd_zero();
+  for (int b = I.pb0 + (int)threadIdx.x; b < I.pb1; b += kStepThreads)
+    dd_merge(sx, DD{ppart[2 * b], ppart[2 * b + 1]});
+  for (int b = I.db0 + (int)threadIdx.x; b < I.db1; b += kStepThreads) {
+    dd_merge(sy, DD{dpart[4 * b], dpart[4 * b + 1]});
+    dd_merge(sxy, DD{dpart[4 * b + 2], dpart[4 * b + 3]});
+  }
+  sx = wave_dd(sx);
+  sy = wave_dd(sy);
+  sxy = wave_dd(sxy);
+  const int w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) {
+    sc[0][2 * w] = sx.s; sc[0][2 * w + 1] = sx.e;
+    sc[1][2 * w] = sy.s; sc[1][2 * w + 1] = sy.e;
+    sc[2][2 * w] = sxy.s; sc[2][2 * w + 1] = sxy.e;
+  }
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  DD r[3];
+  for (int v = 0; v < 3; ++v) {
+    r[v] = DD{sc[v][0], sc[v][1]};
+    for (int k = 1; k < kStepThreads / 64; ++k) dd_merge(r[v], DD{sc[v][2 * k], sc[v][2 * k + 1]});
+  }
+  step_rule(C, dd_val(r[0]), dd_val(r[1]), dd_val(r[2]), red_tab, grow_tab, tab_len, red_exp,
+            grow_exp);
 }
 
 // ---------------------------------------------------------------------------
@@ -814,8 +850,8 @@ int launch_step(hipStream_t s, int ninst, const InstDev *insts, Ctl *ctl, const 
                 const double *dpart, const double *red_tab, const double *grow_tab,
                 int64_t tab_len, double red_exp, double grow_exp) {
   if (ninst <= 0) return 0;
-  hipLaunchKernelGGL(k_step, dim3(grid_of(ninst)), dim3(kThreads), 0, s, insts, ctl, ppart, dpart,
-                     red_tab, grow_tab, tab_len, red_exp, grow_exp);
+  hipLaunchKernelGGL(k_step, dim3(grid_of(ninst)), dim3(kStepThreads), 0, s, insts, ctl, ppart,
+                     dpart, red_tab, grow_tab, tab_len, red_exp, grow_exp);
   PDLP_CHECK_LAUNCH();
 }
 

@@ -63,9 +63,6 @@ def same(a, b, tol):
 
 
 def compare_runs(g, o, rel_tol=REL, vec_tol=REL, check_all_stats=True):
-    assert g.termination_reason == o.termination_reason, (g.termination_reason, o.termination_reason)
-    assert g.iteration_count == o.iteration_count
-    assert g.total_attempts == o.total_attempts
     if check_all_stats:
         assert len(g.iteration_stats) == len(o.iteration_stats)
         for sg, so in zip(g.iteration_stats, o.iteration_stats):
@@ -81,6 +78,9 @@ def compare_runs(g, o, rel_tol=REL, vec_tol=REL, check_all_stats=True):
             ig, io = sg.infeasibility_information[0], so.infeasibility_information[0]
             for f in II_FIELDS:
                 assert same(getattr(ig, f), getattr(io, f), rel_tol), (it, f, getattr(ig, f), getattr(io, f))
+    assert g.termination_reason == o.termination_reason, (g.termination_reason, o.termination_reason)
+    assert g.iteration_count == o.iteration_count
+    assert g.total_attempts == o.total_attempts
     if vec_tol == 0.0:
         assert np.array_equal(g.primal_solution, o.primal_solution)
         assert np.array_equal(g.dual_solution, o.dual_solution)
@@ -170,12 +170,22 @@ def test_long_rows(gpu_ctx):
     compare_runs(g, o)
 
 
+def test_very_long_row_chunked(gpu_ctx):
+    say("case: LP with a 140000-entry row (split into chunks combined across work-groups)")
+    prob, opt = lp_cases.long_row_lp(140000, seed=8)
+    p = params(1e-12, iteration_limit=200)
+    g = gpu_ctx.solve(prob, p)
+    o = O.solve(prob, p)
+    compare_runs(g, o)
+
+
 def test_transportation_small(gpu_ctx):
-    say("case: 60 x 60 transportation LP to 1e-6")
+    say("case: 60 x 60 transportation LP to 1e-6 (has rejected step attempts)")
     prob = transportation_lp(60, 60, seed=5)
     p = params(1e-6)
     g = gpu_ctx.solve(prob, p)
     o = O.solve(prob, p)
+    assert o.total_attempts > o.iteration_count  # exercises rejection after acceptance
     compare_runs(g, o)
 
 
