@@ -183,9 +183,13 @@ int prepare_instance(const pdlp_lp *lp, const pdlp_params *prm, HostInst *h, std
 
 int pdlp::prepare_instance(const pdlp_lp *lp, const pdlp_params *prm, HostInst *h,
                            std::string *err) {
-  int rc = load_problem(lp, &h->P, err);
+  // Rescaling runs on the device unless disabled or the optional L2 pass is
+  // requested (that pass is host-only).
+  const bool dev = prm->device_preconditioning && !prm->l2_norm_rescaling;
+  int rc = load_problem(lp, &h->P, err, !dev);
   if (rc) return rc;
-  rescale_host(prm, &h->P);
+  h->P.device_prep = dev;
+  if (!dev) rescale_host(prm, &h->P);
   return 0;
 }
 

@@ -59,6 +59,7 @@ struct InstDev {
   void *sx, *saty, *sy, *sax;
   void *avg_x, *avg_g, *gbuf, *lx, *lg;
   void *avg_y, *avg_ax, *ly, *lax;
+  const double *a_orig, *t_orig;  // FP32 path: unscaled FP64 values (CSR, CSC order)
 };
 
 // Per-instance control block.  Written by the step kernel and by the host

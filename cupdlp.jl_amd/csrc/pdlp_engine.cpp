@@ -11,6 +11,7 @@
 
 #include "pdlp_hostsum.h"
 #include "pdlp_kernels.h"
+#include "pdlp_precond.h"
 
 namespace pdlp {
 
@@ -104,6 +105,7 @@ Engine::~Engine() {
   pool_->put(d_grow_);
   if (ctl_pinned_) (void)hipHostFree(ctl_pinned_);
   if (out_pinned_) (void)hipHostFree(out_pinned_);
+  if (norm_pinned_) (void)hipHostFree(norm_pinned_);
 }
 
 namespace {
@@ -117,8 +119,22 @@ struct Carver {
 };
 }  // namespace
 
+namespace {
+struct AfterScatterCtx {
+  double *a_w, *a_o, *t_w, *t_o;
+  int64_t nnz;
+};
+hipError_t keep_originals(hipStream_t s, void *p) {
+  auto *c = (AfterScatterCtx *)p;
+  if (c->nnz == 0) return hipSuccess;
+  hipError_t e = hipMemcpyAsync(c->a_o, c->a_w, 8 * c->nnz, hipMemcpyDeviceToDevice, s);
+  if (e == hipSuccess) e = hipMemcpyAsync(c->t_o, c->t_w, 8 * c->nnz, hipMemcpyDeviceToDevice, s);
+  return e;
+}
+}  // namespace
+
 int Engine::upload(HostInst &h, std::string *err) {
-  const Prepared &P = h.P;
+  Prepared &P = h.P;
   const size_t n = (size_t)P.n, m = (size_t)P.m, nnz = (size_t)P.nnz;
   const size_t ts = prec_ ? sizeof(float) : sizeof(double);
   Carver cv;
@@ -134,6 +150,12 @@ int Engine::upload(HostInst &h, std::string *err) {
   size_t o_dn[4], o_dm[2];
   for (int k = 0; k < 4; ++k) o_dn[k] = cv.take(8 * n);
   for (int k = 0; k < 2; ++k) o_dm[k] = cv.take(8 * m);
+  // FP32 path: the unscaled FP64 matrix in both layouts (refinement residuals)
+  size_t o_ao = 0, o_to = 0;
+  if (prec_) {
+    o_ao = cv.take(8 * nnz);
+    o_to = cv.take(8 * nnz);
+  }
   h.arena_bytes = cv.off;
   h.arena = pool_->get(cv.off);
   if (!h.arena) {
@@ -142,38 +164,10 @@ int Engine::upload(HostInst &h, std::string *err) {
   }
   char *base = (char *)h.arena;
   HIPCHK(hipMemsetAsync(base, 0, cv.off, s_));
-  auto put = [&](size_t off, const void *src, size_t bytes) -> hipError_t {
+  auto put = [&](void *dst, const void *src, size_t bytes) -> hipError_t {
     if (bytes == 0) return hipSuccess;
-    return hipMemcpyAsync(base + off, src, bytes, hipMemcpyHostToDevice, s_);
+    return hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s_);
   };
-  // values and vectors in the iteration precision
-  std::vector<float> tmp;
-  auto putT = [&](size_t off, const std::vector<double> &v) -> hipError_t {
-    if (v.empty()) return hipSuccess;
-    if (!prec_) return put(off, v.data(), 8 * v.size());
-    tmp.assign(v.begin(), v.end());
-    hipError_t e = put(off, tmp.data(), 4 * v.size());
-    if (e == hipSuccess) e = hipStreamSynchronize(s_);
-    return e;
-  };
-  HIPCHK(put(o_aptr, P.a_ptr.data(), 4 * (m + 1)));
-  HIPCHK(put(o_acol, P.a_col.data(), 4 * nnz));
-  HIPCHK(putT(o_aval, P.a_val));
-  HIPCHK(put(o_tptr, P.t_ptr.data(), 4 * (n + 1)));
-  HIPCHK(put(o_tcol, P.t_col.data(), 4 * nnz));
-  HIPCHK(putT(o_tval, P.t_val));
-  HIPCHK(putT(o_n[0], P.c));
-  HIPCHK(putT(o_n[1], P.l));
-  HIPCHK(putT(o_n[2], P.u));
-  HIPCHK(putT(o_n[11], P.c));  // gbuf = c - A^T 0
-  HIPCHK(putT(o_n[13], P.c));  // last restart gradient
-  HIPCHK(putT(o_m[0], P.b));
-  HIPCHK(put(o_dn[0], P.oc.data(), 8 * n));
-  HIPCHK(put(o_dn[1], P.ol.data(), 8 * n));
-  HIPCHK(put(o_dn[2], P.ou.data(), 8 * n));
-  HIPCHK(put(o_dn[3], P.D.data(), 8 * n));
-  HIPCHK(put(o_dm[0], P.ob.data(), 8 * m));
-  HIPCHK(put(o_dm[1], P.E.data(), 8 * m));
   InstDev &d = h.dev;
   d.n = (int32_t)n; d.m = (int32_t)m; d.neq = (int32_t)P.neq;
   d.a_ptr = (const int32_t *)(base + o_aptr); d.a_col = (const int32_t *)(base + o_acol);
@@ -190,6 +184,115 @@ int Engine::upload(HostInst &h, std::string *err) {
   d.oc = (const double *)(base + o_dn[0]); d.ol = (const double *)(base + o_dn[1]);
   d.ou = (const double *)(base + o_dn[2]); d.D = (const double *)(base + o_dn[3]);
   d.ob = (const double *)(base + o_dm[0]); d.E = (const double *)(base + o_dm[1]);
+  d.a_orig = prec_ ? (const double *)(base + o_ao) : nullptr;
+  d.t_orig = prec_ ? (const double *)(base + o_to) : nullptr;
+  HIPCHK(put(base + o_aptr, P.a_ptr.data(), 4 * (m + 1)));
+  HIPCHK(put(base + o_tptr, P.t_ptr.data(), 4 * (n + 1)));
+  HIPCHK(put(base + o_tcol, P.t_col.data(), 4 * nnz));
+  HIPCHK(put((void *)d.oc, P.oc.data(), 8 * n));
+  HIPCHK(put((void *)d.ol, P.ol.data(), 8 * n));
+  HIPCHK(put((void *)d.ou, P.ou.data(), 8 * n));
+  HIPCHK(put((void *)d.ob, P.ob.data(), 8 * m));
+
+  if (P.device_prep) {
+    // FP64 working copies: the arena's own arrays on the FP64 path, pool
+    // buffers on the FP32 path (converted at the end).
+    std::vector<void *> tmp;
+    auto scratch = [&](size_t bytes) -> void * {
+      void *p = pool_->get(std::max<size_t>(bytes, 256));
+      if (p) tmp.push_back(p);
+      return p;
+    };
+    double *tw, *aw, *cw, *lw, *uw, *bw;
+    if (prec_) {
+      tw = (double *)scratch(8 * nnz); aw = (double *)scratch(8 * nnz);
+      cw = (double *)scratch(8 * n); lw = (double *)scratch(8 * n); uw = (double *)scratch(8 * n);
+      bw = (double *)scratch(8 * m);
+    } else {
+      tw = (double *)d.t_val; aw = (double *)d.a_val;
+      cw = (double *)d.c; lw = (double *)d.l; uw = (double *)d.u; bw = (double *)d.b;
+    }
+    double *rv = (double *)scratch(8 * m), *vv = (double *)scratch(8 * n);
+    int32_t *cursor = (int32_t *)scratch(4 * m);
+    double *npart = (double *)scratch(8 * 5 * 256);
+    if (!tw || !aw || !cw || !lw || !uw || !bw || !rv || !vv || !cursor || !npart) {
+      for (void *p : tmp) pool_->put(p);
+      if (err) *err = "device allocation for preconditioning failed";
+      return -2;
+    }
+    if (!norm_pinned_) HIPCHK(hipHostMalloc((void **)&norm_pinned_, 8 * 5 * 256, 0));
+    HIPCHK(put(tw, P.t_val.data(), 8 * nnz));
+    HIPCHK(put(cw, P.oc.data(), 8 * n));
+    HIPCHK(put(lw, P.ol.data(), 8 * n));
+    HIPCHK(put(uw, P.ou.data(), 8 * n));
+    HIPCHK(put(bw, P.ob.data(), 8 * m));
+    HIPCHK(put((void *)d.D, P.D.data(), 8 * n));  // ones
+    HIPCHK(put((void *)d.E, P.E.data(), 8 * m));  // ones
+    if (m) HIPCHK(hipMemsetAsync(cursor, 0, 4 * m, s_));
+    PrecondArgs pa;
+    pa.n = (int64_t)n; pa.m = (int64_t)m; pa.nnz = (int64_t)nnz;
+    pa.t_ptr = d.t_ptr; pa.t_col = d.t_col; pa.t_val = tw;
+    pa.a_ptr = d.a_ptr; pa.a_col = (int32_t *)d.a_col; pa.cursor = cursor; pa.a_val = aw;
+    pa.c = cw; pa.l = lw; pa.u = uw; pa.b = bw;
+    pa.D = (double *)d.D; pa.E = (double *)d.E; pa.r = rv; pa.v = vv;
+    pa.norm_part = npart; pa.norm_host = norm_pinned_;
+    pa.ruiz_iterations = prm_.l_inf_ruiz_iterations;
+    pa.has_pc = prm_.has_pock_chambolle_alpha != 0;
+    pa.pc_alpha = prm_.pock_chambolle_alpha;
+    AfterScatterCtx ac{aw, (double *)d.a_orig, tw, (double *)d.t_orig, (int64_t)nnz};
+    if (prec_) {
+      pa.after_scatter = keep_originals;
+      pa.after_scatter_ctx = &ac;
+    }
+    PrecondOut po;
+    hipError_t e = precond_run(s_, pa, &po);
+    if (e == hipSuccess && prec_) {
+      e = to_float(s_, (int64_t)nnz, tw, (float *)d.t_val);
+      if (e == hipSuccess) e = to_float(s_, (int64_t)nnz, aw, (float *)d.a_val);
+      if (e == hipSuccess) e = to_float(s_, (int64_t)n, cw, (float *)d.c);
+      if (e == hipSuccess) e = to_float(s_, (int64_t)n, lw, (float *)d.l);
+      if (e == hipSuccess) e = to_float(s_, (int64_t)n, uw, (float *)d.u);
+      if (e == hipSuccess) e = to_float(s_, (int64_t)m, bw, (float *)d.b);
+    }
+    if (e == hipSuccess && n) e = hipMemcpyAsync(d.gbuf, d.c, ts * n, hipMemcpyDeviceToDevice, s_);
+    if (e == hipSuccess && n) e = hipMemcpyAsync(d.lg, d.c, ts * n, hipMemcpyDeviceToDevice, s_);
+    if (e == hipSuccess && n) e = hipMemcpyAsync(P.D.data(), d.D, 8 * n, hipMemcpyDeviceToHost, s_);
+    if (e == hipSuccess && m) e = hipMemcpyAsync(P.E.data(), d.E, 8 * m, hipMemcpyDeviceToHost, s_);
+    if (e == hipSuccess) e = hipStreamSynchronize(s_);
+    for (void *p : tmp) pool_->put(p);
+    if (e != hipSuccess) {
+      if (err) *err = std::string("device preconditioning failed: ") + hipGetErrorString(e);
+      return -(int)e;
+    }
+    P.max_abs_scaled = po.max_abs;
+    P.norm_b_scaled = po.norm_b;
+    P.norm_c_scaled = po.norm_c;
+    // host copies of the scaled data are not needed any more
+    std::vector<double>().swap(P.t_val);
+    return 0;
+  }
+
+  // host-prepared problem: upload the scaled data
+  std::vector<float> tmpf;
+  auto putT = [&](void *dst, const std::vector<double> &v) -> hipError_t {
+    if (v.empty()) return hipSuccess;
+    if (!prec_) return put(dst, v.data(), 8 * v.size());
+    tmpf.assign(v.begin(), v.end());
+    hipError_t e = put(dst, tmpf.data(), 4 * v.size());
+    if (e == hipSuccess) e = hipStreamSynchronize(s_);
+    return e;
+  };
+  HIPCHK(put(base + o_acol, P.a_col.data(), 4 * nnz));
+  HIPCHK(putT(base + o_aval, P.a_val));
+  HIPCHK(putT(base + o_tval, P.t_val));
+  HIPCHK(putT((void *)d.c, P.c));
+  HIPCHK(putT((void *)d.l, P.l));
+  HIPCHK(putT((void *)d.u, P.u));
+  HIPCHK(putT(d.gbuf, P.c));  // gbuf = c - A^T 0
+  HIPCHK(putT(d.lg, P.c));    // last restart gradient
+  HIPCHK(putT((void *)d.b, P.b));
+  HIPCHK(put((void *)d.D, P.D.data(), 8 * n));
+  HIPCHK(put((void *)d.E, P.E.data(), 8 * m));
   HIPCHK(hipStreamSynchronize(s_));
   return 0;
 }
@@ -212,7 +315,9 @@ int Engine::add(std::unique_ptr<HostInst> h, std::string *err) {
   double omega = prm_.primal_importance;
   if (prm_.scale_invariant_initial_primal_weight) {
     // select_initial_primal_weight — src/saddle_point_gpu.jl:767-785
-    const double rhs = sqrt(1.0) * dd_norm2(P.b), obj = sqrt(1.0) * dd_norm2(P.c);
+    const double nb = P.device_prep ? P.norm_b_scaled : dd_norm2(P.b);
+    const double nc = P.device_prep ? P.norm_c_scaled : dd_norm2(P.c);
+    const double rhs = sqrt(1.0) * nb, obj = sqrt(1.0) * nc;
     if (obj > 0.0 && rhs > 0.0) omega = prm_.primal_importance * (obj / rhs);
   }
   c.omega = omega;

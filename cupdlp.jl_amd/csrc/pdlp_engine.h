@@ -106,6 +106,7 @@ class Engine {
   std::vector<Ctl> ctl_;  // host mirror (pinned copy below)
   Ctl *ctl_pinned_ = nullptr;
   double *out_pinned_ = nullptr;
+  double *norm_pinned_ = nullptr;
   size_t out_cap_ = 0;
   // device tables
   bool tables_dirty_ = true;

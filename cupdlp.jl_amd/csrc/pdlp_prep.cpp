@@ -93,7 +93,7 @@ static double norminf(const std::vector<double> &v) {
   return mx;
 }
 
-int load_problem(const pdlp_lp *lp, Prepared *P, std::string *err) {
+int load_problem(const pdlp_lp *lp, Prepared *P, std::string *err, bool build_csr) {
   int rc = validate_lp(lp, err);
   if (rc) return rc;
   const int64_t n = lp->n, m = lp->m, nnz = lp->nnz;
@@ -121,6 +121,10 @@ int load_problem(const pdlp_lp *lp, Prepared *P, std::string *err) {
   P->a_ptr.assign(m + 1, 0);
   for (int64_t k = 0; k < nnz; ++k) P->a_ptr[P->t_col[k] + 1]++;
   for (int64_t i = 0; i < m; ++i) P->a_ptr[i + 1] += P->a_ptr[i];
+  if (!build_csr) {
+    P->max_abs_scaled = norminf(P->t_val);
+    return 0;
+  }
   P->a_col.resize(nnz);
   P->a_val.resize(nnz);
   P->csr_from_csc.resize(nnz);

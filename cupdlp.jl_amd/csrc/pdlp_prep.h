@@ -32,15 +32,18 @@ struct Prepared {
   std::vector<double> oc, ol, ou, ob;
   QpCache qc{};
   double max_abs_scaled = 0.0;  // norm(scaled A, Inf) (elementwise)
+  bool device_prep = false;     // rescaling and the CSR of A are built on the device
+  double norm_b_scaled = 0.0, norm_c_scaled = 0.0;  // set by the device preparation
 };
 
 // validate (src/preprocess.jl:8-74) plus index checks of the C view.
 // Returns 0 or a positive code with *err set.
 int validate_lp(const pdlp_lp *lp, std::string *err);
 
-// Loads the original problem and its cached norms; builds both layouts.
-// Rescaling is applied by rescale_host (or on the device).
-int load_problem(const pdlp_lp *lp, Prepared *P, std::string *err);
+// Loads the original problem and its cached norms.  With build_csr the CSR
+// of A is built on the host; otherwise only its row pointers (the device
+// builds the rest).  Rescaling is applied by rescale_host or on the device.
+int load_problem(const pdlp_lp *lp, Prepared *P, std::string *err, bool build_csr = true);
 
 // rescale_problem (src/preprocess.jl:619-675) on the host: Ruiz-inf,
 // optional L2, optional Pock-Chambolle.  Updates t_val/a_val and the vectors.

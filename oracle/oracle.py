@@ -51,6 +51,8 @@ def lib():
                                C.POINTER(C.c_double)]
         L.orc_spmv.restype = C.c_int
         L.orc_num_threads.restype = C.c_int
+        L.orc_set_exact.argtypes = [C.c_int]
+        L.orc_set_exact.restype = None
         _lib = L
     return _lib
 
@@ -67,6 +69,11 @@ def solve(problem, params, want_current=False):
     if rc != 0:
         raise OracleError(f"oracle error {rc}: {L.orc_last_error().decode()}")
     return out
+
+
+def set_exact(on):
+    """Exact (compensated) sums for checking; plain FP64 sums for timing."""
+    lib().orc_set_exact(1 if on else 0)
 
 
 def rescale(problem, params):

@@ -101,6 +101,18 @@ def test_c0_first_1000_iterations(gpu_ctx):
     assert np.array_equal(g.dual_current, o.dual_current)
 
 
+def test_host_and_device_preconditioning_agree(gpu_ctx):
+    say("case: n=2000 LP, rescaling on the host vs on the device")
+    prob, _ = config_lp(0)
+    a = gpu_ctx.solve(prob, params(1e-8, device_preconditioning=True))
+    b = gpu_ctx.solve(prob, params(1e-8, device_preconditioning=False))
+    compare_runs(a, b)
+    prob, _ = config_lp(1, scale=0.01)
+    a = gpu_ctx.solve(prob, params(1e-12, iteration_limit=300, device_preconditioning=True))
+    b = gpu_ctx.solve(prob, params(1e-12, iteration_limit=300, device_preconditioning=False))
+    compare_runs(a, b)
+
+
 @pytest.mark.parametrize("tol", [1e-4, 1e-8])
 def test_c0_to_tolerance(gpu_ctx, tol):
     say(f"case: n=2000 LP seed 0 to {tol}")
