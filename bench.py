@@ -106,7 +106,7 @@ def run_batch(args, ctx, lib):
     problems = batch_problems(args.count)
     nnz = sum(p.constraint_matrix.nnz for p in problems)
     log(f"  {time.time() - t:.1f} s, total nnz {nnz}, total n {sum(p.num_variables for p in problems)}")
-    params = make_params(args.tol, args.precision)
+    params = make_params(args.tol, args.precision, time_limit=args.time_limit)
     for w in range(args.warmup):
         log(f"warmup {w}")
         ctx.batch_solve(problems, params)
@@ -193,6 +193,8 @@ def main():
     ap.add_argument("--serial-budget", type=float, default=20.0)
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--time-limit", type=float, default=float("inf"),
+                    help="per-solve time limit (a guard; instances that hit it count as not optimal)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "

@@ -86,6 +86,7 @@ void pdlp_default_params(pdlp_params *p) {
   p->refine_inner_tolerance = 1.0e-4;
   p->refine_max_scale = 1.0e6;
   p->device_preconditioning = 1;
+  p->refine_inner_iteration_limit = 0;
 }
 
 int pdlp_ctx_create(int device, pdlp_ctx **out) {
@@ -174,6 +175,16 @@ static int check_params(const pdlp_params *p, std::string *err) {
     *err = "pock_chambolle_alpha must be in [0, 2]";
     return 2;
   }
+  if (p->precision != PDLP_PRECISION_FP64 && p->precision != PDLP_PRECISION_FP32_REFINE) {
+    *err = "precision must be PDLP_PRECISION_FP64 or PDLP_PRECISION_FP32_REFINE";
+    return 2;
+  }
+  if (p->precision == PDLP_PRECISION_FP32_REFINE &&
+      (p->refine_max_outer < 0 || !(p->refine_inner_tolerance > 0) ||
+       !(p->refine_max_scale >= 1.0) || p->refine_inner_iteration_limit < 0)) {
+    *err = "invalid refinement parameters";
+    return 2;
+  }
   return 0;
 }
 
@@ -189,7 +200,10 @@ int pdlp::prepare_instance(const pdlp_lp *lp, const pdlp_params *prm, HostInst *
   int rc = load_problem(lp, &h->P, err, !dev);
   if (rc) return rc;
   h->P.device_prep = dev;
-  if (!dev) rescale_host(prm, &h->P);
+  if (!dev) {
+    if (prm->precision == PDLP_PRECISION_FP32_REFINE) h->P.t_orig_host = h->P.t_val;
+    rescale_host(prm, &h->P);
+  }
   return 0;
 }
 
@@ -216,6 +230,7 @@ extern "C" int pdlp_solve(pdlp_ctx *ctx, const pdlp_lp *lp, const pdlp_params *p
   h->t_submit = t0;
   h->cb = cb;
   h->user = user;
+  result->refinement_rounds = 0;
   Engine eng(ctx->stream, &ctx->pool, *params, &ctx->prof);
   rc = eng.add(std::move(h), &ctx->err);
   if (rc) return rc;
@@ -223,7 +238,6 @@ extern "C" int pdlp_solve(pdlp_ctx *ctx, const pdlp_lp *lp, const pdlp_params *p
   rc = eng.run(&ctx->err);
   if (rc) return rc;
   result->solve_time_sec = now_sec() - t0;
-  result->refinement_rounds = 0;
   return 0;
 }
 
@@ -293,8 +307,12 @@ struct pdlp_server {
 
 static size_t footprint(const Prepared &P, int prec) {
   const size_t ts = prec ? 4 : 8;
-  return (size_t)P.nnz * 2 * (4 + ts) + (size_t)(P.n + P.m + 2) * 4 + ts * (14 * P.n + 11 * P.m) +
-         8 * (4 * P.n + 2 * P.m) + 64 * 1024;
+  size_t b = (size_t)P.nnz * 2 * (4 + ts) + (size_t)(P.n + P.m + 2) * 4 + ts * (14 * P.n + 11 * P.m) +
+             8 * (4 * P.n + 2 * P.m) + 64 * 1024;
+  // FP32 + refinement: the FP64 matrix twice, ylo (T and FP64) and the
+  // refinement vectors (6 n-vectors, 4 m-vectors)
+  if (prec) b += (size_t)P.nnz * 16 + (ts + 8) * P.m + 8 * (6 * P.n + 4 * P.m) + 4 * (2 * P.n + 4 * P.m);
+  return b;
 }
 
 extern "C" int pdlp_serve_open(pdlp_ctx *ctx, const pdlp_params *params, double hbm_budget_bytes,

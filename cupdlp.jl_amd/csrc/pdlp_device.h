@@ -29,7 +29,7 @@ constexpr int kLongChunk = 65536;
 
 // Evaluation accumulators (see eval kernel): n-side and m-side.
 constexpr int kEvalN = 25;
-constexpr int kEvalM = 18;
+constexpr int kEvalM = 22;
 constexpr int kEvalOut = kEvalN + kEvalM;  // index of the primal ray objective
 constexpr int kEvalStride = kEvalOut + 1;   // doubles per instance in the output
 
@@ -60,6 +60,13 @@ struct InstDev {
   void *avg_x, *avg_g, *gbuf, *lx, *lg;
   void *avg_y, *avg_ax, *ly, *lax;
   const double *a_orig, *t_orig;  // FP32 path: unscaled FP64 values (CSR, CSC order)
+  // FP32 + refinement: dual lower bounds of ">=" rows (scaled T, original FP64;
+  // null = 0), the refined FP64 point, its products, and the true problem data
+  void *ylo;
+  const double *oylo;
+  double *xk, *yk, *rax, *raty;
+  const double *rc, *rl, *ru, *rb;
+  double *xb, *yb;  // the best refined point so far
 };
 
 // Per-instance control block.  Written by the step kernel and by the host
@@ -84,6 +91,9 @@ struct Ctl {
   int32_t eval_flag;      // instance is being evaluated
   int32_t use_current;    // evaluation averages := current point
   int32_t restart_action; // 0 none, 1 reset sums to current, 2 restart to average
+  int32_t ref_flag;       // refinement kernels act on this instance
+  int32_t ref_keep;       // the setup copies (xk, yk) to the best point (xb, yb)
+  double ref_dp, ref_dd;  // refinement scale factors of the primal and dual correction
 };
 
 }  // namespace pdlp

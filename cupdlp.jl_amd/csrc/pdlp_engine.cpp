@@ -106,6 +106,8 @@ Engine::~Engine() {
   if (ctl_pinned_) (void)hipHostFree(ctl_pinned_);
   if (out_pinned_) (void)hipHostFree(out_pinned_);
   if (norm_pinned_) (void)hipHostFree(norm_pinned_);
+  if (rout_pinned_) (void)hipHostFree(rout_pinned_);
+  if (d_rout_) pool_->put(d_rout_);
 }
 
 namespace {
@@ -140,21 +142,29 @@ int Engine::upload(HostInst &h, std::string *err) {
   Carver cv;
   const size_t o_aptr = cv.take(4 * (m + 1)), o_acol = cv.take(4 * nnz), o_aval = cv.take(ts * nnz);
   const size_t o_tptr = cv.take(4 * (n + 1)), o_tcol = cv.take(4 * nnz), o_tval = cv.take(ts * nnz);
-  // T-typed n-vectors: c l u x0 x1 aty dx sx saty avg_x avg_g gbuf lx lg
+  // n-vectors: c l u x0 x1 aty dx sx saty avg_x avg_g gbuf lx lg (T, except
+  // the weighted sums sx, saty: FP64 on both paths)
   size_t o_n[14];
-  for (int k = 0; k < 14; ++k) o_n[k] = cv.take(ts * n);
-  // T-typed m-vectors: b y0 y1 ax0 ax1 sy sax avg_y avg_ax ly lax
+  for (int k = 0; k < 14; ++k) o_n[k] = cv.take((k == 7 || k == 8 ? 8 : ts) * n);
+  // m-vectors: b y0 y1 ax0 ax1 sy sax avg_y avg_ax ly lax (T, except the
+  // products ax0, ax1 and the sums sy, sax: FP64)
   size_t o_m[11];
-  for (int k = 0; k < 11; ++k) o_m[k] = cv.take(ts * m);
+  for (int k = 0; k < 11; ++k) o_m[k] = cv.take((k >= 3 && k <= 6 ? 8 : ts) * m);
   // double: oc ol ou D (n); ob E (m)
   size_t o_dn[4], o_dm[2];
   for (int k = 0; k < 4; ++k) o_dn[k] = cv.take(8 * n);
   for (int k = 0; k < 2; ++k) o_dm[k] = cv.take(8 * m);
-  // FP32 path: the unscaled FP64 matrix in both layouts (refinement residuals)
-  size_t o_ao = 0, o_to = 0;
+  // FP32 path: the unscaled FP64 matrix in both layouts, the true problem
+  // vectors and the refinement state (refinement residuals)
+  size_t o_ao = 0, o_to = 0, o_ylo = 0, o_oylo = 0, o_ref[10] = {0};
   if (prec_) {
     o_ao = cv.take(8 * nnz);
     o_to = cv.take(8 * nnz);
+    o_ylo = cv.take(ts * m);
+    o_oylo = cv.take(8 * m);
+    // xk raty rc rl ru xb (n); yk rax rb yb (m)
+    for (int k = 0; k < 6; ++k) o_ref[k] = cv.take(8 * n);
+    for (int k = 6; k < 10; ++k) o_ref[k] = cv.take(8 * m);
   }
   h.arena_bytes = cv.off;
   h.arena = pool_->get(cv.off);
@@ -186,6 +196,24 @@ int Engine::upload(HostInst &h, std::string *err) {
   d.ob = (const double *)(base + o_dm[0]); d.E = (const double *)(base + o_dm[1]);
   d.a_orig = prec_ ? (const double *)(base + o_ao) : nullptr;
   d.t_orig = prec_ ? (const double *)(base + o_to) : nullptr;
+  d.ylo = prec_ ? (void *)(base + o_ylo) : nullptr;
+  d.oylo = prec_ ? (const double *)(base + o_oylo) : nullptr;
+  d.xk = prec_ ? (double *)(base + o_ref[0]) : nullptr;
+  d.raty = prec_ ? (double *)(base + o_ref[1]) : nullptr;
+  d.rc = prec_ ? (const double *)(base + o_ref[2]) : nullptr;
+  d.rl = prec_ ? (const double *)(base + o_ref[3]) : nullptr;
+  d.ru = prec_ ? (const double *)(base + o_ref[4]) : nullptr;
+  d.xb = prec_ ? (double *)(base + o_ref[5]) : nullptr;
+  d.yk = prec_ ? (double *)(base + o_ref[6]) : nullptr;
+  d.rax = prec_ ? (double *)(base + o_ref[7]) : nullptr;
+  d.rb = prec_ ? (const double *)(base + o_ref[8]) : nullptr;
+  d.yb = prec_ ? (double *)(base + o_ref[9]) : nullptr;
+  if (prec_) {
+    HIPCHK(put((void *)d.rc, P.oc.data(), 8 * n));
+    HIPCHK(put((void *)d.rl, P.ol.data(), 8 * n));
+    HIPCHK(put((void *)d.ru, P.ou.data(), 8 * n));
+    HIPCHK(put((void *)d.rb, P.ob.data(), 8 * m));
+  }
   HIPCHK(put(base + o_aptr, P.a_ptr.data(), 4 * (m + 1)));
   HIPCHK(put(base + o_tptr, P.t_ptr.data(), 4 * (n + 1)));
   HIPCHK(put(base + o_tcol, P.t_col.data(), 4 * nnz));
@@ -283,6 +311,14 @@ int Engine::upload(HostInst &h, std::string *err) {
     return e;
   };
   HIPCHK(put(base + o_acol, P.a_col.data(), 4 * nnz));
+  if (prec_) {  // unscaled FP64 values in both layouts
+    std::vector<double> ao(nnz), to(nnz);
+    for (size_t k = 0; k < nnz; ++k) to[k] = P.t_orig_host.empty() ? 0.0 : P.t_orig_host[k];
+    for (size_t p = 0; p < nnz; ++p) ao[p] = to[P.csr_from_csc[p]];
+    HIPCHK(put((void *)d.t_orig, to.data(), 8 * nnz));
+    HIPCHK(put((void *)d.a_orig, ao.data(), 8 * nnz));
+    HIPCHK(hipStreamSynchronize(s_));
+  }
   HIPCHK(putT(base + o_aval, P.a_val));
   HIPCHK(putT(base + o_tval, P.t_val));
   HIPCHK(putT((void *)d.c, P.c));
@@ -323,6 +359,8 @@ int Engine::add(std::unique_ptr<HostInst> h, std::string *err) {
   c.omega = omega;
   c.kkt_passes = 0.5;
   c.aty_valid = 1;
+  h->qc_cur = P.qc;
+  if (prec_) h->tol_cur = prm_.refine_inner_tolerance;
   h->needs_eval = true;
   h->eval_iter = 1;
   h->start_time = now_sec();
@@ -361,6 +399,19 @@ static void partition_rows(const std::vector<int32_t> &ptr, int64_t R, int inst,
     }
     blocks.push_back({inst, (int32_t)start, (int32_t)(r - start), 0});
   }
+}
+
+// Copies the host control blocks to the device through the pinned staging
+// buffer.  The stream is drained first: an earlier asynchronous copy out of
+// the same buffer may not have run yet, and overwriting the buffer before it
+// runs would hand the earlier kernels the later contents.
+hipError_t Engine::push_ctl() {
+  const size_t ninst = inst_.size();
+  if (ninst == 0) return hipSuccess;
+  hipError_t e = hipStreamSynchronize(s_);
+  if (e != hipSuccess) return e;
+  memcpy(ctl_pinned_, ctl_.data(), sizeof(Ctl) * ninst);
+  return hipMemcpyAsync(d_ctl_, ctl_pinned_, sizeof(Ctl) * ninst, hipMemcpyHostToDevice, s_);
 }
 
 int Engine::rebuild_tables(std::string *err) {
@@ -436,8 +487,7 @@ int Engine::rebuild_tables(std::string *err) {
     HIPCHK(hipHostMalloc((void **)&out_pinned_, sizeof(double) * kEvalStride * std::max<size_t>(ninst, 1), 0));
     out_cap_ = ninst;
   }
-  memcpy(ctl_pinned_, ctl_.data(), sizeof(Ctl) * ninst);
-  HIPCHK(hipMemcpyAsync(d_ctl_, ctl_pinned_, sizeof(Ctl) * ninst, hipMemcpyHostToDevice, s_));
+  HIPCHK(push_ctl());
   HIPCHK(hipStreamSynchronize(s_));
   tables_dirty_ = false;
   return 0;
@@ -476,12 +526,15 @@ int Engine::run_attempts(int64_t count, std::string *err) {
   int rc = ensure_step_table(max_att + count + 2, err);
   if (rc) return rc;
   const bool prof = prof_ && prof_->enabled;
-  // Algorithmic bytes per launch of each instance (FP64: s_A = s_v = 8):
-  // N(s_A+4) + 4(rows+1) + s_v*gathered + 9*s_v*rows.
+  // Algorithmic bytes per launch of each instance (an accepted attempt: the
+  // gathered vector read once, the pending sums read and written):
+  //   FP64  primal 12N + 4(n+1) + 8m + 88n   (x c l u sx saty in; aty x' dx sx saty out)
+  //         dual   12N + 4(m+1) + 8n + 72m   (y ax b sy sax in; y' ax' sy sax out)
+  //   FP32  primal  8N + 4(n+1) + 4m + 56n   (sums FP64; no dx)
+  //         dual    8N + 4(m+1) + 4n + 60m + 4(m - neq)   (products, sums FP64; ylo)
   std::vector<double> pbytes, dbytes;
   std::vector<int64_t> remaining;
   if (prof) {
-    const double sv = prec_ ? 4.0 : 8.0;
     const size_t need = 3 * (size_t)count;
     while (prof_->events.size() < need) {
       hipEvent_t ev;
@@ -491,8 +544,14 @@ int Engine::run_attempts(int64_t count, std::string *err) {
     for (size_t q = 0; q < inst_.size(); ++q) {
       const Prepared &P = inst_[q]->P;
       const double N = (double)P.nnz, n = (double)P.n, m = (double)P.m;
-      pbytes.push_back(N * (sv + 4) + 4 * (n + 1) + sv * m + 9 * sv * n);
-      dbytes.push_back(N * (sv + 4) + 4 * (m + 1) + sv * n + 9 * sv * m);
+      const double mi = (double)(P.m - P.neq);
+      if (prec_) {
+        pbytes.push_back(8 * N + 4 * (n + 1) + 4 * m + 56 * n);
+        dbytes.push_back(8 * N + 4 * (m + 1) + 4 * n + 60 * m + 4 * mi);
+      } else {
+        pbytes.push_back(12 * N + 4 * (n + 1) + 8 * m + 88 * n);
+        dbytes.push_back(12 * N + 4 * (m + 1) + 8 * n + 72 * m);
+      }
       remaining.push_back(inst_[q]->done ? 0 : ctl_[q].target - ctl_[q].accepted);
     }
   }
@@ -594,10 +653,12 @@ void Engine::host_eval_logic(size_t q, const double *o) {
   const Prepared &P = h.P;
   const double *N = o, *M = o + kEvalN;
   pdlp_termination_criteria tc = prm_.termination_criteria;
-  if (h.tolerance_override > 0) {
-    tc.eps_optimal_absolute = h.tolerance_override;
-    tc.eps_optimal_relative = h.tolerance_override;
+  const double tol = h.tol_cur > 0 ? h.tol_cur : h.tolerance_override;
+  if (tol > 0) {
+    tc.eps_optimal_absolute = tol;
+    tc.eps_optimal_relative = tol;
   }
+  const QpCache &qc = h.qc_cur;
   const double c0 = P.c0;
   C.kkt_passes += 2.0;
   const double eps_ratio = tc.eps_optimal_absolute / tc.eps_optimal_relative;
@@ -610,18 +671,19 @@ void Engine::host_eval_logic(size_t q, const double *o) {
   pdlp_convergence_information &ci = st.convergence_information;
   ci.candidate_type = PDLP_POINT_TYPE_AVERAGE_ITERATE;
   ci.primal_objective = c0 + N[2];
+  if (prec_) ci.primal_objective -= M[18];  // -ylo' max(Ax - b, 0) of the correction LP
   ci.l_inf_primal_residual = jl_max(M[1], N[1]);
   ci.l2_primal_residual = sqrt(M[0] + N[0]);
-  ci.relative_l_inf_primal_residual = ci.l_inf_primal_residual / (eps_ratio + P.qc.l_inf_b);
-  ci.relative_l2_primal_residual = ci.l2_primal_residual / (eps_ratio + P.qc.l2_b);
+  ci.relative_l_inf_primal_residual = ci.l_inf_primal_residual / (eps_ratio + qc.l_inf_b);
+  ci.relative_l2_primal_residual = ci.l2_primal_residual / (eps_ratio + qc.l2_b);
   ci.l_inf_primal_variable = N[3];
   ci.l2_primal_variable = sqrt(N[4]);
   const double dri = jl_max(M[3], N[6]);
   ci.dual_objective = (M[4] + c0) + N[7];
   ci.l_inf_dual_residual = dri;
   ci.l2_dual_residual = sqrt(M[2] + N[5]);
-  ci.relative_l_inf_dual_residual = ci.l_inf_dual_residual / (eps_ratio + P.qc.l_inf_c);
-  ci.relative_l2_dual_residual = ci.l2_dual_residual / (eps_ratio + P.qc.l2_c);
+  ci.relative_l_inf_dual_residual = ci.l_inf_dual_residual / (eps_ratio + qc.l_inf_c);
+  ci.relative_l2_dual_residual = ci.l2_dual_residual / (eps_ratio + qc.l2_c);
   ci.l_inf_dual_variable = M[5];
   ci.l2_dual_variable = sqrt(M[6]);
   ci.corrected_dual_objective = dri == 0.0 ? ci.dual_objective : -INFINITY;
@@ -651,8 +713,12 @@ void Engine::host_eval_logic(size_t q, const double *o) {
   st.time_spent_doing_basic_algorithm = h.basic_time;
   st.total_attempts = C.attempts;
 
-  int reason = check_termination(tc, P.qc, st);
+  int reason = check_termination(tc, qc, st);
   if (C.numerical_error && reason == 0) reason = PDLP_TERMINATION_REASON_NUMERICAL_ERROR;
+  // a correction solve that has not met its tolerance within its cap is
+  // refined anyway (the outer rounds are bounded by refine_max_outer)
+  if (reason == 0 && prec_ && h.inner_cap > 0 && (int64_t)st.iteration_number >= h.inner_cap)
+    reason = PDLP_TERMINATION_REASON_ITERATION_LIMIT;
   if (h.eval_iter < 10 && (reason == PDLP_TERMINATION_REASON_PRIMAL_INFEASIBLE ||
                            reason == PDLP_TERMINATION_REASON_DUAL_INFEASIBLE))
     reason = 0;
@@ -661,11 +727,27 @@ void Engine::host_eval_logic(size_t q, const double *o) {
   C.aty_valid = 1;
   C.restart_action = 0;
   if (reason != 0) {
-    h.done = true;
-    h.reason = reason;
     h.last_stats = st;
     C.active = 0;
-    if (h.cb && prm_.record_iteration_stats) h.cb(&st, h.user);
+    const bool final_reason = reason == PDLP_TERMINATION_REASON_TIME_LIMIT ||
+                              reason == PDLP_TERMINATION_REASON_KKT_MATRIX_PASS_LIMIT ||
+                              ((reason == PDLP_TERMINATION_REASON_PRIMAL_INFEASIBLE ||
+                                reason == PDLP_TERMINATION_REASON_DUAL_INFEASIBLE) &&
+                               h.ref_round == 0);
+    if (prec_ && !final_reason) {
+      // the FP32 solve of the current (correction) LP is done: refine
+      h.refine_now = true;
+      h.iters_total += h.eval_iter - 1;
+      h.attempts_total += C.attempts;
+      return;
+    }
+    h.done = true;
+    h.reason = reason;
+    if (prec_) {
+      h.iters_total += h.eval_iter - 1;
+      h.attempts_total += C.attempts;
+    }
+    if (h.cb && prm_.record_iteration_stats && !prec_) h.cb(&st, h.user);
     return;
   }
   // run_restart_scheme — src/saddle_point_gpu.jl:479-622 (scaled space)
@@ -675,7 +757,8 @@ void Engine::host_eval_logic(size_t q, const double *o) {
   const double dnp = 1.0 / C.eta_entry / omega;
   auto kkt = [&](int c) {
     const double l2p = sqrt(M[8 + 3 * c] + N[12 + 4 * c]);
-    const double pobj = c0 + N[13 + 4 * c];
+    double pobj = c0 + N[13 + 4 * c];
+    if (prec_) pobj -= M[19 + c];
     const double l2d = sqrt(M[9 + 3 * c] + N[14 + 4 * c]);
     const double dobj = (M[10 + 3 * c] + c0) + N[15 + 4 * c];
     const double gap = fabs(pobj - dobj);
@@ -729,7 +812,7 @@ void Engine::host_eval_logic(size_t q, const double *o) {
     }
   }
   h.last_stats = st;
-  if (h.cb && prm_.record_iteration_stats) h.cb(&st, h.user);
+  if (h.cb && prm_.record_iteration_stats && !prec_) h.cb(&st, h.user);
   const int64_t next = next_eval_iteration(h.eval_iter, prm_.termination_evaluation_frequency,
                                            tc.iteration_limit);
   h.iter_start = h.eval_iter;
@@ -756,8 +839,7 @@ int Engine::evaluate(std::string *err) {
     }
   }
   if (neval == 0) return 0;
-  memcpy(ctl_pinned_, ctl_.data(), sizeof(Ctl) * ninst);
-  HIPCHK(hipMemcpyAsync(d_ctl_, ctl_pinned_, sizeof(Ctl) * ninst, hipMemcpyHostToDevice, s_));
+  HIPCHK(push_ctl());
   int e = launch_primal(s_, prec_, true, np_, d_pblocks_, d_longs_, d_long_cnt_, d_long_part_,
                         d_insts_, d_ctl_, d_ppart_);
   if (!e)
@@ -775,12 +857,15 @@ int Engine::evaluate(std::string *err) {
   HIPCHK(hipStreamSynchronize(s_));
   for (size_t q = 0; q < ninst; ++q)
     if (ctl_[q].eval_flag) host_eval_logic(q, out_pinned_ + q * kEvalStride);
-  memcpy(ctl_pinned_, ctl_.data(), sizeof(Ctl) * ninst);
-  HIPCHK(hipMemcpyAsync(d_ctl_, ctl_pinned_, sizeof(Ctl) * ninst, hipMemcpyHostToDevice, s_));
+  HIPCHK(push_ctl());
   e = launch_restart(s_, prec_, np_, d_pblocks_, nd_, d_dblocks_, d_insts_, d_ctl_);
   if (e) {
     if (err) *err = std::string("restart launch failed: ") + hipGetErrorString((hipError_t)e);
     return -e;
+  }
+  if (prec_) {
+    int rc = refine(err);
+    if (rc) return rc;
   }
   for (size_t q = 0; q < ninst; ++q) {
     if (ctl_[q].eval_flag && inst_[q]->done) {
@@ -806,6 +891,21 @@ int Engine::finish_instance(size_t q, std::string *err) {
   auto val = [&](const std::vector<char> &b, size_t i) -> double {
     return prec_ ? (double)((const float *)b.data())[i] : ((const double *)b.data())[i];
   };
+  if (h.refined) {  // the refined FP64 point, or the best one if the last round made it worse
+    const bool best = h.last_err > h.best_err;
+    if (n) HIPCHK(hipMemcpyAsync(r->x, best ? h.dev.xb : h.dev.xk, 8 * n, hipMemcpyDeviceToHost, s_));
+    if (m) HIPCHK(hipMemcpyAsync(r->y, best ? h.dev.yb : h.dev.yk, 8 * m, hipMemcpyDeviceToHost, s_));
+    HIPCHK(hipStreamSynchronize(s_));
+    if (r->x_current && n) memcpy(r->x_current, r->x, 8 * n);
+    if (r->y_current && m) memcpy(r->y_current, r->y, 8 * m);
+    r->termination_reason = h.reason;
+    r->solve_time_sec = now_sec() - h.t_submit;
+    r->iteration_count = (int32_t)h.iters_total;
+    r->final_stats = best ? h.best_stats : h.last_stats;
+    r->total_attempts = h.attempts_total;
+    r->refinement_rounds = h.ref_round;
+    return 0;
+  }
   HIPCHK(fetch(h.dev.avg_x, bx, n));
   HIPCHK(fetch(h.dev.avg_y, by, m));
   HIPCHK(hipStreamSynchronize(s_));
@@ -822,10 +922,209 @@ int Engine::finish_instance(size_t q, std::string *err) {
   }
   r->termination_reason = h.reason;
   r->solve_time_sec = now_sec() - h.t_submit;
-  r->iteration_count = (int32_t)(h.eval_iter - 1);
+  r->iteration_count = (int32_t)(prec_ ? h.iters_total : h.eval_iter - 1);
   r->final_stats = h.last_stats;
-  r->total_attempts = C.attempts;
+  r->total_attempts = prec_ ? h.attempts_total : C.attempts;
+  r->refinement_rounds = h.ref_round;
   return 0;
+}
+
+// Refinement of the instances whose FP32 solve just finished: update the FP64
+// point, evaluate it on the true problem in FP64, then either finish or set
+// up the next correction LP.
+int Engine::refine(std::string *err) {
+  const size_t ninst = inst_.size();
+  int nref = 0;
+  for (size_t q = 0; q < ninst; ++q) {
+    HostInst &h = *inst_[q];
+    Ctl &C = ctl_[q];
+    C.ref_flag = h.refine_now ? 1 : 0;  // eval_flag is already set for these
+    if (h.refine_now) {
+      C.ref_dp = h.dp;
+      C.ref_dd = h.dd;
+      ++nref;
+    }
+  }
+  if (!nref) return 0;
+  if (rout_cap_ < ninst) {
+    HIPCHK(hipStreamSynchronize(s_));
+    if (d_rout_) pool_->put(d_rout_);
+    if (rout_pinned_) (void)hipHostFree(rout_pinned_);
+    d_rout_ = nullptr;
+    rout_pinned_ = nullptr;
+    rout_cap_ = 0;
+    d_rout_ = (double *)pool_->get(sizeof(double) * kRefOutStride * ninst);
+    if (!d_rout_) {
+      if (err) *err = "device allocation of refinement statistics failed";
+      return -2;
+    }
+    HIPCHK(hipHostMalloc((void **)&rout_pinned_, sizeof(double) * kRefOutStride * ninst, 0));
+    rout_cap_ = ninst;
+  }
+  HIPCHK(push_ctl());
+  int e = launch_refine(s_, prec_, 1, np_, d_pblocks_, nd_, d_dblocks_, d_longs_, d_long_cnt_,
+                        d_long_part_, d_insts_, d_ctl_, d_epn_, d_epm_, (int)ninst, d_rout_);
+  if (e) {
+    if (err) *err = std::string("refinement launch failed: ") + hipGetErrorString((hipError_t)e);
+    return -e;
+  }
+  HIPCHK(hipMemcpyAsync(rout_pinned_, d_rout_, sizeof(double) * kRefOutStride * ninst,
+                        hipMemcpyDeviceToHost, s_));
+  HIPCHK(hipStreamSynchronize(s_));
+  for (size_t q = 0; q < ninst; ++q)
+    if (inst_[q]->refine_now) refine_logic(q, rout_pinned_ + q * kRefOutStride);
+  HIPCHK(push_ctl());
+  e = launch_refine(s_, prec_, 2, np_, d_pblocks_, nd_, d_dblocks_, d_longs_, d_long_cnt_,
+                    d_long_part_, d_insts_, d_ctl_, d_epn_, d_epm_, (int)ninst, d_rout_);
+  if (e) {
+    if (err) *err = std::string("refinement launch failed: ") + hipGetErrorString((hipError_t)e);
+    return -e;
+  }
+  for (size_t q = 0; q < ninst; ++q) {
+    HostInst &h = *inst_[q];
+    if (!h.refine_now) continue;
+    h.refine_now = false;
+    ctl_[q].ref_flag = 0;
+  }
+  return 0;
+}
+
+// Outer test on the refined point (optimality_criteria_met, src/termination.jl:142-170,
+// in the original space) and the scale factors of the next correction LP.
+void Engine::refine_logic(size_t q, const double *o) {
+  HostInst &h = *inst_[q];
+  Ctl &C = ctl_[q];
+  const Prepared &P = h.P;
+  const double *N = o, *M = o + kRefN;
+  h.refined = true;
+  pdlp_termination_criteria tc = prm_.termination_criteria;
+  if (h.tolerance_override > 0) {
+    tc.eps_optimal_absolute = h.tolerance_override;
+    tc.eps_optimal_relative = h.tolerance_override;
+  }
+  const double eps_ratio = tc.eps_optimal_absolute / tc.eps_optimal_relative;
+  const double c0 = P.c0;
+  pdlp_iteration_stats st;
+  memset(&st, 0, sizeof st);
+  st.iteration_number = (int32_t)h.iters_total;
+  st.cumulative_kkt_matrix_passes = C.kkt_passes;
+  st.cumulative_time_sec = now_sec() - h.start_time;
+  pdlp_convergence_information &ci = st.convergence_information;
+  ci.candidate_type = PDLP_POINT_TYPE_AVERAGE_ITERATE;
+  ci.primal_objective = c0 + N[2];
+  ci.l_inf_primal_residual = jl_max(M[1], N[1]);
+  ci.l2_primal_residual = sqrt(M[0] + N[0]);
+  ci.relative_l_inf_primal_residual = ci.l_inf_primal_residual / (eps_ratio + P.qc.l_inf_b);
+  ci.relative_l2_primal_residual = ci.l2_primal_residual / (eps_ratio + P.qc.l2_b);
+  ci.l_inf_primal_variable = N[3];
+  ci.l2_primal_variable = sqrt(N[4]);
+  const double dri = jl_max(M[3], N[6]);
+  ci.dual_objective = (M[4] + c0) + N[7];
+  ci.l_inf_dual_residual = dri;
+  ci.l2_dual_residual = sqrt(M[2] + N[5]);
+  ci.relative_l_inf_dual_residual = ci.l_inf_dual_residual / (eps_ratio + P.qc.l_inf_c);
+  ci.relative_l2_dual_residual = ci.l2_dual_residual / (eps_ratio + P.qc.l2_c);
+  ci.l_inf_dual_variable = M[5];
+  ci.l2_dual_variable = sqrt(M[6]);
+  ci.corrected_dual_objective = dri == 0.0 ? ci.dual_objective : -INFINITY;
+  const double gap = fabs(ci.primal_objective - ci.dual_objective);
+  const double abs_obj = fabs(ci.primal_objective) + fabs(ci.dual_objective);
+  ci.relative_optimality_gap = gap / (eps_ratio + abs_obj);
+  st.infeasibility_information.candidate_type = PDLP_POINT_TYPE_AVERAGE_ITERATE;
+  st.step_size = C.eta_entry;
+  st.primal_weight = C.omega;
+  st.time_spent_doing_basic_algorithm = h.basic_time;
+  st.total_attempts = h.attempts_total;
+  h.last_stats = st;
+  double pe, pb, de, db;
+  if (tc.optimality_norm == PDLP_L_INF) {
+    pe = ci.l_inf_primal_residual; pb = P.qc.l_inf_b; de = ci.l_inf_dual_residual; db = P.qc.l_inf_c;
+  } else {
+    pe = ci.l2_primal_residual; pb = P.qc.l2_b; de = ci.l2_dual_residual; db = P.qc.l2_c;
+  }
+  const double ea = tc.eps_optimal_absolute, er = tc.eps_optimal_relative;
+  // the point's error as a multiple of the tolerance (optimal iff < 1)
+  const double err = jl_max(jl_max(pe / (ea + er * pb), de / (ea + er * db)),
+                            gap / (ea + er * abs_obj));
+  h.last_err = err;
+  const bool improved = err < h.best_err;
+  if (improved) {
+    h.best_err = err;
+    h.best_stats = st;
+  }
+  int reason = 0;
+  if (de < ea + er * db && pe < ea + er * pb && gap < ea + er * abs_obj)
+    reason = PDLP_TERMINATION_REASON_OPTIMAL;
+  else if (st.cumulative_time_sec >= tc.time_sec_limit)
+    reason = PDLP_TERMINATION_REASON_TIME_LIMIT;
+  else if (h.ref_round >= prm_.refine_max_outer || h.iters_total >= (int64_t)tc.iteration_limit)
+    reason = PDLP_TERMINATION_REASON_ITERATION_LIMIT;
+  if (reason) {
+    h.done = true;
+    h.reason = reason;
+    C.ref_flag = 0;
+    if (h.cb && prm_.record_iteration_stats) h.cb(&st, h.user);
+    return;
+  }
+  // Scale factors: the inverse largest violations, grown by at most
+  // refine_max_scale per round.  The correction LP's own tests are relative
+  // to its violations (not to the full reduced costs / residuals, whose
+  // legitimate parts -- multipliers of active bounds, slacks of inactive
+  // rows -- say nothing about how far (xk, yk) is from optimal), and its
+  // initial primal weight balances the two scaled violations.
+  const double alpha = prm_.refine_max_scale > 1.0 ? prm_.refine_max_scale : 1.0;
+  const double ep = jl_max(M[1], N[1]), edl = jl_max(N[6], M[3]);
+  const double dp = ep > 0.0 ? std::min(1.0 / ep, alpha * h.dp) : alpha * h.dp;
+  const double ddn = edl > 0.0 ? std::min(1.0 / edl, alpha * h.dd) : alpha * h.dd;
+  h.dp = dp;
+  h.dd = ddn;
+  if (h.ref_round == 0)
+    h.inner_cap = prm_.refine_inner_iteration_limit > 0
+                      ? (int64_t)prm_.refine_inner_iteration_limit
+                      : std::max<int64_t>(10000, 20 * h.iters_total);
+  h.ref_round += 1;
+  h.qc_cur.l_inf_b = dp * ep;
+  h.qc_cur.l2_b = dp * sqrt(M[0] + N[0]);
+  h.qc_cur.l_inf_c = ddn * edl;
+  h.qc_cur.l2_c = ddn * sqrt(N[5] + M[2]);
+  const double nc = ddn * sqrt(N[8] + M[8]), nbn = dp * sqrt(M[7]);
+  double omega0 = prm_.primal_importance;
+  if (prm_.scale_invariant_initial_primal_weight && nc > 0.0 && nbn > 0.0)
+    omega0 = prm_.primal_importance * (nc / nbn);
+  C.ref_flag = 2;
+  C.ref_keep = improved ? 1 : 0;
+  C.ref_dp = dp;
+  C.ref_dd = ddn;
+  if (h.cb && prm_.record_iteration_stats) h.cb(&st, h.user);
+  reset_inner(q, omega0);
+}
+
+// Fresh PDHG state for the next correction LP (the device side is reset by
+// the setup kernel).
+void Engine::reset_inner(size_t q, double omega0) {
+  HostInst &h = *inst_[q];
+  Ctl &C = ctl_[q];
+  const double eta0 = h.P.max_abs_scaled > 0.0 ? 1.0 / h.P.max_abs_scaled : 1.0;
+  const double kkt = C.kkt_passes;
+  const double dp = C.ref_dp, ddn = C.ref_dd;
+  const int32_t flag = C.ref_flag, keep = C.ref_keep;
+  memset(&C, 0, sizeof C);
+  C.eta = eta0;
+  C.eta_entry = eta0;
+  C.omega = omega0;
+  C.kkt_passes = kkt;
+  C.aty_valid = 1;
+  C.ref_flag = flag;
+  C.ref_keep = keep;
+  C.ref_dp = dp;
+  C.ref_dd = ddn;
+  h.eval_iter = 1;
+  h.iter_start = 1;
+  h.acc_at_eval = 0;
+  h.needs_eval = true;
+  h.last_trial = 1.0;
+  h.pdist = h.ddist = 0.0;
+  h.tol_cur = prm_.refine_inner_tolerance;
 }
 
 int Engine::step_round(std::string *err) {

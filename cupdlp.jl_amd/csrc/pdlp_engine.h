@@ -10,6 +10,7 @@
 // (src/saddle_point_gpu.jl:479-622) and the primal-weight update (:627-654).
 #pragma once
 #include <hip/hip_runtime_api.h>
+#include <math.h>
 #include <stdint.h>
 
 #include <memory>
@@ -61,6 +62,18 @@ struct HostInst {
   pdlp_stats_cb cb = nullptr;
   void *user = nullptr;
   double t_submit = 0.0;             // wall time the request entered the library
+  // FP32 + refinement state
+  QpCache qc_cur{};                  // norms of the LP the current solve terminates on
+  double tol_cur = -1.0;             // tolerance of the current (inner) solve, -1 = user's
+  int32_t ref_round = 0;             // correction LPs solved so far
+  bool refine_now = false;           // inner solve finished in this evaluation
+  bool refined = false;              // (xk, yk) holds a refined point
+  double dp = 1.0, dd = 1.0;         // scale factors of the current correction LP
+  int64_t inner_cap = 0;             // iteration cap of the current correction solve (0 none)
+  double best_err = INFINITY;        // KKT error of the best refined point (xb, yb)
+  pdlp_iteration_stats best_stats{};
+  double last_err = INFINITY;        // KKT error of (xk, yk)
+  int64_t iters_total = 0, attempts_total = 0;
   double tolerance_override = -1.0;  // serving: per-request tolerance
   int64_t tag = -1;                  // serving: request id
 };
@@ -91,11 +104,15 @@ class Engine {
  private:
   int upload(HostInst &h, std::string *err);
   int rebuild_tables(std::string *err);
+  hipError_t push_ctl();
   int ensure_step_table(int64_t need, std::string *err);
   int evaluate(std::string *err);
   int run_attempts(int64_t count, std::string *err);
   int finish_instance(size_t q, std::string *err);
   void host_eval_logic(size_t q, const double *out);
+  int refine(std::string *err);
+  void refine_logic(size_t q, const double *out);
+  void reset_inner(size_t q, double omega0);
 
   hipStream_t s_;
   DevicePool *pool_;
@@ -107,6 +124,9 @@ class Engine {
   Ctl *ctl_pinned_ = nullptr;
   double *out_pinned_ = nullptr;
   double *norm_pinned_ = nullptr;
+  double *rout_pinned_ = nullptr;
+  double *d_rout_ = nullptr;  // owned here (not a table: tables are rebuilt)
+  size_t rout_cap_ = 0;
   size_t out_cap_ = 0;
   // device tables
   bool tables_dirty_ = true;

@@ -24,4 +24,17 @@ int launch_eval(hipStream_t s, int prec, int np, const BlockDesc *pblocks, int n
 int launch_restart(hipStream_t s, int prec, int np, const BlockDesc *pblocks, int nd,
                    const BlockDesc *dblocks, const InstDev *insts, const Ctl *ctl);
 
+// Refinement (FP32 path).  phase 1: update (xk, yk) from the inner solution,
+// FP64 residual products and statistics (out: kRefOutStride doubles per
+// instance, instances with ref_flag == 1); phase 2: set up the next
+// correction LP (ref_flag == 2).  k_eval_finish checks eval_flag, which the
+// caller keeps set for the refined instances.
+constexpr int kRefN = 9;   // n-side statistics of the refined point
+constexpr int kRefM = 9;   // m-side statistics
+constexpr int kRefOutStride = kRefN + kRefM;
+int launch_refine(hipStream_t s, int prec, int phase, int np, const BlockDesc *pblocks, int nd,
+                  const BlockDesc *dblocks, const LongRow *longs, int32_t *long_cnt,
+                  double *long_part, const InstDev *insts, const Ctl *ctl, double *epn,
+                  double *epm, int ninst, double *out);
+
 }  // namespace pdlp

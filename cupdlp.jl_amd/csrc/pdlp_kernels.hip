@@ -30,6 +30,8 @@
 #include <math.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 #include "pdlp_device.h"
 #include "pdlp_kernels.h"
 
@@ -242,8 +244,8 @@ __global__ __launch_bounds__(kThreads) void k_primal(const BlockDesc *__restrict
   T *__restrict__ xn = (T *)(cur ? I.x0 : I.x1);
   T *__restrict__ aty = (T *)I.aty;
   T *__restrict__ dx = (T *)I.dx;
-  T *__restrict__ sx = (T *)I.sx;
-  T *__restrict__ saty = (T *)I.saty;
+  double *__restrict__ sx = (double *)I.sx;
+  double *__restrict__ saty = (double *)I.saty;
   const T *__restrict__ cv = (const T *)I.c;
   const T *__restrict__ lv = (const T *)I.l;
   const T *__restrict__ uv = (const T *)I.u;
@@ -252,17 +254,26 @@ __global__ __launch_bounds__(kThreads) void k_primal(const BlockDesc *__restrict
     const double a = need ? s : (double)aty[j];
     const double xj = (double)xv[j];
     if (pending) {
-      sx[j] = (T)((double)sx[j] + xj * w);
-      saty[j] = (T)((double)saty[j] + a * w);
+      sx[j] = sx[j] + xj * w;
+      saty[j] = saty[j] + a * w;
     }
     if (need) aty[j] = (T)a;
     if (!FLUSH) {
       double t = xj - tau * ((double)cv[j] - a);
       t = jl_min((double)uv[j], jl_max((double)lv[j], t));
-      const double d = t - xj;
-      dx[j] = (T)d;
-      xn[j] = (T)(xj + d);
-      dd_add_prod(acc, d, d);
+      if constexpr (std::is_same<T, double>::value) {
+        const double d = t - xj;
+        dx[j] = d;
+        xn[j] = xj + d;
+        dd_add_prod(acc, d, d);
+      } else {
+        // FP32 iterates: the step is the change of the stored value (the
+        // dual kernel multiplies A by the new point itself, not by dx)
+        const T xr = (T)t;
+        const double d = (double)xr - xj;
+        xn[j] = xr;
+        dd_add_prod(acc, d, d);
+      }
     }
   };
   if (bd.nrows >= 0) {
@@ -377,28 +388,42 @@ __global__ __launch_bounds__(kThreads) void k_dual(const BlockDesc *__restrict__
   const int neq = I.neq;
   const T *__restrict__ yv = (const T *)(cur ? I.y1 : I.y0);
   T *__restrict__ yn = (T *)(cur ? I.y0 : I.y1);
-  const T *__restrict__ axv = (const T *)(cur ? I.ax1 : I.ax0);
-  T *__restrict__ axn = (T *)(cur ? I.ax0 : I.ax1);
-  T *__restrict__ sy = (T *)I.sy;
-  T *__restrict__ sax = (T *)I.sax;
+  const double *__restrict__ axv = (const double *)(cur ? I.ax1 : I.ax0);
+  double *__restrict__ axn = (double *)(cur ? I.ax0 : I.ax1);
+  double *__restrict__ sy = (double *)I.sy;
+  double *__restrict__ sax = (double *)I.sax;
   const T *__restrict__ bv = (const T *)I.b;
-  const T *__restrict__ dxv = (const T *)I.dx;
+  // FP64: A dx (the products are updated incrementally, as the reference
+  // does); FP32: A x_new, so that A x never drifts from the stored point
+  constexpr bool kF32 = std::is_same<T, float>::value;
+  const T *__restrict__ dxv = (const T *)(kF32 ? (cur ? I.x0 : I.x1) : I.dx);
+  const T *__restrict__ ylo = (const T *)I.ylo;
   DD acc_dy = dd_zero(), acc_int = dd_zero();
-  auto epi = [&](int i, double adx) {
+  auto epi = [&](int i, double prod) {
     const double yi = (double)yv[i];
-    const double axi = (double)axv[i];
+    const double axi = axv[i];
     if (pending) {
-      sy[i] = (T)((double)sy[i] + yi * w);
-      sax[i] = (T)((double)sax[i] + axi * w);
+      sy[i] = sy[i] + yi * w;
+      sax[i] = sax[i] + axi * w;
     }
     if (!FLUSH) {
+      const double adx = kF32 ? prod - axi : prod;
       double t = yi + sig * (((double)bv[i] - 2.0 * adx) - axi);
-      if (i >= neq) t = jl_max(t, 0.0);
-      const double d = t - yi;
-      yn[i] = (T)(yi + d);
-      axn[i] = (T)(axi + adx);
-      dd_add_prod(acc_dy, d, d);
-      dd_add_prod(acc_int, adx, d);
+      if (i >= neq) t = jl_max(t, ylo ? (double)ylo[i] : 0.0);
+      if constexpr (!kF32) {
+        const double d = t - yi;
+        yn[i] = yi + d;
+        axn[i] = axi + adx;
+        dd_add_prod(acc_dy, d, d);
+        dd_add_prod(acc_int, adx, d);
+      } else {
+        const T yr = (T)t;
+        const double d = (double)yr - yi;
+        yn[i] = yr;
+        axn[i] = prod;
+        dd_add_prod(acc_dy, d, d);
+        dd_add_prod(acc_int, adx, d);
+      }
     }
   };
   if (bd.nrows >= 0) {
@@ -483,24 +508,23 @@ __global__ __launch_bounds__(kStepThreads) void k_step(const InstDev *__restrict
 // Each accumulator is a compensated sum or an exact maximum; partials hold
 // (s, e) pairs per block.
 
-__device__ __forceinline__ bool eval_is_max_n(int k) {
-  return k == 1 || k == 3 || k == 6 || k == 8 || k == 10 || k == 11;
-}
-__device__ __forceinline__ bool eval_is_max_m(int k) {
-  return k == 1 || k == 3 || k == 5 || k == 7;
-}
+constexpr uint32_t kEvalMaxN = (1u << 1) | (1u << 3) | (1u << 6) | (1u << 8) | (1u << 10) | (1u << 11);
+constexpr uint32_t kEvalMaxM = (1u << 1) | (1u << 3) | (1u << 5) | (1u << 7);
+__device__ __forceinline__ bool eval_is_max_n(int k) { return (kEvalMaxN >> k) & 1u; }
+__device__ __forceinline__ bool eval_is_max_m(int k) { return (kEvalMaxM >> k) & 1u; }
 
 __device__ __forceinline__ double contrib(double l, double u, double r) {
   return r > 0.0 ? l * r : (r < 0.0 ? u * r : 0.0);
 }
 
-template <int K, bool M>
-__device__ __forceinline__ void block_reduce_acc(DD (&acc)[K], double *lds, double *out) {
+template <int K>
+__device__ __forceinline__ void block_reduce_acc(DD (&acc)[K], uint32_t maxmask, double *lds,
+                                                 double *out) {
   const int w = threadIdx.x >> 6;
   constexpr int NW = kThreads / 64;
 #pragma unroll
   for (int k = 0; k < K; ++k) {
-    const bool ismax = M ? eval_is_max_m(k) : eval_is_max_n(k);
+    const bool ismax = (maxmask >> k) & 1u;
     DD v;
     if (ismax) {
       v = DD{wave_max(acc[k].s), 0.0};
@@ -515,7 +539,7 @@ __device__ __forceinline__ void block_reduce_acc(DD (&acc)[K], double *lds, doub
   __syncthreads();
   if (threadIdx.x < K) {
     const int k = threadIdx.x;
-    const bool ismax = M ? eval_is_max_m(k) : eval_is_max_n(k);
+    const bool ismax = (maxmask >> k) & 1u;
     DD r{lds[2 * (k * NW)], lds[2 * (k * NW) + 1]};
     for (int q = 1; q < NW; ++q) {
       const DD v{lds[2 * (k * NW + q)], lds[2 * (k * NW + q) + 1]};
@@ -568,8 +592,8 @@ __global__ __launch_bounds__(kThreads) void k_eval(const BlockDesc *__restrict__
     for (int j = r0 + (int)threadIdx.x; j < r1; j += kThreads) {
       const double xc = ld<T>(xcur, j);
       const double cs = ld<T>(I.c, j), ls = ld<T>(I.l, j), us = ld<T>(I.u, j);
-      const double ax = usecur ? xc : ld<T>(I.sx, j) / wsum;
-      const double ag = usecur ? ld<T>(I.gbuf, j) : (-ld<T>(I.saty, j)) / wsum + cs;
+      const double ax = usecur ? xc : ld<double>(I.sx, j) / wsum;
+      const double ag = usecur ? ld<T>(I.gbuf, j) : (-ld<double>(I.saty, j)) / wsum + cs;
       const double gnew = cs - ld<T>(I.aty, j);
       st<T>(I.avg_x, j, ax);
       st<T>(I.avg_g, j, ag);
@@ -618,7 +642,7 @@ __global__ __launch_bounds__(kThreads) void k_eval(const BlockDesc *__restrict__
       const double dxl = ax - xs3[2];
       dd_add_prod(acc[24], dxl, dxl);
     }
-    block_reduce_acc<kEvalN, false>(acc, lds, epn + (size_t)blk * 2 * kEvalN);
+    block_reduce_acc<kEvalN>(acc, kEvalMaxN, lds, epn + (size_t)blk * 2 * kEvalN);
   } else {
     DD acc[kEvalM];
 #pragma unroll
@@ -626,10 +650,11 @@ __global__ __launch_bounds__(kThreads) void k_eval(const BlockDesc *__restrict__
     const void *ycur = cur ? I.y1 : I.y0;
     const void *axcur = cur ? I.ax1 : I.ax0;
     const int neq = I.neq;
+    const bool has_ylo = I.ylo != nullptr;
     for (int i = r0 + (int)threadIdx.x; i < r1; i += kThreads) {
-      const double yc = ld<T>(ycur, i), axc = ld<T>(axcur, i);
-      const double ay = usecur ? yc : ld<T>(I.sy, i) / wsum;
-      const double aax = usecur ? axc : ld<T>(I.sax, i) / wsum;
+      const double yc = ld<T>(ycur, i), axc = ld<double>(axcur, i);
+      const double ay = usecur ? yc : ld<double>(I.sy, i) / wsum;
+      const double aax = usecur ? axc : ld<double>(I.sax, i) / wsum;
       st<T>(I.avg_y, i, ay);
       st<T>(I.avg_ax, i, aax);
       const double Ei = I.E[i];
@@ -640,9 +665,13 @@ __global__ __launch_bounds__(kThreads) void k_eval(const BlockDesc *__restrict__
       dd_add_prod(acc[0], cvv, cvv);
       acc[1].s = jl_max(acc[1].s, fabs(cvv));
       if (i >= neq) {
-        const double dr = jl_max(-oy, 0.0);
+        // dual residual max(ylo - y, 0); with the primal-objective term
+        // -ylo * max(Ax - b, 0) of a shifted dual bound (refinement only)
+        const double yl = has_ylo ? I.oylo[i] : 0.0;
+        const double dr = has_ylo ? jl_max(yl - oy, 0.0) : jl_max(-oy, 0.0);
         dd_add_prod(acc[2], dr, dr);
         acc[3].s = jl_max(acc[3].s, dr);
+        if (has_ylo) dd_add_prod(acc[18], yl, jl_max(oax - ob, 0.0));
       }
       dd_add_prod(acc[4], ob, oy);
       acc[5].s = jl_max(acc[5].s, fabs(oy));
@@ -657,34 +686,40 @@ __global__ __launch_bounds__(kThreads) void k_eval(const BlockDesc *__restrict__
         const double cc = i < neq ? tt : jl_max(tt, 0.0);
         dd_add_prod(acc[8 + 3 * q], cc, cc);
         if (i >= neq) {
-          const double dr = jl_max(-ys3[q], 0.0);
+          const double yl = has_ylo ? ld<T>(I.ylo, i) : 0.0;
+          const double dr = has_ylo ? jl_max(yl - ys3[q], 0.0) : jl_max(-ys3[q], 0.0);
           dd_add_prod(acc[9 + 3 * q], dr, dr);
+          if (has_ylo) dd_add_prod(acc[19 + q], yl, jl_max(as3[q] - bs, 0.0));
         }
         dd_add_prod(acc[10 + 3 * q], bs, ys3[q]);
       }
       const double dyl = ay - ys3[2];
       dd_add_prod(acc[17], dyl, dyl);
     }
-    block_reduce_acc<kEvalM, true>(acc, lds, epm + (size_t)blk * 2 * kEvalM);
+    block_reduce_acc<kEvalM>(acc, kEvalMaxM, lds, epm + (size_t)blk * 2 * kEvalM);
   }
 }
 
-// Per-instance combination of the evaluation partials: one work-group per
+// Per-instance combination of block partials: one work-group per
 // (instance, accumulator), blocks combined in a fixed strided order.
+// Accumulator k < KN comes from epn, the rest from epm; bit k of maxN /
+// bit (k - KN) of maxM marks a maximum (else a compensated sum).
 __global__ __launch_bounds__(kThreads) void k_eval_finish(const InstDev *__restrict__ insts,
                                                           const Ctl *__restrict__ ctl,
                                                           const double *__restrict__ epn,
                                                           const double *__restrict__ epm,
-                                                          double *__restrict__ out) {
+                                                          double *__restrict__ out, int KN,
+                                                          int KM, uint32_t maxN, uint32_t maxM,
+                                                          int stride) {
   __shared__ double scratch[2 * kThreads / 64];
   const int q = blockIdx.x;
   const int k = blockIdx.y;
   if (!ctl[q].eval_flag) return;
   const InstDev &I = insts[q];
-  const bool nside = k < kEvalN;
-  const int kk = nside ? k : k - kEvalN;
-  const int K = nside ? kEvalN : kEvalM;
-  const bool ismax = nside ? eval_is_max_n(kk) : eval_is_max_m(kk);
+  const bool nside = k < KN;
+  const int kk = nside ? k : k - KN;
+  const int K = nside ? KN : KM;
+  const bool ismax = ((nside ? maxN : maxM) >> kk) & 1u;
   const double *ep = nside ? epn : epm;
   const int b0 = nside ? I.pb0 : I.db0, b1 = nside ? I.pb1 : I.db1;
   if (ismax) {
@@ -698,14 +733,14 @@ __global__ __launch_bounds__(kThreads) void k_eval_finish(const InstDev *__restr
     if (threadIdx.x == 0) {
       double t = scratch[0];
       for (int i = 1; i < kThreads / 64; ++i) t = jl_max(t, scratch[i]);
-      out[(size_t)q * kEvalStride + k] = t;
+      out[(size_t)q * stride + k] = t;
     }
   } else {
     DD s = dd_zero();
     for (int b = b0 + (int)threadIdx.x; b < b1; b += kThreads)
       dd_merge(s, DD{ep[((size_t)b * K + kk) * 2], ep[((size_t)b * K + kk) * 2 + 1]});
     s = block_dd(s, scratch);
-    if (threadIdx.x == 0) out[(size_t)q * kEvalStride + k] = dd_val(s);
+    if (threadIdx.x == 0) out[(size_t)q * stride + k] = dd_val(s);
   }
 }
 
@@ -780,23 +815,241 @@ __global__ __launch_bounds__(kThreads) void k_restart(const BlockDesc *__restric
         st<T>(I.aty, j, ld<T>(I.c, j) - ag);
         st<T>(I.gbuf, j, ag);
       }
-      ((T *)I.sx)[j] = (T)0;
-      ((T *)I.saty)[j] = (T)0;
+      ((double *)I.sx)[j] = 0.0;
+      ((double *)I.saty)[j] = 0.0;
       ((T *)I.lx)[j] = xc[j];
       ((T *)I.lg)[j] = ((const T *)I.gbuf)[j];
     }
   } else {
     T *yc = (T *)(cur ? I.y1 : I.y0);
-    T *axc = (T *)(cur ? I.ax1 : I.ax0);
+    double *axc = (double *)(cur ? I.ax1 : I.ax0);
     for (int i = r0 + (int)threadIdx.x; i < r1; i += kThreads) {
       if (action == 2) {
         yc[i] = ((const T *)I.avg_y)[i];
-        axc[i] = ((const T *)I.avg_ax)[i];
+        axc[i] = (double)((const T *)I.avg_ax)[i];
       }
-      ((T *)I.sy)[i] = (T)0;
-      ((T *)I.sax)[i] = (T)0;
+      ((double *)I.sy)[i] = 0.0;
+      ((double *)I.sax)[i] = 0.0;
       ((T *)I.ly)[i] = yc[i];
-      ((T *)I.lax)[i] = axc[i];
+      ((T *)I.lax)[i] = (T)axc[i];
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// FP32 + FP64 iterative refinement (no reference counterpart: the reference
+// iterates in FP64 only).  Round k keeps an FP64 point (xk, yk); the FP32
+// engine solves the correction LP
+//   min  dD (c - A^T yk)' xh + ...  s.t.  A xh (=, >=) dP (b - A xk),
+//        dP (l - xk) <= xh <= dP (u - xk),   yh_i >= -dD yk_i on ">=" rows,
+// i.e. the original saddle problem shifted to (xk, yk) and scaled by (dP, dD);
+// then xk += xh / dP, yk += yh / dD.  Residuals, statistics and the update
+// are FP64 with compensated sums.
+
+constexpr uint32_t kRefMaxN = (1u << 1) | (1u << 3) | (1u << 6);
+constexpr uint32_t kRefMaxM = (1u << 1) | (1u << 3) | (1u << 5);
+
+// xk += (avg_x / D) / dP clamped to [l, u]; yk += (avg_y / E) / dD, ">=" rows >= 0.
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_ref_update(const BlockDesc *__restrict__ pblocks,
+                                                         int np,
+                                                         const BlockDesc *__restrict__ dblocks,
+                                                         const InstDev *__restrict__ insts,
+                                                         const Ctl *__restrict__ ctl) {
+  const bool nside = (int)blockIdx.x < np;
+  const int blk = nside ? (int)blockIdx.x : (int)blockIdx.x - np;
+  const BlockDesc bd = nside ? pblocks[blk] : dblocks[blk];
+  const Ctl *C = &ctl[bd.inst];
+  if (C->ref_flag != 1) return;
+  const InstDev &I = insts[bd.inst];
+  int r0, r1;
+  block_rows(bd, &r0, &r1);
+  if (nside) {
+    const double dp = C->ref_dp;
+    for (int j = r0 + (int)threadIdx.x; j < r1; j += kThreads) {
+      const double xh = ld<T>(I.avg_x, j) / I.D[j];
+      const double x = I.xk[j] + xh / dp;
+      I.xk[j] = jl_min(I.ru[j], jl_max(I.rl[j], x));
+    }
+  } else {
+    const double ddl = C->ref_dd;
+    for (int i = r0 + (int)threadIdx.x; i < r1; i += kThreads) {
+      const double yh = ld<T>(I.avg_y, i) / I.E[i];
+      double y = I.yk[i] + yh / ddl;
+      if (i >= I.neq) y = jl_max(y, 0.0);
+      I.yk[i] = y;
+    }
+  }
+}
+
+// raty = A^T yk and rax = A xk with the unscaled FP64 matrix.
+__global__ __launch_bounds__(kThreads) void k_ref_spmv(const BlockDesc *__restrict__ pblocks,
+                                                       int np,
+                                                       const BlockDesc *__restrict__ dblocks,
+                                                       const LongRow *__restrict__ longs,
+                                                       int32_t *__restrict__ long_cnt,
+                                                       double *__restrict__ long_part,
+                                                       const InstDev *__restrict__ insts,
+                                                       const Ctl *__restrict__ ctl) {
+  __shared__ double lds_p[kBlockNnz];
+  __shared__ double lds_q[kBlockNnz];
+  __shared__ double rowsum[kBlockRows];
+  __shared__ double scratch[2 * kThreads / 64];
+  const bool nside = (int)blockIdx.x < np;
+  const int blk = nside ? (int)blockIdx.x : (int)blockIdx.x - np;
+  const BlockDesc bd = nside ? pblocks[blk] : dblocks[blk];
+  if (ctl[bd.inst].ref_flag != 1) return;
+  const InstDev &I = insts[bd.inst];
+  const int32_t *ptr = nside ? I.t_ptr : I.a_ptr;
+  const int32_t *col = nside ? I.t_col : I.a_col;
+  const double *val = nside ? I.t_orig : I.a_orig;
+  const double *xv = nside ? I.yk : I.xk;
+  double *outv = nside ? I.raty : I.rax;
+  if (bd.nrows >= 0) {
+    rowblock_sums(ptr, col, val, xv, bd.row0, bd.nrows, lds_p, lds_q, rowsum);
+    for (int r = threadIdx.x; r < bd.nrows; r += kThreads) outv[bd.row0 + r] = rowsum[r];
+  } else {
+    const int chunk = -bd.nrows - 1;
+    const DD sm = longrow_chunk(ptr, col, val, xv, bd.row0, chunk, scratch);
+    if (threadIdx.x == 0) {
+      double tot;
+      if (longrow_publish(longs[bd.aux], chunk, bd.aux, long_cnt, long_part, sm, &tot))
+        outv[bd.row0] = tot;
+    }
+  }
+}
+
+// KKT statistics of (xk, yk) for the true problem, plus the norms the next
+// correction LP needs.  N: 0 sum lv^2+uv^2, 1 max viol, 2 c'x, 3 max|x|, 4 sum x^2,
+// 5 sum rv^2, 6 max|rv|, 7 sum contributions, 8 sum (rv/D)^2 (dual violation,
+// scaled space); M: 0 sum cv^2, 1 max|cv|, 2 sum dres^2, 3 max dres, 4 b'y,
+// 5 max|y|, 6 sum y^2, 7 sum (cv/E)^2 (primal violation, scaled space),
+// 8 sum (dres E)^2.
+__global__ __launch_bounds__(kThreads) void k_ref_stats(const BlockDesc *__restrict__ pblocks,
+                                                        int np,
+                                                        const BlockDesc *__restrict__ dblocks,
+                                                        const InstDev *__restrict__ insts,
+                                                        const Ctl *__restrict__ ctl,
+                                                        double *__restrict__ epn,
+                                                        double *__restrict__ epm) {
+  __shared__ double lds[2 * kRefN * (kThreads / 64)];
+  const bool nside = (int)blockIdx.x < np;
+  const int blk = nside ? (int)blockIdx.x : (int)blockIdx.x - np;
+  const BlockDesc bd = nside ? pblocks[blk] : dblocks[blk];
+  if (ctl[bd.inst].ref_flag != 1) return;
+  const InstDev &I = insts[bd.inst];
+  int r0, r1;
+  block_rows(bd, &r0, &r1);
+  if (nside) {
+    DD acc[kRefN];
+#pragma unroll
+    for (int k = 0; k < kRefN; ++k) acc[k] = dd_zero();
+    for (int j = r0 + (int)threadIdx.x; j < r1; j += kThreads) {
+      const double x = I.xk[j], l = I.rl[j], u = I.ru[j], c = I.rc[j];
+      const double fl = isfinite(l) ? 1.0 : 0.0, fu = isfinite(u) ? 1.0 : 0.0;
+      const double lvv = jl_max(l - x, 0.0), uvv = jl_max(x - u, 0.0);
+      dd_add_prod(acc[0], lvv, lvv);
+      dd_add_prod(acc[0], uvv, uvv);
+      acc[1].s = jl_max(acc[1].s, jl_max(lvv, uvv));
+      dd_add_prod(acc[2], c, x);
+      acc[3].s = jl_max(acc[3].s, fabs(x));
+      dd_add_prod(acc[4], x, x);
+      const double g = c - I.raty[j];
+      const double r = jl_max(g, 0.0) * fl + jl_min(g, 0.0) * fu;
+      const double rv = g - r;
+      dd_add_prod(acc[5], rv, rv);
+      acc[6].s = jl_max(acc[6].s, fabs(rv));
+      dd_add(acc[7], contrib(l, u, r));
+      const double rvs = rv / I.D[j];  // the violation in the scaled space
+      dd_add_prod(acc[8], rvs, rvs);
+    }
+    block_reduce_acc<kRefN>(acc, kRefMaxN, lds, epn + (size_t)blk * 2 * kRefN);
+  } else {
+    DD acc[kRefM];
+#pragma unroll
+    for (int k = 0; k < kRefM; ++k) acc[k] = dd_zero();
+    const int neq = I.neq;
+    for (int i = r0 + (int)threadIdx.x; i < r1; i += kThreads) {
+      const double y = I.yk[i], b = I.rb[i];
+      const double t = b - I.rax[i];
+      const double cvv = i < neq ? t : jl_max(t, 0.0);
+      dd_add_prod(acc[0], cvv, cvv);
+      acc[1].s = jl_max(acc[1].s, fabs(cvv));
+      if (i >= neq) {
+        const double dr = jl_max(-y, 0.0);
+        dd_add_prod(acc[2], dr, dr);
+        acc[3].s = jl_max(acc[3].s, dr);
+      }
+      dd_add_prod(acc[4], b, y);
+      acc[5].s = jl_max(acc[5].s, fabs(y));
+      dd_add_prod(acc[6], y, y);
+      const double cs = cvv / I.E[i];  // violations in the scaled space
+      dd_add_prod(acc[7], cs, cs);
+      if (i >= neq) {
+        const double ds = jl_max(-y, 0.0) * I.E[i];
+        dd_add_prod(acc[8], ds, ds);
+      }
+    }
+    block_reduce_acc<kRefM>(acc, kRefMaxM, lds, epm + (size_t)blk * 2 * kRefM);
+  }
+}
+
+// Data of the next correction LP (original and scaled) and a fresh PDHG state.
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_ref_setup(const BlockDesc *__restrict__ pblocks,
+                                                        int np,
+                                                        const BlockDesc *__restrict__ dblocks,
+                                                        const InstDev *__restrict__ insts,
+                                                        const Ctl *__restrict__ ctl) {
+  const bool nside = (int)blockIdx.x < np;
+  const int blk = nside ? (int)blockIdx.x : (int)blockIdx.x - np;
+  const BlockDesc bd = nside ? pblocks[blk] : dblocks[blk];
+  const Ctl *C = &ctl[bd.inst];
+  if (C->ref_flag != 2) return;
+  const InstDev &I = insts[bd.inst];
+  const double dp = C->ref_dp, ddl = C->ref_dd;
+  int r0, r1;
+  block_rows(bd, &r0, &r1);
+  const bool keep = C->ref_keep != 0;  // (xk, yk) is the best refined point so far
+  if (nside) {
+    for (int j = r0 + (int)threadIdx.x; j < r1; j += kThreads) {
+      const double x = I.xk[j];
+      if (keep) I.xb[j] = x;
+      const double ch = ddl * (I.rc[j] - I.raty[j]);
+      const double lh = dp * (I.rl[j] - x), uh = dp * (I.ru[j] - x);
+      ((double *)I.oc)[j] = ch;
+      ((double *)I.ol)[j] = lh;
+      ((double *)I.ou)[j] = uh;
+      const double Dj = I.D[j];
+      const double cs = ch / Dj;
+      st<T>((void *)I.c, j, cs);
+      st<T>((void *)I.l, j, lh * Dj);
+      st<T>((void *)I.u, j, uh * Dj);
+      st<T>(I.gbuf, j, cs);
+      st<T>(I.lg, j, cs);
+      void *zero[] = {I.x0, I.x1, I.aty, I.dx, I.avg_x, I.avg_g, I.lx};
+#pragma unroll
+      for (int k = 0; k < 7; ++k) st<T>(zero[k], j, 0.0);
+      ((double *)I.sx)[j] = 0.0;
+      ((double *)I.saty)[j] = 0.0;
+    }
+  } else {
+    const int neq = I.neq;
+    for (int i = r0 + (int)threadIdx.x; i < r1; i += kThreads) {
+      if (keep) I.yb[i] = I.yk[i];
+      const double bh = dp * (I.rb[i] - I.rax[i]);
+      ((double *)I.ob)[i] = bh;
+      const double Ei = I.E[i];
+      st<T>((void *)I.b, i, bh / Ei);
+      const double yl = i >= neq ? -ddl * I.yk[i] : 0.0;
+      ((double *)I.oylo)[i] = yl;
+      st<T>(I.ylo, i, yl * Ei);
+      void *zero[] = {I.y0, I.y1, I.avg_y, I.avg_ax, I.ly, I.lax};
+#pragma unroll
+      for (int k = 0; k < 6; ++k) st<T>(zero[k], i, 0.0);
+      void *zero64[] = {I.ax0, I.ax1, I.sy, I.sax};
+#pragma unroll
+      for (int k = 0; k < 4; ++k) ((double *)zero64[k])[i] = 0.0;
     }
   }
 }
@@ -868,7 +1121,7 @@ int launch_eval(hipStream_t s, int prec, int np, const BlockDesc *pblocks, int n
   }
   if (ninst > 0)
     hipLaunchKernelGGL(k_eval_finish, dim3(grid_of(ninst), kEvalOut), dim3(kThreads), 0, s, insts,
-                       ctl, epn, epm, out);
+                       ctl, epn, epm, out, kEvalN, kEvalM, kEvalMaxN, kEvalMaxM, kEvalStride);
   if (np > 0) {
     if (prec == 0)
       hipLaunchKernelGGL(k_eval_ray<double>, dim3(grid_of(np)), dim3(kThreads), 0, s, pblocks,
@@ -893,6 +1146,36 @@ int launch_restart(hipStream_t s, int prec, int np, const BlockDesc *pblocks, in
     hipLaunchKernelGGL(k_restart<float>, dim3(grid_of(np + nd)), dim3(kThreads), 0, s, pblocks,
                        np, dblocks, insts, ctl);
   PDLP_CHECK_LAUNCH();
+}
+
+}  // namespace pdlp
+
+namespace pdlp {
+
+int launch_refine(hipStream_t s, int prec, int phase, int np, const BlockDesc *pblocks, int nd,
+                  const BlockDesc *dblocks, const LongRow *longs, int32_t *long_cnt,
+                  double *long_part, const InstDev *insts, const Ctl *ctl, double *epn,
+                  double *epm, int ninst, double *out) {
+  const dim3 g(grid_of(np + nd)), b(kThreads);
+  if (np + nd <= 0) return 0;
+  if (phase == 1) {
+    if (prec == 0)
+      hipLaunchKernelGGL(k_ref_update<double>, g, b, 0, s, pblocks, np, dblocks, insts, ctl);
+    else
+      hipLaunchKernelGGL(k_ref_update<float>, g, b, 0, s, pblocks, np, dblocks, insts, ctl);
+    hipLaunchKernelGGL(k_ref_spmv, g, b, 0, s, pblocks, np, dblocks, longs, long_cnt, long_part,
+                       insts, ctl);
+    hipLaunchKernelGGL(k_ref_stats, g, b, 0, s, pblocks, np, dblocks, insts, ctl, epn, epm);
+    hipLaunchKernelGGL(k_eval_finish, dim3(grid_of(ninst), kRefN + kRefM), dim3(kThreads), 0, s,
+                       insts, ctl, epn, epm, out, kRefN, kRefM, kRefMaxN, kRefMaxM,
+                       kRefN + kRefM);
+  } else {
+    if (prec == 0)
+      hipLaunchKernelGGL(k_ref_setup<double>, g, b, 0, s, pblocks, np, dblocks, insts, ctl);
+    else
+      hipLaunchKernelGGL(k_ref_setup<float>, g, b, 0, s, pblocks, np, dblocks, insts, ctl);
+  }
+  return (int)hipGetLastError();
 }
 
 }  // namespace pdlp

@@ -32,6 +32,7 @@ struct Prepared {
   std::vector<double> oc, ol, ou, ob;
   QpCache qc{};
   double max_abs_scaled = 0.0;  // norm(scaled A, Inf) (elementwise)
+  std::vector<double> t_orig_host;  // unscaled CSC values (host-prepared FP32 path)
   bool device_prep = false;     // rescaling and the CSR of A are built on the device
   double norm_b_scaled = 0.0, norm_c_scaled = 0.0;  // set by the device preparation
 };

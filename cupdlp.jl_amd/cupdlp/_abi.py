@@ -64,7 +64,8 @@ class pdlp_params(C.Structure):
         ("refine_inner_tolerance", C.c_double),
         ("refine_max_scale", C.c_double),
         ("device_preconditioning", C.c_int32),
-        ("reserved", C.c_int32 * 7),
+        ("refine_inner_iteration_limit", C.c_int32),
+        ("reserved", C.c_int32 * 6),
     ]
 
 

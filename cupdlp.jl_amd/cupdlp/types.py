@@ -217,6 +217,7 @@ class PdhgParameters:
     refine_inner_tolerance: float = 1.0e-4
     refine_max_scale: float = 1.0e6
     device_preconditioning: bool = True
+    refine_inner_iteration_limit: int = 0
 
 
 @dataclass

@@ -141,7 +141,9 @@ typedef struct pdlp_params {
   double refine_inner_tolerance;     /* FP32_REFINE: tolerance of each FP32 solve */
   double refine_max_scale;           /* FP32_REFINE: cap of the per-round scale-up */
   int32_t device_preconditioning;    /* 1: Ruiz/Pock-Chambolle on the GPU */
-  int32_t reserved[7];
+  int32_t refine_inner_iteration_limit; /* FP32_REFINE: iteration cap of each correction
+                                           solve; 0 = max(10000, 20 x the first solve's) */
+  int32_t reserved[6];
 } pdlp_params;
 
 /* ConvergenceInformation — src/solve_log.jl:51-155 (same field order). */

@@ -1,0 +1,120 @@
+"""FP32 + FP64 iterative refinement on the GPU.
+
+The refined point is checked in FP64 by the CPU oracle's statistics of the
+original problem (the reference's optimality test, src/termination.jl:142-170,
+with eps_abs = eps_rel = tol) and against HiGHS objectives; batching must not
+change any instance's result (bitwise).
+"""
+import json
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import lp_cases
+import oracle as O
+from cupdlp import (PdhgParameters, Precision, TerminationReason as TR,
+                   construct_termination_criteria)
+from cupdlp.generators import config_lp, random_lp, transportation_lp
+
+pytestmark = pytest.mark.gpu
+HERE = os.path.dirname(os.path.abspath(__file__))
+with open(os.path.join(HERE, "golden", "highs_objectives.json")) as f:
+    HIGHS = json.load(f)["objectives"]
+
+
+def params(tol, time_limit=120.0, **kw):
+    # the time limit only guards the test run; every case here finishes in seconds
+    tc = construct_termination_criteria(eps_optimal_absolute=tol, eps_optimal_relative=tol,
+                                        time_sec_limit=time_limit)
+    return PdhgParameters(termination_criteria=tc, precision=Precision.FP32_REFINE, **kw)
+
+
+def say(m):
+    print(m, flush=True)
+    sys.stdout.flush()
+
+
+def check_kkt(prob, out, tol):
+    ci, _ = O.point_stats(prob, out.primal_solution, out.dual_solution, tol, tol)
+    b = np.linalg.norm(prob.right_hand_side)
+    c = np.linalg.norm(prob.objective_vector)
+    gap = abs(ci.primal_objective - ci.dual_objective)
+    assert ci.l2_primal_residual < tol * (1 + b), ci.l2_primal_residual
+    assert ci.l2_dual_residual < tol * (1 + c), ci.l2_dual_residual
+    assert gap < tol * (1 + abs(ci.primal_objective) + abs(ci.dual_objective)), gap
+    return ci
+
+
+@pytest.mark.parametrize("tol", [1e-4, 1e-8])
+def test_refined_c0(gpu_ctx, tol):
+    say(f"case: n=2000 LP, FP32 + refinement to {tol}")
+    prob, cert = config_lp(0)
+    out = gpu_ctx.solve(prob, params(tol))
+    assert out.termination_reason == TR.TERMINATION_REASON_OPTIMAL
+    ci = check_kkt(prob, out, tol)
+    h = HIGHS["c0_seed0"]["objective"]
+    assert abs(ci.primal_objective - h) <= 10 * tol * (1 + abs(h))
+    say(f"  iterations {out.iteration_count}, refinement rounds {out.refinement_rounds}")
+
+
+def test_refined_transportation(gpu_ctx):
+    say("case: 60 x 60 transportation, FP32 + refinement to 1e-8")
+    prob = transportation_lp(60, 60, seed=5)
+    out = gpu_ctx.solve(prob, params(1e-8))
+    say(f"  {out.termination_string}, iterations {out.iteration_count}, "
+        f"refinement rounds {out.refinement_rounds}, {out.solve_time_sec:.2f} s")
+    assert out.termination_reason == TR.TERMINATION_REASON_OPTIMAL
+    ci = check_kkt(prob, out, 1e-8)
+    h = HIGHS["transport_60x60_s5"]["objective"]
+    assert abs(ci.primal_objective - h) <= 1e-7 * abs(h)
+    assert out.refinement_rounds <= 3
+
+
+def test_inner_iteration_cap_and_best_point(gpu_ctx):
+    """Correction solves capped at 64 iterations: the solve still ends, and the
+    returned point is the one its final statistics describe."""
+    say("case: n=2000 LP, FP32 + refinement, inner cap 64, 6 rounds")
+    prob, cert = config_lp(0)
+    out = gpu_ctx.solve(prob, params(1e-8, refine_inner_iteration_limit=64, refine_max_outer=6))
+    say(f"  {out.termination_string}, iterations {out.iteration_count}, "
+        f"rounds {out.refinement_rounds}")
+    assert out.termination_reason in (TR.TERMINATION_REASON_OPTIMAL,
+                                      TR.TERMINATION_REASON_ITERATION_LIMIT)
+    assert out.refinement_rounds <= 6
+    ci, _ = O.point_stats(prob, out.primal_solution, out.dual_solution, 1e-8, 1e-8)
+    assert out.iteration_stats
+    fs = out.iteration_stats[-1].convergence_information[0]
+    for k in ("l2_primal_residual", "l2_dual_residual", "primal_objective", "dual_objective"):
+        a, b = getattr(fs, k), getattr(ci, k)
+        assert abs(a - b) <= 1e-9 * (1 + abs(b)), (k, a, b)
+
+
+def test_refined_small_cases(gpu_ctx):
+    say("case: tiny LPs, FP32 + refinement")
+    prob, opt = lp_cases.tiny_two_var()
+    out = gpu_ctx.solve(prob, params(1e-8))
+    assert out.termination_reason == TR.TERMINATION_REASON_OPTIMAL
+    np.testing.assert_allclose(out.primal_solution, [2.0, 0.0], atol=1e-7)
+    out = gpu_ctx.solve(lp_cases.primal_infeasible(), params(1e-8))
+    assert out.termination_reason == TR.TERMINATION_REASON_PRIMAL_INFEASIBLE
+    prob, opt = lp_cases.long_row_lp(6000)
+    out = gpu_ctx.solve(prob, params(1e-8))
+    assert out.termination_reason == TR.TERMINATION_REASON_OPTIMAL
+    check_kkt(prob, out, 1e-8)
+
+
+def test_refined_batch_alone_vs_mixed(gpu_ctx):
+    say("case: FP32 + refinement, batch of 4 vs alone (bitwise)")
+    probs = [random_lp(n=n, m=n // 2, seed=s)[0] for n, s in ((300, 21), (2000, 22), (9000, 23), (700, 24))]
+    p = params(1e-8)
+    alone = [gpu_ctx.solve(q, p) for q in probs]
+    mixed = gpu_ctx.batch_solve(probs, p)
+    for prob, a, b in zip(probs, alone, mixed):
+        assert a.termination_reason == b.termination_reason == TR.TERMINATION_REASON_OPTIMAL
+        assert a.iteration_count == b.iteration_count
+        assert a.refinement_rounds == b.refinement_rounds
+        assert np.array_equal(a.primal_solution, b.primal_solution)
+        assert np.array_equal(a.dual_solution, b.dual_solution)
+        check_kkt(prob, b, 1e-8)
