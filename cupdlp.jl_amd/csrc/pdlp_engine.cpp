@@ -612,10 +612,13 @@ int Engine::run_attempts(int64_t count, std::string *err) {
   return 0;
 }
 
+// gap_norm >= 0 replaces |primal objective| + |dual objective| in the gap test
+// (correction solves of the refinement path).
 static int check_termination(const pdlp_termination_criteria &tc, const QpCache &qc,
-                             const pdlp_iteration_stats &st) {
+                             const pdlp_iteration_stats &st, double gap_norm = -1.0) {
   const pdlp_convergence_information &ci = st.convergence_information;
-  const double abs_obj = fabs(ci.primal_objective) + fabs(ci.dual_objective);
+  const double abs_obj = gap_norm >= 0.0 ? gap_norm
+                                         : fabs(ci.primal_objective) + fabs(ci.dual_objective);
   const double gap = fabs(ci.primal_objective - ci.dual_objective);
   double pe, pb, de, db;
   if (tc.optimality_norm == PDLP_L_INF) {
@@ -713,7 +716,7 @@ void Engine::host_eval_logic(size_t q, const double *o) {
   st.time_spent_doing_basic_algorithm = h.basic_time;
   st.total_attempts = C.attempts;
 
-  int reason = check_termination(tc, qc, st);
+  int reason = check_termination(tc, qc, st, h.gap_norm_cur);
   if (C.numerical_error && reason == 0) reason = PDLP_TERMINATION_REASON_NUMERICAL_ERROR;
   // a correction solve that has not met its tolerance within its cap is
   // refined anyway (the outer rounds are bounded by refine_max_outer)
@@ -1087,6 +1090,11 @@ void Engine::refine_logic(size_t q, const double *o) {
   h.qc_cur.l2_b = dp * sqrt(M[0] + N[0]);
   h.qc_cur.l_inf_c = ddn * edl;
   h.qc_cur.l2_c = ddn * sqrt(N[5] + M[2]);
+  // The correction's gap is its objective error in units scaled by dP dD:
+  // test it against the current gap of (xk, yk) in those units, not against
+  // the correction's own objective values (small, so that test would ask for
+  // far more accuracy than the refined point can use).
+  h.gap_norm_cur = dp * ddn * gap;
   const double nc = ddn * sqrt(N[8] + M[8]), nbn = dp * sqrt(M[7]);
   double omega0 = prm_.primal_importance;
   if (prm_.scale_invariant_initial_primal_weight && nc > 0.0 && nbn > 0.0)

@@ -70,6 +70,7 @@ struct HostInst {
   bool refined = false;              // (xk, yk) holds a refined point
   double dp = 1.0, dd = 1.0;         // scale factors of the current correction LP
   int64_t inner_cap = 0;             // iteration cap of the current correction solve (0 none)
+  double gap_norm_cur = -1.0;        // gap-test normalisation of the current solve (-1: |p|+|d|)
   double best_err = INFINITY;        // KKT error of the best refined point (xb, yb)
   pdlp_iteration_stats best_stats{};
   double last_err = INFINITY;        // KKT error of (xk, yk)
