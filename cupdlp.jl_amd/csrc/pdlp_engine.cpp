@@ -526,12 +526,13 @@ int Engine::run_attempts(int64_t count, std::string *err) {
   int rc = ensure_step_table(max_att + count + 2, err);
   if (rc) return rc;
   const bool prof = prof_ && prof_->enabled;
-  // Algorithmic bytes per launch of each instance (an accepted attempt: the
-  // gathered vector read once, the pending sums read and written):
-  //   FP64  primal 12N + 4(n+1) + 8m + 88n   (x c l u sx saty in; aty x' dx sx saty out)
-  //         dual   12N + 4(m+1) + 8n + 72m   (y ax b sy sax in; y' ax' sy sax out)
-  //   FP32  primal  8N + 4(n+1) + 4m + 56n   (sums FP64; no dx)
-  //         dual    8N + 4(m+1) + 4n + 60m + 4(m - neq)   (products, sums FP64; ylo)
+  // Algorithmic bytes per launch of each instance: the matrix with its
+  // indices, the row pointers, the gathered vector read once, and nine fused
+  // vector streams per row (primal: x c l u in, x' out, the two weighted sums
+  // read and written; dual: y b Ax in, y' Ax' out, two sums read and written):
+  //   N (s_A + 4) + 4 (rows + 1) + s_v (gathered) + 9 s_v rows,
+  // with s_A = s_v = 8 (FP64) or 4 (FP32).  Traffic the engine adds beyond
+  // this (A^T y and dx stored, FP64 sums on the FP32 path) is not counted.
   std::vector<double> pbytes, dbytes;
   std::vector<int64_t> remaining;
   if (prof) {
@@ -544,14 +545,9 @@ int Engine::run_attempts(int64_t count, std::string *err) {
     for (size_t q = 0; q < inst_.size(); ++q) {
       const Prepared &P = inst_[q]->P;
       const double N = (double)P.nnz, n = (double)P.n, m = (double)P.m;
-      const double mi = (double)(P.m - P.neq);
-      if (prec_) {
-        pbytes.push_back(8 * N + 4 * (n + 1) + 4 * m + 56 * n);
-        dbytes.push_back(8 * N + 4 * (m + 1) + 4 * n + 60 * m + 4 * mi);
-      } else {
-        pbytes.push_back(12 * N + 4 * (n + 1) + 8 * m + 88 * n);
-        dbytes.push_back(12 * N + 4 * (m + 1) + 8 * n + 72 * m);
-      }
+      const double sv = prec_ ? 4.0 : 8.0;
+      pbytes.push_back(N * (sv + 4) + 4 * (n + 1) + sv * m + 9 * sv * n);
+      dbytes.push_back(N * (sv + 4) + 4 * (m + 1) + sv * n + 9 * sv * m);
       remaining.push_back(inst_[q]->done ? 0 : ctl_[q].target - ctl_[q].accepted);
     }
   }

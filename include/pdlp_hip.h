@@ -263,8 +263,9 @@ int pdlp_serve_close(pdlp_server *srv);
 /* Timing of the two iteration kernels (extension, for benchmarks): when
  * enabled, every k_primal / k_dual launch of pdlp_solve / pdlp_batch_solve is
  * bracketed by HIP events on the context's stream.  Bytes are the algorithmic
- * bytes of each launch: N(s_A+4) + 4(rows+1) + s_v*(gathered length) + 9*s_v*rows
- * summed over the instances active in that launch. */
+ * bytes of each launch, summed over the instances active in it:
+ * N(s_A+4) + 4(rows+1) + s_v*(gathered length) + 9*s_v*rows, with
+ * s_A = s_v = 8 (FP64) or 4 (FP32_REFINE). */
 typedef struct pdlp_kernel_profile {
   int64_t primal_launches;
   int64_t dual_launches;
