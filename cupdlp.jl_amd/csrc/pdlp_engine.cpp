@@ -890,11 +890,16 @@ int Engine::finish_instance(size_t q, std::string *err) {
   auto val = [&](const std::vector<char> &b, size_t i) -> double {
     return prec_ ? (double)((const float *)b.data())[i] : ((const double *)b.data())[i];
   };
+  // device entry j is the caller's entry col_perm[j] (rows: row_perm)
+  const int32_t *cp = h.P.col_perm.data(), *rp = h.P.row_perm.data();
   if (h.refined) {  // the refined FP64 point, or the best one if the last round made it worse
     const bool best = h.last_err > h.best_err;
-    if (n) HIPCHK(hipMemcpyAsync(r->x, best ? h.dev.xb : h.dev.xk, 8 * n, hipMemcpyDeviceToHost, s_));
-    if (m) HIPCHK(hipMemcpyAsync(r->y, best ? h.dev.yb : h.dev.yk, 8 * m, hipMemcpyDeviceToHost, s_));
+    std::vector<double> hx(std::max<size_t>(n, 1)), hy(std::max<size_t>(m, 1));
+    if (n) HIPCHK(hipMemcpyAsync(hx.data(), best ? h.dev.xb : h.dev.xk, 8 * n, hipMemcpyDeviceToHost, s_));
+    if (m) HIPCHK(hipMemcpyAsync(hy.data(), best ? h.dev.yb : h.dev.yk, 8 * m, hipMemcpyDeviceToHost, s_));
     HIPCHK(hipStreamSynchronize(s_));
+    for (size_t j = 0; j < n; ++j) r->x[cp[j]] = hx[j];
+    for (size_t i = 0; i < m; ++i) r->y[rp[i]] = hy[i];
     if (r->x_current && n) memcpy(r->x_current, r->x, 8 * n);
     if (r->y_current && m) memcpy(r->y_current, r->y, 8 * m);
     r->termination_reason = h.reason;
@@ -908,16 +913,16 @@ int Engine::finish_instance(size_t q, std::string *err) {
   HIPCHK(fetch(h.dev.avg_x, bx, n));
   HIPCHK(fetch(h.dev.avg_y, by, m));
   HIPCHK(hipStreamSynchronize(s_));
-  for (size_t j = 0; j < n; ++j) r->x[j] = val(bx, j) / h.P.D[j];
-  for (size_t i = 0; i < m; ++i) r->y[i] = val(by, i) / h.P.E[i];
+  for (size_t j = 0; j < n; ++j) r->x[cp[j]] = val(bx, j) / h.P.D[j];
+  for (size_t i = 0; i < m; ++i) r->y[rp[i]] = val(by, i) / h.P.E[i];
   if (r->x_current || r->y_current) {
     HIPCHK(fetch(C.cur ? h.dev.x1 : h.dev.x0, bx, n));
     HIPCHK(fetch(C.cur ? h.dev.y1 : h.dev.y0, by, m));
     HIPCHK(hipStreamSynchronize(s_));
     if (r->x_current)
-      for (size_t j = 0; j < n; ++j) r->x_current[j] = val(bx, j) / h.P.D[j];
+      for (size_t j = 0; j < n; ++j) r->x_current[cp[j]] = val(bx, j) / h.P.D[j];
     if (r->y_current)
-      for (size_t i = 0; i < m; ++i) r->y_current[i] = val(by, i) / h.P.E[i];
+      for (size_t i = 0; i < m; ++i) r->y_current[rp[i]] = val(by, i) / h.P.E[i];
   }
   r->termination_reason = h.reason;
   r->solve_time_sec = now_sec() - h.t_submit;

@@ -96,20 +96,55 @@ static double norminf(const std::vector<double> &v) {
 int load_problem(const pdlp_lp *lp, Prepared *P, std::string *err, bool build_csr) {
   int rc = validate_lp(lp, err);
   if (rc) return rc;
-  const int64_t n = lp->n, m = lp->m, nnz = lp->nnz;
-  P->n = n; P->m = m; P->neq = lp->num_eq; P->nnz = nnz; P->c0 = lp->c0;
+  const int64_t n = lp->n, m = lp->m, nnz = lp->nnz, neq = lp->num_eq;
+  P->n = n; P->m = m; P->neq = neq; P->nnz = nnz; P->c0 = lp->c0;
+  // Device numbering: columns by nonzero count (descending), rows by nonzero
+  // count (descending) within the equality and the ">=" group, ties in the
+  // original order.  The vectors the SpMVs gather from then keep their most
+  // used entries together in few cache lines.  Every sum the engine forms is
+  // exactly rounded, so the numbering changes no computed value; results are
+  // mapped back to the caller's numbering.
+  std::vector<int64_t> rcount(m, 0);
+  for (int64_t k = 0; k < nnz; ++k) rcount[lp->rowval[k] - 1]++;
+  // stable counting sorts by descending count
+  auto order = [](int64_t lo, int64_t hi, auto count_of, int32_t *out) {
+    int64_t mx = 0;
+    for (int64_t i = lo; i < hi; ++i) mx = std::max<int64_t>(mx, count_of(i));
+    std::vector<int64_t> start(mx + 2, 0);
+    for (int64_t i = lo; i < hi; ++i) start[mx - count_of(i) + 1]++;
+    for (int64_t c = 0; c <= mx; ++c) start[c + 1] += start[c];
+    for (int64_t i = lo; i < hi; ++i) out[lo + start[mx - count_of(i)]++] = (int32_t)i;
+  };
+  P->col_perm.resize(n);
+  P->row_perm.resize(m);
+  order(0, n, [&](int64_t j) { return (int64_t)(lp->colptr[j + 1] - lp->colptr[j]); },
+        P->col_perm.data());
+  auto rc_of = [&](int64_t i) { return rcount[i]; };
+  order(0, neq, rc_of, P->row_perm.data());
+  order(neq, m, rc_of, P->row_perm.data());
+  std::vector<int32_t> new_row(m);
+  for (int64_t i = 0; i < m; ++i) new_row[P->row_perm[i]] = (int32_t)i;
   P->t_ptr.resize(n + 1);
-  for (int64_t j = 0; j <= n; ++j) P->t_ptr[j] = (int32_t)(lp->colptr[j] - 1);
   P->t_col.resize(nnz);
   P->t_val.resize(nnz);
-  for (int64_t k = 0; k < nnz; ++k) {
-    P->t_col[k] = (int32_t)(lp->rowval[k] - 1);
-    P->t_val[k] = lp->nzval[k];
+  P->t_ptr[0] = 0;
+  for (int64_t jn = 0; jn < n; ++jn) {
+    const int64_t j = P->col_perm[jn];
+    int32_t q = P->t_ptr[jn];
+    for (int64_t k = lp->colptr[j] - 1; k < lp->colptr[j + 1] - 1; ++k, ++q) {
+      P->t_col[q] = new_row[lp->rowval[k] - 1];
+      P->t_val[q] = lp->nzval[k];
+    }
+    P->t_ptr[jn + 1] = q;
   }
-  P->oc.assign(lp->c, lp->c + n);
-  P->ol.assign(lp->lb, lp->lb + n);
-  P->ou.assign(lp->ub, lp->ub + n);
-  P->ob.assign(lp->b, lp->b + m);
+  P->oc.resize(n); P->ol.resize(n); P->ou.resize(n); P->ob.resize(m);
+  for (int64_t jn = 0; jn < n; ++jn) {
+    const int64_t j = P->col_perm[jn];
+    P->oc[jn] = lp->c[j];
+    P->ol[jn] = lp->lb[j];
+    P->ou[jn] = lp->ub[j];
+  }
+  for (int64_t i = 0; i < m; ++i) P->ob[i] = lp->b[P->row_perm[i]];
   P->c = P->oc; P->l = P->ol; P->u = P->ou; P->b = P->ob;
   P->E.assign(m, 1.0);
   P->D.assign(n, 1.0);

@@ -33,6 +33,9 @@ struct Prepared {
   QpCache qc{};
   double max_abs_scaled = 0.0;  // norm(scaled A, Inf) (elementwise)
   std::vector<double> t_orig_host;  // unscaled CSC values (host-prepared FP32 path)
+  // device numbering -> the caller's: column j here is column col_perm[j] of
+  // the input, row i is row row_perm[i]
+  std::vector<int32_t> col_perm, row_perm;
   bool device_prep = false;     // rescaling and the CSR of A are built on the device
   double norm_b_scaled = 0.0, norm_c_scaled = 0.0;  // set by the device preparation
 };
