@@ -20,7 +20,9 @@ Reported beside it:
     sample of the same LPs on this host's cores.
 
 Other workloads: --workload c1 (n = 1,000,000 LP, fixed iteration count, for
-the FP64 SpMV roofline), --workload c0 (n = 2,000 LP).
+the FP64 SpMV roofline), --workload c2 (2,000 x 2,000 transportation LP),
+--workload c0 (n = 2,000 LP), --workload copy (a plain 1 GiB device copy, the
+calibration kernel for the rocprofv3 FETCH_SIZE / WRITE_SIZE counters).
 """
 import argparse
 import json
@@ -180,12 +182,34 @@ def run_single(args, ctx, lib, which):
     return args.steps / dt, dt, cfg, extra, roof, [prob]
 
 
+def run_copy(args):
+    """A plain device-to-device copy of a known size (1 GiB, HBM-resident),
+    the calibration kernel for the FETCH_SIZE / WRITE_SIZE counters."""
+    import torch
+    nbytes = 1 << 30
+    src = torch.empty(nbytes // 8, dtype=torch.float64, device="cuda:0").uniform_()
+    dst = torch.empty_like(src)
+    for _ in range(args.warmup):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    ev0.record()
+    for _ in range(args.steps):
+        dst.copy_(src)
+    ev1.record()
+    torch.cuda.synchronize()
+    ms = ev0.elapsed_time(ev1) / args.steps
+    gbs = 2 * nbytes / (ms * 1e-3) / 1e9
+    print(json.dumps({"workload": "copy", "bytes_read": nbytes, "bytes_written": nbytes,
+                      "avg_ms": round(ms, 4), "GBs": round(gbs, 1)}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", choices=("batch", "c1", "c0"), default="batch")
+    ap.add_argument("--workload", choices=("batch", "c1", "c2", "c0", "copy"), default="batch")
     ap.add_argument("--precision", choices=("fp64", "fp32"), default="fp64")
     ap.add_argument("--tol", type=float, default=1e-8)
     ap.add_argument("--count", type=int, default=512)
@@ -200,6 +224,9 @@ def main():
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
               "(the engine runs on a single device)", file=sys.stderr)
         sys.exit(2)
+    if args.workload == "copy":
+        run_copy(args)
+        return
     from cupdlp.solver import Context
     ctx = Context(0)
     lib = ctx.lib
@@ -207,7 +234,7 @@ def main():
         value, dt, cfg, extra, roof, problems = run_batch(args, ctx, lib)
         metric_unit = "solves/s"
     else:
-        value, dt, cfg, extra, roof, problems = run_single(args, ctx, lib, 1 if args.workload == "c1" else 0)
+        value, dt, cfg, extra, roof, problems = run_single(args, ctx, lib, int(args.workload[1]))
         metric_unit = "solves/s"
     cpu = None
     if not args.no_cpu_baseline:

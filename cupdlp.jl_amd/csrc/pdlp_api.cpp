@@ -309,6 +309,7 @@ static size_t footprint(const Prepared &P, int prec) {
   const size_t ts = prec ? 4 : 8;
   size_t b = (size_t)P.nnz * 2 * (4 + ts) + (size_t)(P.n + P.m + 2) * 4 + ts * (14 * P.n + 11 * P.m) +
              8 * (4 * P.n + 2 * P.m) + 64 * 1024;
+  b += (size_t)(P.ps.slots + P.ds.slots) * (4 + ts);  // sliced copies of A^T and A
   // FP32 + refinement: the FP64 matrix twice, ylo (T and FP64) and the
   // refinement vectors (6 n-vectors, 4 m-vectors)
   if (prec) b += (size_t)P.nnz * 16 + (ts + 8) * P.m + 8 * (6 * P.n + 4 * P.m) + 4 * (2 * P.n + 4 * P.m);

@@ -4,6 +4,7 @@
 // Layout in HBM (per LP "instance"; a single solve is a batch of one):
 //   A   (m x n) CSR : a_ptr[m+1] int32, a_col[nnz] int32, a_val[nnz] T
 //   A^T (n x m) CSR : t_ptr[n+1] int32, t_col[nnz] int32, t_val[nnz] T
+//   sliced A, A^T   : as_col/ts_col int32, as_val/ts_val T (see SliceDesc)
 //   scaled data     : c, l, u (n) and b (m), type T
 //   original data   : oc, ol, ou (n), ob (m), D (n), E (m), double
 //   iterates        : x[2], aty, dx (n); y[2], ax[2] (m)          (T)
@@ -33,6 +34,29 @@ constexpr int kEvalM = 22;
 constexpr int kEvalOut = kEvalN + kEvalM;  // index of the primal ray objective
 constexpr int kEvalStride = kEvalOut + 1;   // doubles per instance in the output
 
+// Sliced layout of the step kernels (one wave per slice, no LDS, no
+// barriers).  A slice covers `nrows` consecutive rows with 2^lg lanes per
+// row; entry k of the slice's row g sits at slot off + (k >> lg) * 64 +
+// (g << lg) + (k & (2^lg - 1)), so each load instruction of a wave reads 64
+// consecutive slots.  Unused slots hold column -1.  Rows of up to
+// 64 * kSellWidth entries get the fewest lanes (a power of two) for which a
+// lane sums at most kSellWidth entries; longer rows get a whole wave, and a
+// row longer than kSellChunk entries is cut into chunk slices (nrows = 0)
+// whose partial sums the last chunk combines in chunk order.
+constexpr int kSellWidth = 32;
+constexpr int kSellChunk = 16384;
+
+struct SliceDesc {
+  int32_t inst;   // instance index
+  int32_t row0;   // first row (chunk slices: the long row)
+  int32_t off;    // first slot in the instance's sliced arrays
+  int32_t width;  // slots per lane
+  int32_t lg;     // log2 of the lanes per row (chunk slices: 6)
+  int32_t nrows;  // rows in the slice; 0 = chunk slice
+  int32_t lid;    // chunk slices: long-row id
+  int32_t chunk;  // chunk slices: chunk index
+};
+
 struct BlockDesc {
   int32_t inst;   // instance index
   int32_t row0;   // first row (or the long row)
@@ -60,6 +84,12 @@ struct InstDev {
   void *avg_x, *avg_g, *gbuf, *lx, *lg;
   void *avg_y, *avg_ax, *ly, *lax;
   const double *a_orig, *t_orig;  // FP32 path: unscaled FP64 values (CSR, CSC order)
+  // sliced copies of A (dual step) and A^T (primal step), and the ranges of
+  // this instance in the slice tables (step partials are per slice)
+  const int32_t *as_col, *ts_col;
+  const void *as_val, *ts_val;
+  int32_t ps0, ps1, ds0, ds1;  // normal slices
+  int32_t pl0, pl1, dl0, dl1;  // chunk slices (indices into the same partial arrays)
   // FP32 + refinement: dual lower bounds of ">=" rows (scaled T, original FP64;
   // null = 0), the refined FP64 point, its products, and the true problem data
   void *ylo;

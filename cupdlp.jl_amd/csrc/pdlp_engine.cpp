@@ -142,6 +142,10 @@ int Engine::upload(HostInst &h, std::string *err) {
   Carver cv;
   const size_t o_aptr = cv.take(4 * (m + 1)), o_acol = cv.take(4 * nnz), o_aval = cv.take(ts * nnz);
   const size_t o_tptr = cv.take(4 * (n + 1)), o_tcol = cv.take(4 * nnz), o_tval = cv.take(ts * nnz);
+  // sliced copies of A^T and A for the step kernels
+  const size_t tsl = (size_t)P.ps.slots, asl = (size_t)P.ds.slots;
+  const size_t o_tscol = cv.take(4 * tsl), o_tsval = cv.take(ts * tsl);
+  const size_t o_ascol = cv.take(4 * asl), o_asval = cv.take(ts * asl);
   // n-vectors: c l u x0 x1 aty dx sx saty avg_x avg_g gbuf lx lg (T, except
   // the weighted sums sx, saty: FP64 on both paths)
   size_t o_n[14];
@@ -184,6 +188,8 @@ int Engine::upload(HostInst &h, std::string *err) {
   d.a_val = base + o_aval;
   d.t_ptr = (const int32_t *)(base + o_tptr); d.t_col = (const int32_t *)(base + o_tcol);
   d.t_val = base + o_tval;
+  d.ts_col = (const int32_t *)(base + o_tscol); d.ts_val = base + o_tsval;
+  d.as_col = (const int32_t *)(base + o_ascol); d.as_val = base + o_asval;
   d.c = base + o_n[0]; d.l = base + o_n[1]; d.u = base + o_n[2];
   d.x0 = base + o_n[3]; d.x1 = base + o_n[4]; d.aty = base + o_n[5]; d.dx = base + o_n[6];
   d.sx = base + o_n[7]; d.saty = base + o_n[8]; d.avg_x = base + o_n[9]; d.avg_g = base + o_n[10];
@@ -297,7 +303,7 @@ int Engine::upload(HostInst &h, std::string *err) {
     P.norm_c_scaled = po.norm_c;
     // host copies of the scaled data are not needed any more
     std::vector<double>().swap(P.t_val);
-    return 0;
+    return fill_slices(h, err);
   }
 
   // host-prepared problem: upload the scaled data
@@ -330,6 +336,36 @@ int Engine::upload(HostInst &h, std::string *err) {
   HIPCHK(put((void *)d.D, P.D.data(), 8 * n));
   HIPCHK(put((void *)d.E, P.E.data(), 8 * m));
   HIPCHK(hipStreamSynchronize(s_));
+  return fill_slices(h, err);
+}
+
+// Builds the sliced copies of the scaled A^T and A from the device CSR.
+int Engine::fill_slices(HostInst &h, std::string *err) {
+  const Prepared &P = h.P;
+  const InstDev &d = h.dev;
+  const size_t np = P.ps.sl.size(), nd = P.ds.sl.size();
+  if (np + nd == 0) return 0;
+  SliceDesc *tab = (SliceDesc *)pool_->get(sizeof(SliceDesc) * (np + nd));
+  if (!tab) {
+    if (err) *err = "device allocation of the slice tables failed";
+    return -2;
+  }
+  hipError_t e = hipSuccess;
+  if (np) e = hipMemcpyAsync(tab, P.ps.sl.data(), sizeof(SliceDesc) * np, hipMemcpyHostToDevice, s_);
+  if (e == hipSuccess && nd)
+    e = hipMemcpyAsync(tab + np, P.ds.sl.data(), sizeof(SliceDesc) * nd, hipMemcpyHostToDevice, s_);
+  if (e == hipSuccess)
+    e = (hipError_t)launch_slice_fill(s_, prec_, (int)np, tab, d.t_ptr, d.t_col, d.t_val,
+                                      (int32_t *)d.ts_col, (void *)d.ts_val);
+  if (e == hipSuccess)
+    e = (hipError_t)launch_slice_fill(s_, prec_, (int)nd, tab + np, d.a_ptr, d.a_col, d.a_val,
+                                      (int32_t *)d.as_col, (void *)d.as_val);
+  if (e == hipSuccess) e = hipStreamSynchronize(s_);
+  pool_->put(tab);
+  if (e != hipSuccess) {
+    if (err) *err = std::string("building the sliced matrices failed: ") + hipGetErrorString(e);
+    return -(int)e;
+  }
   return 0;
 }
 
@@ -421,22 +457,71 @@ int Engine::rebuild_tables(std::string *err) {
   std::vector<BlockDesc> pb, db;
   std::vector<LongRow> longs;
   int nslots = 0;
+  // Finished instances get no blocks: launches then cost only what the
+  // running instances need.  An instance's own blocks do not depend on where
+  // it sits in the tables, so dropping others changes none of its sums.
   for (size_t q = 0; q < inst_.size(); ++q) {
     HostInst &h = *inst_[q];
     h.dev.pb0 = (int32_t)pb.size();
-    partition_rows(h.P.t_ptr, h.P.n, (int)q, pb, longs, &nslots);
+    if (!h.done) partition_rows(h.P.t_ptr, h.P.n, (int)q, pb, longs, &nslots);
     h.dev.pb1 = (int32_t)pb.size();
   }
-  const int nlong_p = (int)longs.size();
   for (size_t q = 0; q < inst_.size(); ++q) {
     HostInst &h = *inst_[q];
     h.dev.db0 = (int32_t)db.size();
-    partition_rows(h.P.a_ptr, h.P.m, (int)q, db, longs, &nslots);
+    if (!h.done) partition_rows(h.P.a_ptr, h.P.m, (int)q, db, longs, &nslots);
     h.dev.db1 = (int32_t)db.size();
   }
-  (void)nlong_p;
+  // slice tables of the step kernels: normal slices, then the chunk slices
+  // of long rows (with LongRow entries after the row blocks' ones); the
+  // chunk slices' partials follow the normal slices' in the partial arrays
+  std::vector<SliceDesc> ps, ds, pls, dls;
+  auto add_slices = [&](const SliceTable &st, int q, std::vector<SliceDesc> &out,
+                        std::vector<SliceDesc> &lout) {
+    const int lid0 = (int)longs.size();
+    for (int32_t nch : st.nchunks) {
+      longs.push_back({nslots, nch});
+      nslots += nch;
+    }
+    for (SliceDesc sd : st.sl) {
+      sd.inst = q;
+      if (sd.nrows == 0) {
+        sd.lid += lid0;
+        lout.push_back(sd);
+      } else {
+        out.push_back(sd);
+      }
+    }
+  };
+  for (size_t q = 0; q < inst_.size(); ++q) {
+    HostInst &h = *inst_[q];
+    h.dev.ps0 = (int32_t)ps.size();
+    h.dev.pl0 = (int32_t)pls.size();
+    h.dev.ds0 = (int32_t)ds.size();
+    h.dev.dl0 = (int32_t)dls.size();
+    if (!h.done) {
+      add_slices(h.P.ps, (int)q, ps, pls);
+      add_slices(h.P.ds, (int)q, ds, dls);
+    }
+    h.dev.ps1 = (int32_t)ps.size();
+    h.dev.pl1 = (int32_t)pls.size();
+    h.dev.ds1 = (int32_t)ds.size();
+    h.dev.dl1 = (int32_t)dls.size();
+  }
+  for (auto &h : inst_) {
+    h->dev.pl0 += (int32_t)ps.size();
+    h->dev.pl1 += (int32_t)ps.size();
+    h->dev.dl0 += (int32_t)ds.size();
+    h->dev.dl1 += (int32_t)ds.size();
+  }
+  npl_ = (int)pls.size();
+  ndl_ = (int)dls.size();
+  ps.insert(ps.end(), pls.begin(), pls.end());  // one upload: normal slices, then chunks
+  ds.insert(ds.end(), dls.begin(), dls.end());
   np_ = (int)pb.size();
   nd_ = (int)db.size();
+  nps_ = (int)ps.size() - npl_;
+  nds_ = (int)ds.size() - ndl_;
   nlong_ = (int)longs.size();
   nslots_ = nslots;
   const size_t ninst = inst_.size();
@@ -445,6 +530,8 @@ int Engine::rebuild_tables(std::string *err) {
     if (p) table_allocs_.push_back(p);
     return p;
   };
+  d_pslices_ = (SliceDesc *)alloc(sizeof(SliceDesc) * ps.size());
+  d_dslices_ = (SliceDesc *)alloc(sizeof(SliceDesc) * ds.size());
   d_pblocks_ = (BlockDesc *)alloc(sizeof(BlockDesc) * pb.size());
   d_dblocks_ = (BlockDesc *)alloc(sizeof(BlockDesc) * db.size());
   d_longs_ = (LongRow *)alloc(sizeof(LongRow) * longs.size());
@@ -453,19 +540,23 @@ int Engine::rebuild_tables(std::string *err) {
   d_insts_ = (InstDev *)alloc(sizeof(InstDev) * ninst);
   d_ctl_ = (Ctl *)alloc(sizeof(Ctl) * ninst);
   // compensated partial sums: (s, e) pairs
-  d_ppart_ = (double *)alloc(sizeof(double) * 2 * pb.size());
-  d_dpart_ = (double *)alloc(sizeof(double) * 4 * db.size());
+  d_ppart_ = (double *)alloc(sizeof(double) * 2 * ps.size());
+  d_dpart_ = (double *)alloc(sizeof(double) * 4 * ds.size());
   d_epn_ = (double *)alloc(sizeof(double) * 2 * kEvalN * pb.size());
   d_epm_ = (double *)alloc(sizeof(double) * 2 * kEvalM * db.size());
   d_epr_ = (double *)alloc(sizeof(double) * 2 * pb.size());
   d_out_ = (double *)alloc(sizeof(double) * kEvalStride * ninst);
   d_step_cnt_ = (int32_t *)alloc(sizeof(int32_t) * ninst);
-  if (!d_pblocks_ || !d_dblocks_ || !d_longs_ || !d_long_cnt_ || !d_long_part_ || !d_insts_ ||
+  if (!d_pslices_ || !d_dslices_ || !d_pblocks_ || !d_dblocks_ || !d_longs_ || !d_long_cnt_ || !d_long_part_ || !d_insts_ ||
       !d_ctl_ || !d_ppart_ || !d_dpart_ || !d_epn_ || !d_epm_ || !d_epr_ || !d_out_ ||
       !d_step_cnt_) {
     if (err) *err = "device allocation of block tables failed";
     return -2;
   }
+  if (!ps.empty())
+    HIPCHK(hipMemcpyAsync(d_pslices_, ps.data(), sizeof(SliceDesc) * ps.size(), hipMemcpyHostToDevice, s_));
+  if (!ds.empty())
+    HIPCHK(hipMemcpyAsync(d_dslices_, ds.data(), sizeof(SliceDesc) * ds.size(), hipMemcpyHostToDevice, s_));
   if (!pb.empty())
     HIPCHK(hipMemcpyAsync(d_pblocks_, pb.data(), sizeof(BlockDesc) * pb.size(), hipMemcpyHostToDevice, s_));
   if (!db.empty())
@@ -554,12 +645,12 @@ int Engine::run_attempts(int64_t count, std::string *err) {
   const double t0 = now_sec();
   for (int64_t a = 0; a < count; ++a) {
     if (prof) HIPCHK(hipEventRecord(prof_->events[3 * a + 0], s_));
-    int e = launch_primal(s_, prec_, false, np_, d_pblocks_, d_longs_, d_long_cnt_, d_long_part_,
-                          d_insts_, d_ctl_, d_ppart_);
+    int e = launch_primal(s_, prec_, nps_, d_pslices_, npl_, d_pslices_ + nps_, d_longs_,
+                          d_long_cnt_, d_long_part_, d_insts_, d_ctl_, d_ppart_);
     if (prof) HIPCHK(hipEventRecord(prof_->events[3 * a + 1], s_));
     if (!e)
-      e = launch_dual(s_, prec_, false, nd_, d_dblocks_, d_longs_, d_long_cnt_, d_long_part_,
-                      d_insts_, d_ctl_, d_dpart_);
+      e = launch_dual(s_, prec_, nds_, d_dslices_, ndl_, d_dslices_ + nds_, d_longs_,
+                      d_long_cnt_, d_long_part_, d_insts_, d_ctl_, d_dpart_);
     if (prof) HIPCHK(hipEventRecord(prof_->events[3 * a + 2], s_));
     if (!e)
       e = launch_step(s_, (int)inst_.size(), d_insts_, d_ctl_, d_ppart_, d_dpart_, d_red_, d_grow_,
@@ -839,11 +930,9 @@ int Engine::evaluate(std::string *err) {
   }
   if (neval == 0) return 0;
   HIPCHK(push_ctl());
-  int e = launch_primal(s_, prec_, true, np_, d_pblocks_, d_longs_, d_long_cnt_, d_long_part_,
-                        d_insts_, d_ctl_, d_ppart_);
-  if (!e)
-    e = launch_dual(s_, prec_, true, nd_, d_dblocks_, d_longs_, d_long_cnt_, d_long_part_,
-                    d_insts_, d_ctl_, d_dpart_);
+  int e = launch_primal_flush(s_, prec_, np_, d_pblocks_, d_longs_, d_long_cnt_, d_long_part_,
+                              d_insts_, d_ctl_);
+  if (!e) e = launch_dual_flush(s_, prec_, nd_, d_dblocks_, d_insts_, d_ctl_);
   if (!e)
     e = launch_eval(s_, prec_, np_, d_pblocks_, nd_, d_dblocks_, d_insts_, d_ctl_, d_epn_, d_epm_,
                     d_epr_, (int)ninst, d_out_);
@@ -1137,6 +1226,11 @@ void Engine::reset_inner(size_t q, double omega0) {
 }
 
 int Engine::step_round(std::string *err) {
+  // Rebuild the block tables once finished instances hold a quarter of them.
+  int64_t dead = 0;
+  for (auto &h : inst_)
+    if (h->done) dead += (h->dev.pb1 - h->dev.pb0) + (h->dev.db1 - h->dev.db0);
+  if (4 * dead > (int64_t)np_ + nd_) tables_dirty_ = true;
   if (tables_dirty_) {
     int rc = rebuild_tables(err);
     if (rc) return rc;

@@ -104,6 +104,7 @@ class Engine {
 
  private:
   int upload(HostInst &h, std::string *err);
+  int fill_slices(HostInst &h, std::string *err);
   int rebuild_tables(std::string *err);
   hipError_t push_ctl();
   int ensure_step_table(int64_t need, std::string *err);
@@ -131,7 +132,8 @@ class Engine {
   size_t out_cap_ = 0;
   // device tables
   bool tables_dirty_ = true;
-  int np_ = 0, nd_ = 0, nlong_ = 0, nslots_ = 0;
+  int np_ = 0, nd_ = 0, nps_ = 0, nds_ = 0, npl_ = 0, ndl_ = 0, nlong_ = 0, nslots_ = 0;
+  SliceDesc *d_pslices_ = nullptr, *d_dslices_ = nullptr;
   BlockDesc *d_pblocks_ = nullptr, *d_dblocks_ = nullptr;
   LongRow *d_longs_ = nullptr;
   int32_t *d_long_cnt_ = nullptr;

@@ -9,12 +9,27 @@
 namespace pdlp {
 
 // prec: 0 = FP64 iterates/matrix, 1 = FP32.
-int launch_primal(hipStream_t s, int prec, bool flush, int nblocks, const BlockDesc *blocks,
-                  const LongRow *longs, int32_t *long_cnt, double *long_part,
-                  const InstDev *insts, const Ctl *ctl, double *part);
-int launch_dual(hipStream_t s, int prec, bool flush, int nblocks, const BlockDesc *blocks,
-                const LongRow *longs, int32_t *long_cnt, double *long_part, const InstDev *insts,
-                const Ctl *ctl, double *part);
+// Step kernels over the slice tables (one wave per slice): normal slices,
+// then the chunk slices of long rows (their partials follow the normal
+// slices' in part).
+int launch_primal(hipStream_t s, int prec, int nslices, const SliceDesc *slices, int nlong,
+                  const SliceDesc *lslices, const LongRow *longs, int32_t *long_cnt,
+                  double *long_part, const InstDev *insts, const Ctl *ctl, double *part);
+int launch_dual(hipStream_t s, int prec, int nslices, const SliceDesc *slices, int nlong,
+                const SliceDesc *lslices, const LongRow *longs, int32_t *long_cnt,
+                double *long_part, const InstDev *insts, const Ctl *ctl, double *part);
+// Evaluation preparation over the row-block tables: pending weighted sums,
+// and A^T y of the current y when it is not valid.
+int launch_primal_flush(hipStream_t s, int prec, int nblocks, const BlockDesc *blocks,
+                        const LongRow *longs, int32_t *long_cnt, double *long_part,
+                        const InstDev *insts, const Ctl *ctl);
+int launch_dual_flush(hipStream_t s, int prec, int nblocks, const BlockDesc *blocks,
+                      const InstDev *insts, const Ctl *ctl);
+// Sliced copy (scol, sval) of the CSR matrix (ptr, col, val) for the given
+// instance-local slices.
+int launch_slice_fill(hipStream_t s, int prec, int nslices, const SliceDesc *slices,
+                      const int32_t *ptr, const int32_t *col, const void *val, int32_t *scol,
+                      void *sval);
 int launch_step(hipStream_t s, int ninst, const InstDev *insts, Ctl *ctl, const double *ppart,
                 const double *dpart, const double *red_tab, const double *grow_tab,
                 int64_t tab_len, double red_exp, double grow_exp);

@@ -30,6 +30,7 @@
 #include <math.h>
 #include <stdint.h>
 
+#include <algorithm>
 #include <type_traits>
 
 #include "pdlp_device.h"
@@ -214,67 +215,39 @@ __device__ __forceinline__ bool longrow_publish(const LongRow lr, int chunk, int
 }
 
 // ---------------------------------------------------------------------------
-// Primal half-step over the rows of A^T.
-// FLUSH: only the pending weighted sums and A^T y (evaluation preparation).
-// part[2b..2b+1]: compensated sum of dx^2 of block b.
-template <typename T, bool FLUSH>
-__global__ __launch_bounds__(kThreads) void k_primal(const BlockDesc *__restrict__ blocks,
-                                                     const LongRow *__restrict__ longs,
-                                                     int32_t *__restrict__ long_cnt,
-                                                     double *__restrict__ long_part,
-                                                     const InstDev *__restrict__ insts,
-                                                     const Ctl *__restrict__ ctl,
-                                                     double *__restrict__ part) {
+// Evaluation preparation on the A^T row blocks: adds the pending weighted
+// sums of x and A^T y and forms A^T y of the current y when it is not valid.
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_primal_flush(const BlockDesc *__restrict__ blocks,
+                                                           const LongRow *__restrict__ longs,
+                                                           int32_t *__restrict__ long_cnt,
+                                                           double *__restrict__ long_part,
+                                                           const InstDev *__restrict__ insts,
+                                                           const Ctl *__restrict__ ctl) {
   __shared__ double lds_p[kBlockNnz];
   __shared__ double lds_q[kBlockNnz];
   __shared__ double rowsum[kBlockRows];
   __shared__ double scratch[2 * kThreads / 64];
   const BlockDesc bd = blocks[blockIdx.x];
   const Ctl *C = &ctl[bd.inst];
-  const bool run = FLUSH ? (C->eval_flag != 0) : (C->active != 0);
-  if (!run) return;
+  if (C->eval_flag == 0) return;
   const InstDev &I = insts[bd.inst];
   const int cur = C->cur;
   const bool pending = C->pending != 0;
   const bool need = C->aty_valid == 0;
   const double w = C->w_pending;
-  const double tau = C->eta / C->omega;
   const T *__restrict__ yv = (const T *)(cur ? I.y1 : I.y0);
   const T *__restrict__ xv = (const T *)(cur ? I.x1 : I.x0);
-  T *__restrict__ xn = (T *)(cur ? I.x0 : I.x1);
   T *__restrict__ aty = (T *)I.aty;
-  T *__restrict__ dx = (T *)I.dx;
   double *__restrict__ sx = (double *)I.sx;
   double *__restrict__ saty = (double *)I.saty;
-  const T *__restrict__ cv = (const T *)I.c;
-  const T *__restrict__ lv = (const T *)I.l;
-  const T *__restrict__ uv = (const T *)I.u;
-  DD acc = dd_zero();
   auto epi = [&](int j, double s) {
     const double a = need ? s : (double)aty[j];
-    const double xj = (double)xv[j];
     if (pending) {
-      sx[j] = sx[j] + xj * w;
+      sx[j] = sx[j] + (double)xv[j] * w;
       saty[j] = saty[j] + a * w;
     }
     if (need) aty[j] = (T)a;
-    if (!FLUSH) {
-      double t = xj - tau * ((double)cv[j] - a);
-      t = jl_min((double)uv[j], jl_max((double)lv[j], t));
-      if constexpr (std::is_same<T, double>::value) {
-        const double d = t - xj;
-        dx[j] = d;
-        xn[j] = xj + d;
-        dd_add_prod(acc, d, d);
-      } else {
-        // FP32 iterates: the step is the change of the stored value (the
-        // dual kernel multiplies A by the new point itself, not by dx)
-        const T xr = (T)t;
-        const double d = (double)xr - xj;
-        xn[j] = xr;
-        dd_add_prod(acc, d, d);
-      }
-    }
   };
   if (bd.nrows >= 0) {
     if (need)
@@ -294,12 +267,207 @@ __global__ __launch_bounds__(kThreads) void k_primal(const BlockDesc *__restrict
       }
     }
   }
-  if (!FLUSH) {
-    const DD t = block_dd(acc, scratch);
-    if (threadIdx.x == 0) {
-      part[2 * blockIdx.x] = t.s;
-      part[2 * blockIdx.x + 1] = t.e;
+}
+
+// One lane's share of a slice row: entries off + lane + 64 k, k < width,
+// gathered from xv, as a compensated sum.  Padding slots (column -1) are
+// skipped.  Entries go in groups of 8 whose index and value loads are all
+// issued before their gathers; the last group is predicated.
+template <typename T, typename V>
+__device__ __forceinline__ DD slice_lane_sum(const int32_t *__restrict__ scol,
+                                             const T *__restrict__ sval,
+                                             const V *__restrict__ xv, int off, int width,
+                                             int lane) {
+  constexpr int U = 8;
+  DD acc = dd_zero();
+  const int32_t *cp = scol + off + lane;
+  const T *vp = sval + off + lane;
+  int k = 0;
+  for (; k + U <= width; k += U) {
+    int32_t c[U];
+    double v[U], g[U];
+#pragma unroll
+    for (int i = 0; i < U; ++i) {
+      c[i] = __builtin_nontemporal_load(cp + 64 * (k + i));
+      v[i] = (double)__builtin_nontemporal_load(vp + 64 * (k + i));
     }
+#pragma unroll
+    for (int i = 0; i < U; ++i) g[i] = c[i] >= 0 ? (double)xv[c[i]] : 0.0;
+#pragma unroll
+    for (int i = 0; i < U; ++i)
+      if (c[i] >= 0) dd_add_prod(acc, v[i], g[i]);
+  }
+  if (k < width) {
+    int32_t c[U];
+    double v[U], g[U];
+#pragma unroll
+    for (int i = 0; i < U; ++i) {
+      const bool in = k + i < width;
+      c[i] = in ? __builtin_nontemporal_load(cp + 64 * (k + i)) : -1;
+      v[i] = in ? (double)__builtin_nontemporal_load(vp + 64 * (k + i)) : 0.0;
+    }
+#pragma unroll
+    for (int i = 0; i < U; ++i) g[i] = c[i] >= 0 ? (double)xv[c[i]] : 0.0;
+#pragma unroll
+    for (int i = 0; i < U; ++i)
+      if (c[i] >= 0) dd_add_prod(acc, v[i], g[i]);
+  }
+  return acc;
+}
+
+// Compensated sum over each group of 2^lg lanes (every lane of the group
+// holds the result), rounded once.
+__device__ __forceinline__ double group_sum(DD acc, int lg) {
+  for (int o = (1 << lg) >> 1; o > 0; o >>= 1) {
+    const DD w{__shfl_xor(acc.s, o, 64), __shfl_xor(acc.e, o, 64)};
+    dd_merge(acc, w);
+  }
+  return dd_val(acc);
+}
+
+// Per-launch view of one instance for the primal step.
+template <typename T>
+struct PrimalView {
+  const T *yv, *xv, *cv, *lv, *uv;
+  T *xn, *aty, *dx;
+  double *sx, *saty;
+  bool pending, need;
+  double w, tau;
+  __device__ PrimalView(const InstDev &I, const Ctl *C) {
+    const int cur = C->cur;
+    pending = C->pending != 0;
+    need = C->aty_valid == 0;
+    w = C->w_pending;
+    tau = C->eta / C->omega;
+    yv = (const T *)(cur ? I.y1 : I.y0);
+    xv = (const T *)(cur ? I.x1 : I.x0);
+    xn = (T *)(cur ? I.x0 : I.x1);
+    aty = (T *)I.aty;
+    dx = (T *)I.dx;
+    sx = (double *)I.sx;
+    saty = (double *)I.saty;
+    cv = (const T *)I.c;
+    lv = (const T *)I.l;
+    uv = (const T *)I.u;
+  }
+};
+
+// Operands of one primal row.
+struct PrimalRow {
+  double x, c, l, u, sx, saty, aty;
+};
+
+template <typename T>
+__device__ __forceinline__ PrimalRow primal_load(const PrimalView<T> &v, int j) {
+  PrimalRow r;
+  r.x = (double)v.xv[j];
+  r.c = (double)v.cv[j];
+  r.l = (double)v.lv[j];
+  r.u = (double)v.uv[j];
+  r.sx = v.pending ? v.sx[j] : 0.0;
+  r.saty = v.pending ? v.saty[j] : 0.0;
+  r.aty = v.need ? 0.0 : (double)v.aty[j];
+  return r;
+}
+
+// The pending weighted-sum update and the projected primal step (K1,
+// src/primal_dual_hybrid_gradient_gpu.jl:177-197) of row j, given the row
+// sum s of A^T y (used when A^T y is recomputed); adds dx^2 to acc.
+template <typename T>
+__device__ __forceinline__ void primal_update(const PrimalView<T> &v, int j, const PrimalRow &r,
+                                              double s, DD &acc) {
+  const double a = v.need ? s : r.aty;
+  if (v.pending) {
+    v.sx[j] = r.sx + r.x * v.w;
+    v.saty[j] = r.saty + a * v.w;
+  }
+  if (v.need) v.aty[j] = (T)a;
+  double t = r.x - v.tau * (r.c - a);
+  t = jl_min(r.u, jl_max(r.l, t));
+  if constexpr (std::is_same<T, double>::value) {
+    const double d = t - r.x;
+    v.dx[j] = d;
+    v.xn[j] = r.x + d;
+    dd_add_prod(acc, d, d);
+  } else {
+    // FP32 iterates: the step is the change of the stored value (the dual
+    // kernel multiplies A by the new point itself, not by dx)
+    const T xr = (T)t;
+    const double d = (double)xr - r.x;
+    v.xn[j] = xr;
+    dd_add_prod(acc, d, d);
+  }
+}
+
+// Primal half-step over the slices of A^T: A^T y (only when y changed), the
+// pending weighted-sum update of x and A^T y, the projected primal step and
+// the partial sum of dx^2 of each slice (part[2w], part[2w+1] for slice w).
+// One wave per slice; each row's operands are loaded before the gathers of
+// its sum.
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_primal(const SliceDesc *__restrict__ slices, int nsl,
+                                                     const InstDev *__restrict__ insts,
+                                                     const Ctl *__restrict__ ctl,
+                                                     double *__restrict__ part) {
+  const int wv = (int)blockIdx.x * (kThreads / 64) + ((int)threadIdx.x >> 6);
+  if (wv >= nsl) return;
+  const int lane = threadIdx.x & 63;
+  const SliceDesc sd = slices[wv];
+  const Ctl *C = &ctl[sd.inst];
+  if (C->active == 0) return;
+  const InstDev &I = insts[sd.inst];
+  const PrimalView<T> v(I, C);
+  const int j = sd.row0 + lane;
+  const bool mine = lane < sd.nrows;
+  PrimalRow r{};
+  if (mine) r = primal_load(v, j);
+  double s = 0.0;
+  if (v.need) {
+    s = group_sum(slice_lane_sum(I.ts_col, (const T *)I.ts_val, v.yv, sd.off, sd.width, lane), sd.lg);
+    s = __shfl(s, (lane << sd.lg) & 63, 64);
+  }
+  DD acc = dd_zero();
+  if (mine) primal_update(v, j, r, s, acc);
+  acc = wave_dd(acc);
+  if (lane == 0) {
+    part[2 * wv] = acc.s;
+    part[2 * wv + 1] = acc.e;
+  }
+}
+
+// The chunk slices of rows longer than kSellChunk entries: each wave sums one
+// chunk; the last chunk of a row to finish combines the chunk sums in chunk
+// order and updates the row.  part as in k_primal.
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_primal_long(const SliceDesc *__restrict__ slices,
+                                                          int nsl, const LongRow *__restrict__ longs,
+                                                          int32_t *__restrict__ long_cnt,
+                                                          double *__restrict__ long_part,
+                                                          const InstDev *__restrict__ insts,
+                                                          const Ctl *__restrict__ ctl,
+                                                          double *__restrict__ part) {
+  const int wv = (int)blockIdx.x * (kThreads / 64) + ((int)threadIdx.x >> 6);
+  if (wv >= nsl) return;
+  const int lane = threadIdx.x & 63;
+  const SliceDesc sd = slices[wv];
+  const Ctl *C = &ctl[sd.inst];
+  if (C->active == 0) return;
+  const InstDev &I = insts[sd.inst];
+  const PrimalView<T> v(I, C);
+  DD acc = dd_zero();
+  if (v.need) {
+    const DD t = wave_dd(slice_lane_sum(I.ts_col, (const T *)I.ts_val, v.yv, sd.off, sd.width, lane));
+    if (lane == 0) {
+      double tot;
+      if (longrow_publish(longs[sd.lid], sd.chunk, sd.lid, long_cnt, long_part, t, &tot))
+        primal_update(v, sd.row0, primal_load(v, sd.row0), tot, acc);
+    }
+  } else if (sd.chunk == 0 && lane == 0) {
+    primal_update(v, sd.row0, primal_load(v, sd.row0), 0.0, acc);
+  }
+  if (lane == 0) {
+    part[2 * wv] = acc.s;
+    part[2 * wv + 1] = acc.e;
   }
 }
 
@@ -362,96 +530,175 @@ __device__ __forceinline__ void step_rule(Ctl *C, double ssx, double ssy, double
   }
 }
 
-// Dual half-step over the rows of A.  FLUSH: only the pending weighted sums.
-// part[4b..4b+3]: compensated sums of dy^2 and Adx*dy of block b.
-template <typename T, bool FLUSH>
-__global__ __launch_bounds__(kThreads) void k_dual(const BlockDesc *__restrict__ blocks,
-                                                   const LongRow *__restrict__ longs,
-                                                   int32_t *__restrict__ long_cnt,
-                                                   double *__restrict__ long_part,
+// Evaluation preparation on the A row blocks: adds the pending weighted sums
+// of y and A x.
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_dual_flush(const BlockDesc *__restrict__ blocks,
+                                                         const InstDev *__restrict__ insts,
+                                                         const Ctl *__restrict__ ctl) {
+  const BlockDesc bd = blocks[blockIdx.x];
+  const Ctl *C = &ctl[bd.inst];
+  if (C->eval_flag == 0 || C->pending == 0) return;
+  const InstDev &I = insts[bd.inst];
+  const int cur = C->cur;
+  const double w = C->w_pending;
+  const T *__restrict__ yv = (const T *)(cur ? I.y1 : I.y0);
+  const double *__restrict__ axv = (const double *)(cur ? I.ax1 : I.ax0);
+  double *__restrict__ sy = (double *)I.sy;
+  double *__restrict__ sax = (double *)I.sax;
+  int r0, r1;
+  if (bd.nrows >= 0) {
+    r0 = bd.row0;
+    r1 = bd.row0 + bd.nrows;
+  } else {
+    if (bd.nrows != -1) return;  // chunk 0 of a long row handles it
+    r0 = bd.row0;
+    r1 = bd.row0 + 1;
+  }
+  for (int i = r0 + (int)threadIdx.x; i < r1; i += kThreads) {
+    sy[i] = sy[i] + (double)yv[i] * w;
+    sax[i] = sax[i] + axv[i] * w;
+  }
+}
+
+// Per-launch view of one instance for the dual step.  The dual kernel
+// multiplies A by dx (FP64; the products are updated incrementally, as the
+// reference does) or by the new point x' (FP32, so that A x never drifts
+// from the stored point).
+template <typename T>
+struct DualView {
+  const T *yv, *bv, *dxv, *ylo;
+  T *yn;
+  const double *axv;
+  double *axn, *sy, *sax;
+  bool pending;
+  double w, sig;
+  int neq;
+  __device__ DualView(const InstDev &I, const Ctl *C) {
+    const int cur = C->cur;
+    pending = C->pending != 0;
+    w = C->w_pending;
+    sig = C->omega * C->eta;
+    neq = I.neq;
+    yv = (const T *)(cur ? I.y1 : I.y0);
+    yn = (T *)(cur ? I.y0 : I.y1);
+    axv = (const double *)(cur ? I.ax1 : I.ax0);
+    axn = (double *)(cur ? I.ax0 : I.ax1);
+    sy = (double *)I.sy;
+    sax = (double *)I.sax;
+    bv = (const T *)I.b;
+    dxv = (const T *)(std::is_same<T, float>::value ? (cur ? I.x0 : I.x1) : I.dx);
+    ylo = (const T *)I.ylo;
+  }
+};
+
+struct DualRow {
+  double y, ax, b, sy, sax, ylo;
+};
+
+template <typename T>
+__device__ __forceinline__ DualRow dual_load(const DualView<T> &v, int i) {
+  DualRow r;
+  r.y = (double)v.yv[i];
+  r.ax = v.axv[i];
+  r.b = (double)v.bv[i];
+  r.sy = v.pending ? v.sy[i] : 0.0;
+  r.sax = v.pending ? v.sax[i] : 0.0;
+  r.ylo = (v.ylo && i >= v.neq) ? (double)v.ylo[i] : 0.0;
+  return r;
+}
+
+// The pending weighted-sum update and the projected dual step with
+// extrapolation (K2, src/primal_dual_hybrid_gradient_gpu.jl:232-260) of row
+// i, given its product prod (A dx or A x'); adds dy^2 and Adx*dy.
+template <typename T>
+__device__ __forceinline__ void dual_update(const DualView<T> &v, int i, const DualRow &r,
+                                            double prod, DD &acc_dy, DD &acc_int) {
+  constexpr bool kF32 = std::is_same<T, float>::value;
+  if (v.pending) {
+    v.sy[i] = r.sy + r.y * v.w;
+    v.sax[i] = r.sax + r.ax * v.w;
+  }
+  const double adx = kF32 ? prod - r.ax : prod;
+  double t = r.y + v.sig * ((r.b - 2.0 * adx) - r.ax);
+  if (i >= v.neq) t = jl_max(t, r.ylo);
+  if constexpr (!kF32) {
+    const double d = t - r.y;
+    v.yn[i] = r.y + d;
+    v.axn[i] = r.ax + adx;
+    dd_add_prod(acc_dy, d, d);
+    dd_add_prod(acc_int, adx, d);
+  } else {
+    const T yr = (T)t;
+    const double d = (double)yr - r.y;
+    v.yn[i] = yr;
+    v.axn[i] = prod;
+    dd_add_prod(acc_dy, d, d);
+    dd_add_prod(acc_int, adx, d);
+  }
+}
+
+// Dual half-step over the slices of A: the product, the pending weighted-sum
+// update of y and A x, the projected dual step and the partial sums of dy^2
+// and Adx*dy of each slice (part[4w..4w+3]).
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_dual(const SliceDesc *__restrict__ slices, int nsl,
                                                    const InstDev *__restrict__ insts,
                                                    const Ctl *__restrict__ ctl,
                                                    double *__restrict__ part) {
-  __shared__ double lds_p[kBlockNnz];
-  __shared__ double lds_q[kBlockNnz];
-  __shared__ double rowsum[kBlockRows];
-  __shared__ double scratch[2 * kThreads / 64];
-  const BlockDesc bd = blocks[blockIdx.x];
-  const Ctl *C = &ctl[bd.inst];
-  const bool run = FLUSH ? (C->eval_flag != 0) : (C->active != 0);
-  if (!run) return;
-  const InstDev &I = insts[bd.inst];
-  const int cur = C->cur;
-  const bool pending = C->pending != 0;
-  const double w = C->w_pending;
-  const double sig = C->omega * C->eta;
-  const int neq = I.neq;
-  const T *__restrict__ yv = (const T *)(cur ? I.y1 : I.y0);
-  T *__restrict__ yn = (T *)(cur ? I.y0 : I.y1);
-  const double *__restrict__ axv = (const double *)(cur ? I.ax1 : I.ax0);
-  double *__restrict__ axn = (double *)(cur ? I.ax0 : I.ax1);
-  double *__restrict__ sy = (double *)I.sy;
-  double *__restrict__ sax = (double *)I.sax;
-  const T *__restrict__ bv = (const T *)I.b;
-  // FP64: A dx (the products are updated incrementally, as the reference
-  // does); FP32: A x_new, so that A x never drifts from the stored point
-  constexpr bool kF32 = std::is_same<T, float>::value;
-  const T *__restrict__ dxv = (const T *)(kF32 ? (cur ? I.x0 : I.x1) : I.dx);
-  const T *__restrict__ ylo = (const T *)I.ylo;
+  const int wv = (int)blockIdx.x * (kThreads / 64) + ((int)threadIdx.x >> 6);
+  if (wv >= nsl) return;
+  const int lane = threadIdx.x & 63;
+  const SliceDesc sd = slices[wv];
+  const Ctl *C = &ctl[sd.inst];
+  if (C->active == 0) return;
+  const InstDev &I = insts[sd.inst];
+  const DualView<T> v(I, C);
+  const int i = sd.row0 + lane;
+  const bool mine = lane < sd.nrows;
+  DualRow r{};
+  if (mine) r = dual_load(v, i);
+  double prod = group_sum(slice_lane_sum(I.as_col, (const T *)I.as_val, v.dxv, sd.off, sd.width, lane), sd.lg);
+  prod = __shfl(prod, (lane << sd.lg) & 63, 64);
   DD acc_dy = dd_zero(), acc_int = dd_zero();
-  auto epi = [&](int i, double prod) {
-    const double yi = (double)yv[i];
-    const double axi = axv[i];
-    if (pending) {
-      sy[i] = sy[i] + yi * w;
-      sax[i] = sax[i] + axi * w;
-    }
-    if (!FLUSH) {
-      const double adx = kF32 ? prod - axi : prod;
-      double t = yi + sig * (((double)bv[i] - 2.0 * adx) - axi);
-      if (i >= neq) t = jl_max(t, ylo ? (double)ylo[i] : 0.0);
-      if constexpr (!kF32) {
-        const double d = t - yi;
-        yn[i] = yi + d;
-        axn[i] = axi + adx;
-        dd_add_prod(acc_dy, d, d);
-        dd_add_prod(acc_int, adx, d);
-      } else {
-        const T yr = (T)t;
-        const double d = (double)yr - yi;
-        yn[i] = yr;
-        axn[i] = prod;
-        dd_add_prod(acc_dy, d, d);
-        dd_add_prod(acc_int, adx, d);
-      }
-    }
-  };
-  if (bd.nrows >= 0) {
-    if (!FLUSH)
-      rowblock_sums(I.a_ptr, I.a_col, (const T *)I.a_val, dxv, bd.row0, bd.nrows, lds_p, lds_q,
-                    rowsum);
-    for (int r = threadIdx.x; r < bd.nrows; r += kThreads) epi(bd.row0 + r, FLUSH ? 0.0 : rowsum[r]);
-  } else {
-    const int chunk = -bd.nrows - 1;
-    if (FLUSH) {
-      if (chunk == 0 && threadIdx.x == 0) epi(bd.row0, 0.0);
-    } else {
-      const DD sm = longrow_chunk(I.a_ptr, I.a_col, (const T *)I.a_val, dxv, bd.row0, chunk, scratch);
-      if (threadIdx.x == 0) {
-        double tot;
-        if (longrow_publish(longs[bd.aux], chunk, bd.aux, long_cnt, long_part, sm, &tot))
-          epi(bd.row0, tot);
-      }
-    }
+  if (mine) dual_update(v, i, r, prod, acc_dy, acc_int);
+  acc_dy = wave_dd(acc_dy);
+  acc_int = wave_dd(acc_int);
+  if (lane == 0) {
+    part[4 * wv] = acc_dy.s;
+    part[4 * wv + 1] = acc_dy.e;
+    part[4 * wv + 2] = acc_int.s;
+    part[4 * wv + 3] = acc_int.e;
   }
-  if (FLUSH) return;
-  const DD a = block_dd(acc_dy, scratch);
-  const DD b = block_dd(acc_int, scratch);
-  if (threadIdx.x == 0) {
-    part[4 * blockIdx.x] = a.s;
-    part[4 * blockIdx.x + 1] = a.e;
-    part[4 * blockIdx.x + 2] = b.s;
-    part[4 * blockIdx.x + 3] = b.e;
+}
+
+// Chunk slices of long rows of A (see k_primal_long).
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_dual_long(const SliceDesc *__restrict__ slices,
+                                                        int nsl, const LongRow *__restrict__ longs,
+                                                        int32_t *__restrict__ long_cnt,
+                                                        double *__restrict__ long_part,
+                                                        const InstDev *__restrict__ insts,
+                                                        const Ctl *__restrict__ ctl,
+                                                        double *__restrict__ part) {
+  const int wv = (int)blockIdx.x * (kThreads / 64) + ((int)threadIdx.x >> 6);
+  if (wv >= nsl) return;
+  const int lane = threadIdx.x & 63;
+  const SliceDesc sd = slices[wv];
+  const Ctl *C = &ctl[sd.inst];
+  if (C->active == 0) return;
+  const InstDev &I = insts[sd.inst];
+  const DualView<T> v(I, C);
+  const DD t = wave_dd(slice_lane_sum(I.as_col, (const T *)I.as_val, v.dxv, sd.off, sd.width, lane));
+  DD acc_dy = dd_zero(), acc_int = dd_zero();
+  if (lane == 0) {
+    double tot;
+    if (longrow_publish(longs[sd.lid], sd.chunk, sd.lid, long_cnt, long_part, t, &tot))
+      dual_update(v, sd.row0, dual_load(v, sd.row0), tot, acc_dy, acc_int);
+    part[4 * wv] = acc_dy.s;
+    part[4 * wv + 1] = acc_dy.e;
+    part[4 * wv + 2] = acc_int.s;
+    part[4 * wv + 3] = acc_int.e;
   }
 }
 
@@ -473,9 +720,15 @@ __global__ __launch_bounds__(kStepThreads) void k_step(const InstDev *__restrict
   if (!C->active) return;
   const InstDev &I = insts[q];
   DD sx = dd_zero(), sy = dd_zero(), sxy = dd_zero();
-  for (int b = I.pb0 + (int)threadIdx.x; b < I.pb1; b += kStepThreads)
+  for (int b = I.ps0 + (int)threadIdx.x; b < I.ps1; b += kStepThreads)
     dd_merge(sx, DD{ppart[2 * b], ppart[2 * b + 1]});
-  for (int b = I.db0 + (int)threadIdx.x; b < I.db1; b += kStepThreads) {
+  for (int b = I.pl0 + (int)threadIdx.x; b < I.pl1; b += kStepThreads)
+    dd_merge(sx, DD{ppart[2 * b], ppart[2 * b + 1]});
+  for (int b = I.ds0 + (int)threadIdx.x; b < I.ds1; b += kStepThreads) {
+    dd_merge(sy, DD{dpart[4 * b], dpart[4 * b + 1]});
+    dd_merge(sxy, DD{dpart[4 * b + 2], dpart[4 * b + 3]});
+  }
+  for (int b = I.dl0 + (int)threadIdx.x; b < I.dl1; b += kStepThreads) {
     dd_merge(sy, DD{dpart[4 * b], dpart[4 * b + 1]});
     dd_merge(sxy, DD{dpart[4 * b + 2], dpart[4 * b + 3]});
   }
@@ -1061,41 +1314,120 @@ static inline unsigned grid_of(int n) { return (unsigned)(n > 0 ? n : 0); }
 
 #define PDLP_CHECK_LAUNCH() return (int)hipGetLastError()
 
-int launch_primal(hipStream_t s, int prec, bool flush, int nblocks, const BlockDesc *blocks,
-                  const LongRow *longs, int32_t *long_cnt, double *long_part,
-                  const InstDev *insts, const Ctl *ctl, double *part) {
+int launch_primal_flush(hipStream_t s, int prec, int nblocks, const BlockDesc *blocks,
+                        const LongRow *longs, int32_t *long_cnt, double *long_part,
+                        const InstDev *insts, const Ctl *ctl) {
   if (nblocks <= 0) return 0;
   dim3 g(grid_of(nblocks)), b(kThreads);
-  if (prec == 0) {
-    if (flush)
-      hipLaunchKernelGGL((k_primal<double, true>), g, b, 0, s, blocks, longs, long_cnt, long_part, insts, ctl, part);
+  if (prec == 0)
+    hipLaunchKernelGGL(k_primal_flush<double>, g, b, 0, s, blocks, longs, long_cnt, long_part, insts, ctl);
+  else
+    hipLaunchKernelGGL(k_primal_flush<float>, g, b, 0, s, blocks, longs, long_cnt, long_part, insts, ctl);
+  PDLP_CHECK_LAUNCH();
+}
+
+int launch_dual_flush(hipStream_t s, int prec, int nblocks, const BlockDesc *blocks,
+                      const InstDev *insts, const Ctl *ctl) {
+  if (nblocks <= 0) return 0;
+  dim3 g(grid_of(nblocks)), b(kThreads);
+  if (prec == 0)
+    hipLaunchKernelGGL(k_dual_flush<double>, g, b, 0, s, blocks, insts, ctl);
+  else
+    hipLaunchKernelGGL(k_dual_flush<float>, g, b, 0, s, blocks, insts, ctl);
+  PDLP_CHECK_LAUNCH();
+}
+
+static inline unsigned grid_of_slices(int nsl) {
+  return (unsigned)(nsl > 0 ? (nsl + kThreads / 64 - 1) / (kThreads / 64) : 0);
+}
+
+// Normal slices [0, nslices) write part[0 .. 2 nslices); chunk slices
+// write the partials after them.
+int launch_primal(hipStream_t s, int prec, int nslices, const SliceDesc *slices, int nlong,
+                  const SliceDesc *lslices, const LongRow *longs, int32_t *long_cnt,
+                  double *long_part, const InstDev *insts, const Ctl *ctl, double *part) {
+  const dim3 b(kThreads);
+  if (nslices > 0) {
+    if (prec == 0)
+      hipLaunchKernelGGL(k_primal<double>, dim3(grid_of_slices(nslices)), b, 0, s, slices, nslices, insts, ctl, part);
     else
-      hipLaunchKernelGGL((k_primal<double, false>), g, b, 0, s, blocks, longs, long_cnt, long_part, insts, ctl, part);
-  } else {
-    if (flush)
-      hipLaunchKernelGGL((k_primal<float, true>), g, b, 0, s, blocks, longs, long_cnt, long_part, insts, ctl, part);
+      hipLaunchKernelGGL(k_primal<float>, dim3(grid_of_slices(nslices)), b, 0, s, slices, nslices, insts, ctl, part);
+  }
+  if (nlong > 0) {
+    double *lp = part + 2 * (size_t)std::max(nslices, 0);
+    if (prec == 0)
+      hipLaunchKernelGGL(k_primal_long<double>, dim3(grid_of_slices(nlong)), b, 0, s, lslices, nlong, longs, long_cnt, long_part, insts, ctl, lp);
     else
-      hipLaunchKernelGGL((k_primal<float, false>), g, b, 0, s, blocks, longs, long_cnt, long_part, insts, ctl, part);
+      hipLaunchKernelGGL(k_primal_long<float>, dim3(grid_of_slices(nlong)), b, 0, s, lslices, nlong, longs, long_cnt, long_part, insts, ctl, lp);
   }
   PDLP_CHECK_LAUNCH();
 }
 
-int launch_dual(hipStream_t s, int prec, bool flush, int nblocks, const BlockDesc *blocks,
-                const LongRow *longs, int32_t *long_cnt, double *long_part, const InstDev *insts,
-                const Ctl *ctl, double *part) {
-  if (nblocks <= 0) return 0;
-  dim3 g(grid_of(nblocks)), b(kThreads);
-  if (prec == 0) {
-    if (flush)
-      hipLaunchKernelGGL((k_dual<double, true>), g, b, 0, s, blocks, longs, long_cnt, long_part, insts, ctl, part);
+int launch_dual(hipStream_t s, int prec, int nslices, const SliceDesc *slices, int nlong,
+                const SliceDesc *lslices, const LongRow *longs, int32_t *long_cnt,
+                double *long_part, const InstDev *insts, const Ctl *ctl, double *part) {
+  const dim3 b(kThreads);
+  if (nslices > 0) {
+    if (prec == 0)
+      hipLaunchKernelGGL(k_dual<double>, dim3(grid_of_slices(nslices)), b, 0, s, slices, nslices, insts, ctl, part);
     else
-      hipLaunchKernelGGL((k_dual<double, false>), g, b, 0, s, blocks, longs, long_cnt, long_part, insts, ctl, part);
-  } else {
-    if (flush)
-      hipLaunchKernelGGL((k_dual<float, true>), g, b, 0, s, blocks, longs, long_cnt, long_part, insts, ctl, part);
-    else
-      hipLaunchKernelGGL((k_dual<float, false>), g, b, 0, s, blocks, longs, long_cnt, long_part, insts, ctl, part);
+      hipLaunchKernelGGL(k_dual<float>, dim3(grid_of_slices(nslices)), b, 0, s, slices, nslices, insts, ctl, part);
   }
+  if (nlong > 0) {
+    double *lp = part + 4 * (size_t)std::max(nslices, 0);
+    if (prec == 0)
+      hipLaunchKernelGGL(k_dual_long<double>, dim3(grid_of_slices(nlong)), b, 0, s, lslices, nlong, longs, long_cnt, long_part, insts, ctl, lp);
+    else
+      hipLaunchKernelGGL(k_dual_long<float>, dim3(grid_of_slices(nlong)), b, 0, s, lslices, nlong, longs, long_cnt, long_part, insts, ctl, lp);
+  }
+  PDLP_CHECK_LAUNCH();
+}
+
+// Sliced copy of a CSR matrix (one wave per slice; see SliceDesc).
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_slice_fill(const SliceDesc *__restrict__ slices,
+                                                         int nsl, const int32_t *__restrict__ ptr,
+                                                         const int32_t *__restrict__ col,
+                                                         const T *__restrict__ val,
+                                                         int32_t *__restrict__ scol,
+                                                         T *__restrict__ sval) {
+  const int wv = (int)blockIdx.x * (kThreads / 64) + ((int)threadIdx.x >> 6);
+  if (wv >= nsl) return;
+  const int lane = threadIdx.x & 63;
+  const SliceDesc sd = slices[wv];
+  int row, k, kstep, len;
+  if (sd.nrows == 0) {
+    row = sd.row0;
+    k = sd.chunk * kSellChunk + lane;
+    kstep = 64;
+    len = ptr[row + 1] - ptr[row];
+  } else {
+    const int g = lane >> sd.lg;
+    row = sd.row0 + g;
+    k = lane & ((1 << sd.lg) - 1);
+    kstep = 1 << sd.lg;
+    len = g < sd.nrows ? ptr[row + 1] - ptr[row] : 0;
+  }
+  const int base = len > 0 ? ptr[row] : 0;
+  for (int q = 0; q < sd.width; ++q, k += kstep) {
+    const int slot = sd.off + 64 * q + lane;
+    const bool ok = k < len;
+    scol[slot] = ok ? col[base + k] : -1;
+    sval[slot] = ok ? val[base + k] : (T)0;
+  }
+}
+
+int launch_slice_fill(hipStream_t s, int prec, int nslices, const SliceDesc *slices,
+                      const int32_t *ptr, const int32_t *col, const void *val, int32_t *scol,
+                      void *sval) {
+  if (nslices <= 0) return 0;
+  dim3 g(grid_of_slices(nslices)), b(kThreads);
+  if (prec == 0)
+    hipLaunchKernelGGL(k_slice_fill<double>, g, b, 0, s, slices, nslices, ptr, col,
+                       (const double *)val, scol, (double *)sval);
+  else
+    hipLaunchKernelGGL(k_slice_fill<float>, g, b, 0, s, slices, nslices, ptr, col,
+                       (const float *)val, scol, (float *)sval);
   PDLP_CHECK_LAUNCH();
 }
 

@@ -156,6 +156,12 @@ int load_problem(const pdlp_lp *lp, Prepared *P, std::string *err, bool build_cs
   P->a_ptr.assign(m + 1, 0);
   for (int64_t k = 0; k < nnz; ++k) P->a_ptr[P->t_col[k] + 1]++;
   for (int64_t i = 0; i < m; ++i) P->a_ptr[i + 1] += P->a_ptr[i];
+  build_slices(P->t_ptr.data(), n, &P->ps);
+  build_slices(P->a_ptr.data(), m, &P->ds);
+  if (P->ps.slots > INT32_MAX || P->ds.slots > INT32_MAX) {
+    if (err) *err = "the constraint matrix is too large for 32-bit slot offsets";
+    return 3;
+  }
   if (!build_csr) {
     P->max_abs_scaled = norminf(P->t_val);
     return 0;
@@ -174,6 +180,42 @@ int load_problem(const pdlp_lp *lp, Prepared *P, std::string *err, bool build_cs
     }
   P->max_abs_scaled = norminf(P->t_val);
   return 0;
+}
+
+void build_slices(const int32_t *ptr, int64_t R, SliceTable *out) {
+  out->sl.clear();
+  out->nchunks.clear();
+  int64_t off = 0, r = 0;
+  while (r < R) {
+    const int64_t len0 = ptr[r + 1] - ptr[r];
+    if (len0 > kSellChunk) {  // a long row: chunk slices of kSellChunk entries
+      const int32_t nch = (int32_t)((len0 + kSellChunk - 1) / kSellChunk);
+      const int32_t lid = (int32_t)out->nchunks.size();
+      out->nchunks.push_back(nch);
+      for (int32_t c = 0; c < nch; ++c) {
+        const int64_t cl = std::min<int64_t>(kSellChunk, len0 - (int64_t)c * kSellChunk);
+        const int32_t w = (int32_t)((cl + 63) / 64);
+        out->sl.push_back({0, (int32_t)r, (int32_t)off, w, 6, 0, lid, c});
+        off += 64 * (int64_t)w;
+      }
+      ++r;
+      continue;
+    }
+    int lg = 0;
+    while (lg < 6 && len0 > ((int64_t)kSellWidth << lg)) ++lg;
+    const int64_t cap = lg == 6 ? kSellChunk : (int64_t)kSellWidth << lg, maxrows = 64 >> lg;
+    int64_t nr = 0, w = 0;
+    while (r + nr < R && nr < maxrows) {
+      const int64_t len = ptr[r + nr + 1] - ptr[r + nr];
+      if (len > cap) break;
+      w = std::max<int64_t>(w, (len + (1 << lg) - 1) >> lg);
+      ++nr;
+    }
+    out->sl.push_back({0, (int32_t)r, (int32_t)off, (int32_t)w, lg, (int32_t)nr, 0, 0});
+    off += 64 * w;
+    r += nr;
+  }
+  out->slots = off;
 }
 
 // scale_problem — src/preprocess.jl:541-561, with Diag(1/r) * A * Diag(1/v)

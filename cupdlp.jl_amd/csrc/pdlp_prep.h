@@ -7,6 +7,7 @@
 #include <string>
 #include <vector>
 
+#include "pdlp_device.h"
 #include "pdlp_hip.h"
 
 namespace pdlp {
@@ -14,6 +15,22 @@ namespace pdlp {
 struct QpCache {  // CachedQuadraticProgramInfo, src/termination.jl:123-137
   double l_inf_c, l_inf_b, l2_c, l2_b;
 };
+
+// The slices of one matrix layout (see SliceDesc): descriptors with inst = 0
+// and instance-local long-row ids, the chunk count of each long row, and the
+// number of slots of the sliced arrays.
+struct SliceTable {
+  std::vector<SliceDesc> sl;
+  std::vector<int32_t> nchunks;
+  int64_t slots = 0;
+};
+
+// Cuts rows [0, R) with row pointers ptr into slices: a slice takes
+// consecutive rows while each fits 2^lg * kSellWidth entries for the lane
+// count 2^lg chosen by its first row (and at most 64 >> lg rows); a row of
+// more than 64 * kSellWidth entries is a slice of its own, one of more than
+// kSellChunk entries becomes chunk slices.
+void build_slices(const int32_t *ptr, int64_t R, SliceTable *out);
 
 // An LP after rescale_problem, in the device layouts (0-based, int32).
 struct Prepared {
@@ -36,6 +53,7 @@ struct Prepared {
   // device numbering -> the caller's: column j here is column col_perm[j] of
   // the input, row i is row row_perm[i]
   std::vector<int32_t> col_perm, row_perm;
+  SliceTable ps, ds;            // slices of A^T (primal step) and A (dual step)
   bool device_prep = false;     // rescaling and the CSR of A are built on the device
   double norm_b_scaled = 0.0, norm_c_scaled = 0.0;  // set by the device preparation
 };

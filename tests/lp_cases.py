@@ -54,3 +54,23 @@ def long_row_lp(n=6000, seed=7):
     p = QuadraticProgrammingProblem(prob.variable_lower_bound, prob.variable_upper_bound, None,
                                     c, 0.0, A, b, prob.num_equalities)
     return p, float(c @ cert.x)
+
+
+def long_col_lp(m=6000, seed=9):
+    """A random LP with one dense column of m entries (longer than a slice of
+    the primal step) plus sparse columns; certified by construction: b and
+    c[0] follow the new column so that the generator's (x*, y*, r) stays
+    optimal."""
+    from cupdlp.generators import random_lp
+    prob, cert = random_lp(n=2 * m, m=m, num_eq=m // 2, nnz_per_col=3, seed=seed)
+    A = prob.constraint_matrix.tolil()
+    old = np.asarray(A[:, 0].todense()).ravel()
+    new = np.random.default_rng(seed).standard_normal(m)
+    A[:, 0] = new.reshape(-1, 1)
+    A = sp.csc_matrix(A)
+    b = prob.right_hand_side + (new - old) * cert.x[0]
+    c = prob.objective_vector.copy()
+    c[0] = float(new @ cert.y) + cert.r[0]
+    p = QuadraticProgrammingProblem(prob.variable_lower_bound, prob.variable_upper_bound, None,
+                                    c, 0.0, A, b, prob.num_equalities)
+    return p, float(c @ cert.x)

@@ -191,6 +191,15 @@ def test_very_long_row_chunked(gpu_ctx):
     compare_runs(g, o)
 
 
+def test_long_column(gpu_ctx):
+    say("case: LP with a 20000-entry column (chunked long rows of A^T in the primal step)")
+    prob, opt = lp_cases.long_col_lp(20000)
+    p = params(1e-12, iteration_limit=500)
+    g = gpu_ctx.solve(prob, p)
+    o = O.solve(prob, p)
+    compare_runs(g, o)
+
+
 def test_transportation_small(gpu_ctx):
     say("case: 60 x 60 transportation LP to 1e-6 (has rejected step attempts)")
     prob = transportation_lp(60, 60, seed=5)
