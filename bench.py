@@ -19,7 +19,8 @@ Reported beside it:
   * `cpu_baseline`: the CPU oracle (same restarted PDHG, OpenMP) on a bounded
     sample of the same LPs on this host's cores.
 
-Other workloads: --workload c1 (n = 1,000,000 LP, fixed iteration count, for
+Other workloads: --workload serve (a Poisson request stream through the
+continuous-batching loop, p50/p99 latency), --workload c1 (n = 1,000,000 LP, fixed iteration count, for
 the FP64 SpMV roofline), --workload c2 (2,000 x 2,000 transportation LP),
 --workload c0 (n = 2,000 LP), --workload copy (a plain 1 GiB device copy, the
 calibration kernel for the rocprofv3 FETCH_SIZE / WRITE_SIZE counters).
@@ -155,6 +156,77 @@ def run_batch(args, ctx, lib):
     return value, dt, cfg, extra, roof, problems
 
 
+def serve_stream(count, nmax, seed=4):
+    """A request stream after the served configuration: 90 % random LPs with
+    n log-uniform in [1,000, nmax] (m = n/2, 8 nonzeros per column), 10 %
+    transportation LPs with 200..1,000 sources and sinks; tolerance 1e-4
+    (70 %) or 1e-8 (30 %).  Per-request seeds come from the stream seed."""
+    from cupdlp.generators import random_lp, transportation_lp
+    rng = np.random.default_rng(seed)
+    probs, tols = [], []
+    for k in range(count):
+        s = int(rng.integers(0, 2**31 - 1))
+        if rng.random() < 0.9:
+            n = int(np.exp(rng.uniform(np.log(1000), np.log(nmax))))
+            probs.append(random_lp(n=n, m=n // 2, seed=s)[0])
+        else:
+            a, b = (int(v) for v in rng.integers(200, 1001, size=2))
+            probs.append(transportation_lp(a, b, seed=s))
+        tols.append(1e-4 if rng.random() < 0.7 else 1e-8)
+    return probs, tols
+
+
+def run_serve(args, ctx, lib):
+    from cupdlp.solver import Server
+    log(f"generating a stream of {args.count} requests ...")
+    probs, tols = serve_stream(args.count, args.serve_nmax)
+    rng = np.random.default_rng(5)
+    gaps = rng.exponential(1.0 / args.rate, size=len(probs)) if args.rate > 0 else np.zeros(len(probs))
+    arrive = np.cumsum(gaps) - gaps[0]
+    params = make_params(1e-4, args.precision)
+    for w in range(args.warmup):  # warm-up: the first few requests alone
+        for p, t in list(zip(probs, tols))[:4]:
+            ctx.solve(p, make_params(t, args.precision))
+    srv = Server(ctx, params, hbm_budget_bytes=args.hbm_budget * 1e9)
+    lat, bad, done = [], 0, 0
+    nxt = 0
+    t0 = time.time()
+    ids = {}
+    while done < len(probs):
+        now = time.time() - t0
+        while nxt < len(probs) and arrive[nxt] <= now:
+            ids[srv.submit(probs[nxt], tols[nxt])] = nxt
+            nxt += 1
+        if done < nxt:
+            for f in srv.poll():
+                o = srv.result(f)
+                lat.append(o.solve_time_sec)
+                bad += int(o.termination_string != "OPTIMAL")
+                done += 1
+        elif nxt < len(probs):
+            time.sleep(max(0.0, min(arrive[nxt] - now, 0.05)))
+    dt = time.time() - t0
+    srv.close()
+    value = len(probs) / dt
+    t1 = time.time()
+    nser = 0
+    for p, t in zip(probs, tols):
+        ctx.solve(p, make_params(t, "fp64"))
+        nser += 1
+        if time.time() - t1 > args.serial_budget:
+            break
+    ser = nser / (time.time() - t1)
+    extra = {"p50_latency_s": round(float(np.percentile(lat, 50)), 4),
+             "p99_latency_s": round(float(np.percentile(lat, 99)), 4),
+             "not_optimal": bad, "fp64_serial_solves_per_s": round(ser, 3),
+             "fp64_serial_sample": nser, "speedup_vs_fp64_serial": round(value / ser, 3)}
+    cfg = {"workload": f"served stream of {len(probs)} requests (90% random LPs n log-uniform "
+                       f"1000..{args.serve_nmax}, 10% transportation 200..1000 per side; tolerance "
+                       f"1e-4 (70%) / 1e-8 (30%)), Poisson arrivals at {args.rate:g}/s",
+           "precision": args.precision, "hbm_budget_GB": args.hbm_budget}
+    return value, dt, cfg, extra, None, probs
+
+
 def run_single(args, ctx, lib, which):
     import ctypes as C
     from cupdlp import _abi
@@ -209,7 +281,10 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", choices=("batch", "c1", "c2", "c0", "copy"), default="batch")
+    ap.add_argument("--workload", choices=("batch", "serve", "c1", "c2", "c0", "copy"), default="batch")
+    ap.add_argument("--rate", type=float, default=50.0, help="serve: Poisson arrival rate (requests/s)")
+    ap.add_argument("--serve-nmax", type=int, default=200000, help="serve: largest random-LP n")
+    ap.add_argument("--hbm-budget", type=float, default=200.0, help="serve: HBM budget (GB)")
     ap.add_argument("--precision", choices=("fp64", "fp32"), default="fp64")
     ap.add_argument("--tol", type=float, default=1e-8)
     ap.add_argument("--count", type=int, default=512)
@@ -232,6 +307,9 @@ def main():
     lib = ctx.lib
     if args.workload == "batch":
         value, dt, cfg, extra, roof, problems = run_batch(args, ctx, lib)
+        metric_unit = "solves/s"
+    elif args.workload == "serve":
+        value, dt, cfg, extra, roof, problems = run_serve(args, ctx, lib)
         metric_unit = "solves/s"
     else:
         value, dt, cfg, extra, roof, problems = run_single(args, ctx, lib, int(args.workload[1]))

@@ -11,5 +11,5 @@ from .types import (AdaptiveStepsizeParams, ConstantStepsizeParams, PdhgParamete
                     RestartParameters, construct_restart_parameters,
                     construct_termination_criteria, linear_programming_problem,
                     termination_reason_to_string)
-from .solver import Context, PdlpError, default_context, default_params, optimize  # noqa: F401
+from .solver import Context, PdlpError, Server, default_context, default_params, optimize  # noqa: F401
 from .moi import Optimizer  # noqa: F401

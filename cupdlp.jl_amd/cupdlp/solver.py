@@ -206,15 +206,80 @@ class Context:
             raise PdlpError(rc, self.error())
         for i, p in enumerate(problems):
             (_, x, y) = keeps[i]
-            r = results[i]
-            reason = TerminationReason(r.termination_reason)
-            outs.append(SaddlePointOutput(
-                primal_solution=x[:p.num_variables].copy(), dual_solution=y[:p.num_constraints].copy(),
-                termination_reason=reason, termination_string=termination_reason_to_string(reason),
-                iteration_count=int(r.iteration_count), iteration_stats=[from_c_stats(r.final_stats)],
-                solve_time_sec=r.solve_time_sec, total_attempts=int(r.total_attempts),
-                refinement_rounds=int(r.refinement_rounds)))
+            outs.append(_output_from_result(results[i], x, y, p.num_variables, p.num_constraints))
         return outs
+
+
+def _output_from_result(r, x, y, n, m):
+    reason = TerminationReason(r.termination_reason)
+    return SaddlePointOutput(
+        primal_solution=x[:n].copy(), dual_solution=y[:m].copy(),
+        termination_reason=reason, termination_string=termination_reason_to_string(reason),
+        iteration_count=int(r.iteration_count), iteration_stats=[from_c_stats(r.final_stats)],
+        solve_time_sec=r.solve_time_sec, total_attempts=int(r.total_attempts),
+        refinement_rounds=int(r.refinement_rounds))
+
+
+class Server:
+    """Continuous-batching request loop (pdlp_serve_*; no counterpart in the
+    reference).  Requests are admitted by HBM footprint and slot count as
+    they arrive, stepped together with the running ones and retired when
+    they finish; each follows ``optimize`` with eps_optimal_absolute =
+    eps_optimal_relative = its tolerance, so its result equals a lone solve
+    with those criteria.  ``solve_time_sec`` of a result is its latency from
+    ``submit``."""
+
+    def __init__(self, ctx: Context, params: PdhgParameters, hbm_budget_bytes=0.0, max_slots=0):
+        self.ctx = ctx
+        self.lib = ctx.lib
+        self._params = to_c_params(params)
+        h = C.c_void_p()
+        rc = self.lib.pdlp_serve_open(ctx.handle, C.byref(self._params), float(hbm_budget_bytes),
+                                      int(max_slots), C.byref(h))
+        if rc != 0:
+            raise PdlpError(rc, ctx.error())
+        self.handle = h
+        self._shape = {}
+
+    def submit(self, problem: QuadraticProgrammingProblem, tolerance: float) -> int:
+        lp, keep = to_c_lp(problem)
+        rid = C.c_int64()
+        rc = self.lib.pdlp_serve_submit(self.handle, C.byref(lp), float(tolerance), C.byref(rid))
+        del keep
+        if rc != 0:
+            raise PdlpError(rc, self.ctx.error())
+        self._shape[rid.value] = (problem.num_variables, problem.num_constraints)
+        return rid.value
+
+    def poll(self, max_ids=4096):
+        """Advances the device work by one round; returns the ids finished."""
+        ids = (C.c_int64 * max_ids)()
+        k = C.c_int32()
+        rc = self.lib.pdlp_serve_poll(self.handle, ids, int(max_ids), C.byref(k))
+        if rc != 0:
+            raise PdlpError(rc, self.ctx.error())
+        return [ids[i] for i in range(k.value)]
+
+    def result(self, rid: int) -> SaddlePointOutput:
+        n, m = self._shape.pop(rid)
+        x = np.zeros(max(n, 1)); y = np.zeros(max(m, 1))
+        r = _abi.pdlp_result()
+        r.x = _ptr(x, C.c_double); r.y = _ptr(y, C.c_double)
+        rc = self.lib.pdlp_serve_result(self.handle, int(rid), C.byref(r))
+        if rc != 0:
+            raise PdlpError(rc, self.ctx.error())
+        return _output_from_result(r, x, y, n, m)
+
+    def close(self):
+        if self.handle:
+            self.lib.pdlp_serve_close(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 _ctx_lock = threading.Lock()

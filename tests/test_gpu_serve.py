@@ -1,0 +1,121 @@
+"""GPU tests of the continuous-batching request loop (pdlp_serve_*).
+
+The bar: a request's result is the same whether it runs alone or arrives
+while other requests are on the device -- bitwise (x, y, iteration and
+attempt counts, termination reason, final statistics), since instances
+packed into the same launches share no arithmetic.  A lone solve with
+eps_optimal_absolute = eps_optimal_relative = the request's tolerance is
+the reference for "alone".
+"""
+import numpy as np
+import pytest
+
+from cupdlp import PdhgParameters, PdlpError, Precision, Server, construct_termination_criteria
+from cupdlp.generators import random_lp, transportation_lp
+
+pytestmark = pytest.mark.gpu
+
+
+def params(tol, precision=Precision.FP64):
+    tc = construct_termination_criteria(eps_optimal_absolute=tol, eps_optimal_relative=tol)
+    return PdhgParameters(termination_criteria=tc, precision=precision)
+
+
+def same_result(a, b):
+    assert a.termination_reason == b.termination_reason
+    assert a.iteration_count == b.iteration_count
+    assert a.total_attempts == b.total_attempts
+    assert np.array_equal(a.primal_solution, b.primal_solution)
+    assert np.array_equal(a.dual_solution, b.dual_solution)
+    ca = a.iteration_stats[-1].convergence_information[0]
+    cb = b.iteration_stats[-1].convergence_information[0]
+    assert ca.primal_objective == cb.primal_objective
+    assert ca.l2_primal_residual == cb.l2_primal_residual
+    assert ca.l2_dual_residual == cb.l2_dual_residual
+
+
+def stream():
+    sizes = [300, 4000, 900, 12000, 2500, 600, 7000, 1500]
+    probs = [random_lp(n=n, m=n // 2, seed=40 + i)[0] for i, n in enumerate(sizes)]
+    probs.append(transportation_lp(30, 40, seed=3))
+    tols = [1e-4 if i % 3 else 1e-8 for i in range(len(probs))]
+    return probs, tols
+
+
+def drain(srv, ids, out, max_rounds=1_000_000):
+    for _ in range(max_rounds):
+        if all(i in out for i in ids):
+            return
+        for f in srv.poll():
+            out[f] = srv.result(f)
+    raise AssertionError("requests did not finish")
+
+
+def test_serve_alone_vs_mixed(gpu_ctx):
+    print("case: 9 requests (random LPs n=300..12000, one transportation LP), tolerances 1e-4/1e-8, "
+          "staggered arrivals", flush=True)
+    probs, tols = stream()
+    alone = [gpu_ctx.solve(p, params(t)) for p, t in zip(probs, tols)]
+    srv = Server(gpu_ctx, params(1e-4))
+    try:
+        out = {}
+        ids = [srv.submit(probs[i], tols[i]) for i in range(3)]
+        for _ in range(2):
+            for f in srv.poll():
+                out[f] = srv.result(f)
+        ids += [srv.submit(probs[i], tols[i]) for i in range(3, 6)]
+        while not out:
+            for f in srv.poll():
+                out[f] = srv.result(f)
+        ids += [srv.submit(probs[i], tols[i]) for i in range(6, len(probs))]
+        drain(srv, ids, out)
+    finally:
+        srv.close()
+    for k, rid in enumerate(ids):
+        assert out[rid].termination_string == "OPTIMAL"
+        same_result(out[rid], alone[k])
+        assert out[rid].solve_time_sec > 0.0
+
+
+def test_serve_one_slot_and_budget(gpu_ctx):
+    print("case: one slot at a time (admission waits for retirement); a request over the HBM budget",
+          flush=True)
+    probs, tols = stream()
+    probs, tols = probs[:4], tols[:4]
+    alone = [gpu_ctx.solve(p, params(t)) for p, t in zip(probs, tols)]
+    srv = Server(gpu_ctx, params(1e-4), max_slots=1)
+    try:
+        out = {}
+        ids = [srv.submit(p, t) for p, t in zip(probs, tols)]
+        drain(srv, ids, out)
+    finally:
+        srv.close()
+    for k, rid in enumerate(ids):
+        same_result(out[rid], alone[k])
+    small = Server(gpu_ctx, params(1e-4), hbm_budget_bytes=4096.0)
+    try:
+        with pytest.raises(PdlpError) as e:
+            small.submit(probs[1], 1e-4)
+        assert e.value.code == 5
+    finally:
+        small.close()
+
+
+def test_serve_fp32_refine_alone_vs_mixed(gpu_ctx):
+    print("case: FP32 + refinement server, 3 requests to 1e-8", flush=True)
+    probs = [random_lp(n=n, m=n // 2, seed=60 + i)[0] for i, n in enumerate((800, 3000, 1500))]
+    alone = [gpu_ctx.solve(p, params(1e-8, Precision.FP32_REFINE)) for p in probs]
+    srv = Server(gpu_ctx, params(1e-4, Precision.FP32_REFINE))
+    try:
+        out = {}
+        ids = [srv.submit(probs[0], 1e-8)]
+        for _ in range(3):
+            for f in srv.poll():
+                out[f] = srv.result(f)
+        ids += [srv.submit(p, 1e-8) for p in probs[1:]]
+        drain(srv, ids, out)
+    finally:
+        srv.close()
+    for k, rid in enumerate(ids):
+        assert out[rid].termination_string == "OPTIMAL"
+        same_result(out[rid], alone[k])
