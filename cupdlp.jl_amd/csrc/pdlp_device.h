@@ -51,10 +51,11 @@ struct SliceDesc {
   int32_t row0;   // first row (chunk slices: the long row)
   int32_t off;    // first slot in the instance's sliced arrays
   int32_t width;  // slots per lane
-  int32_t lg;     // log2 of the lanes per row (chunk slices: 6)
-  int32_t nrows;  // rows in the slice; 0 = chunk slice
+  int16_t lg;     // log2 of the lanes per row (chunk slices: 6)
+  int16_t nrows;  // rows in the slice; 0 = chunk slice; < 0 = no work
   int32_t lid;    // chunk slices: long-row id
   int32_t chunk;  // chunk slices: chunk index
+  int32_t part;   // index of the slice's partial sums (instance-contiguous)
 };
 
 struct BlockDesc {

@@ -514,6 +514,12 @@ int Engine::rebuild_tables(std::string *err) {
     h->dev.dl0 += (int32_t)ds.size();
     h->dev.dl1 += (int32_t)ds.size();
   }
+  // partial indices: normal slices in instance order, then the chunk slices
+  for (size_t k = 0; k < ps.size(); ++k) ps[k].part = (int32_t)k;
+  for (size_t k = 0; k < ds.size(); ++k) ds[k].part = (int32_t)k;
+  for (size_t k = 0; k < pls.size(); ++k) pls[k].part = (int32_t)(ps.size() + k);
+  for (size_t k = 0; k < dls.size(); ++k) dls[k].part = (int32_t)(ds.size() + k);
+  const size_t nparts_p = ps.size() + pls.size(), nparts_d = ds.size() + dls.size();
   npl_ = (int)pls.size();
   ndl_ = (int)dls.size();
   ps.insert(ps.end(), pls.begin(), pls.end());  // one upload: normal slices, then chunks
@@ -540,8 +546,8 @@ int Engine::rebuild_tables(std::string *err) {
   d_insts_ = (InstDev *)alloc(sizeof(InstDev) * ninst);
   d_ctl_ = (Ctl *)alloc(sizeof(Ctl) * ninst);
   // compensated partial sums: (s, e) pairs
-  d_ppart_ = (double *)alloc(sizeof(double) * 2 * ps.size());
-  d_dpart_ = (double *)alloc(sizeof(double) * 4 * ds.size());
+  d_ppart_ = (double *)alloc(sizeof(double) * 2 * nparts_p);
+  d_dpart_ = (double *)alloc(sizeof(double) * 4 * nparts_d);
   d_epn_ = (double *)alloc(sizeof(double) * 2 * kEvalN * pb.size());
   d_epm_ = (double *)alloc(sizeof(double) * 2 * kEvalM * db.size());
   d_epr_ = (double *)alloc(sizeof(double) * 2 * pb.size());
@@ -627,7 +633,7 @@ int Engine::run_attempts(int64_t count, std::string *err) {
   std::vector<double> pbytes, dbytes;
   std::vector<int64_t> remaining;
   if (prof) {
-    const size_t need = 3 * (size_t)count;
+    const size_t need = 4 * (size_t)count;
     while (prof_->events.size() < need) {
       hipEvent_t ev;
       HIPCHK(hipEventCreate(&ev));
@@ -644,14 +650,14 @@ int Engine::run_attempts(int64_t count, std::string *err) {
   }
   const double t0 = now_sec();
   for (int64_t a = 0; a < count; ++a) {
-    if (prof) HIPCHK(hipEventRecord(prof_->events[3 * a + 0], s_));
+    hipEvent_t *ev = prof ? &prof_->events[4 * a] : nullptr;
     int e = launch_primal(s_, prec_, nps_, d_pslices_, npl_, d_pslices_ + nps_, d_longs_,
-                          d_long_cnt_, d_long_part_, d_insts_, d_ctl_, d_ppart_);
-    if (prof) HIPCHK(hipEventRecord(prof_->events[3 * a + 1], s_));
+                          d_long_cnt_, d_long_part_, d_insts_, d_ctl_, d_ppart_,
+                          ev ? ev[0] : nullptr, ev ? ev[1] : nullptr);
     if (!e)
       e = launch_dual(s_, prec_, nds_, d_dslices_, ndl_, d_dslices_ + nds_, d_longs_,
-                      d_long_cnt_, d_long_part_, d_insts_, d_ctl_, d_dpart_);
-    if (prof) HIPCHK(hipEventRecord(prof_->events[3 * a + 2], s_));
+                      d_long_cnt_, d_long_part_, d_insts_, d_ctl_, d_dpart_,
+                      ev ? ev[2] : nullptr, ev ? ev[3] : nullptr);
     if (!e)
       e = launch_step(s_, (int)inst_.size(), d_insts_, d_ctl_, d_ppart_, d_dpart_, d_red_, d_grow_,
                       tab_len_, prm_.reduction_exponent, prm_.growth_exponent);
@@ -668,8 +674,8 @@ int Engine::run_attempts(int64_t count, std::string *err) {
   if (prof) {
     for (int64_t a = 0; a < count; ++a) {
       float ms_p = 0.f, ms_d = 0.f;
-      HIPCHK(hipEventElapsedTime(&ms_p, prof_->events[3 * a + 0], prof_->events[3 * a + 1]));
-      HIPCHK(hipEventElapsedTime(&ms_d, prof_->events[3 * a + 1], prof_->events[3 * a + 2]));
+      HIPCHK(hipEventElapsedTime(&ms_p, prof_->events[4 * a + 0], prof_->events[4 * a + 1]));
+      HIPCHK(hipEventElapsedTime(&ms_d, prof_->events[4 * a + 2], prof_->events[4 * a + 3]));
       double bp = 0.0, bd = 0.0;
       for (size_t q = 0; q < ninst; ++q)
         if (a < remaining[q]) {

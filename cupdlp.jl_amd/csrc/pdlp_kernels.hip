@@ -25,6 +25,7 @@
 //
 // Floating-point contraction is disabled for this file (-ffp-contract=off) so
 // each elementwise formula rounds exactly as the reference's written form.
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <math.h>
@@ -401,7 +402,8 @@ __device__ __forceinline__ void primal_update(const PrimalView<T> &v, int j, con
 
 // Primal half-step over the slices of A^T: A^T y (only when y changed), the
 // pending weighted-sum update of x and A^T y, the projected primal step and
-// the partial sum of dx^2 of each slice (part[2w], part[2w+1] for slice w).
+// the partial sum of dx^2 of each slice (part[2p], part[2p+1] for the
+// slice's partial index p).
 // One wave per slice; each row's operands are loaded before the gathers of
 // its sum.
 template <typename T>
@@ -413,6 +415,7 @@ __global__ __launch_bounds__(kThreads) void k_primal(const SliceDesc *__restrict
   if (wv >= nsl) return;
   const int lane = threadIdx.x & 63;
   const SliceDesc sd = slices[wv];
+  if (sd.nrows < 0) return;
   const Ctl *C = &ctl[sd.inst];
   if (C->active == 0) return;
   const InstDev &I = insts[sd.inst];
@@ -430,8 +433,8 @@ __global__ __launch_bounds__(kThreads) void k_primal(const SliceDesc *__restrict
   if (mine) primal_update(v, j, r, s, acc);
   acc = wave_dd(acc);
   if (lane == 0) {
-    part[2 * wv] = acc.s;
-    part[2 * wv + 1] = acc.e;
+    part[2 * sd.part] = acc.s;
+    part[2 * sd.part + 1] = acc.e;
   }
 }
 
@@ -466,8 +469,8 @@ __global__ __launch_bounds__(kThreads) void k_primal_long(const SliceDesc *__res
     primal_update(v, sd.row0, primal_load(v, sd.row0), 0.0, acc);
   }
   if (lane == 0) {
-    part[2 * wv] = acc.s;
-    part[2 * wv + 1] = acc.e;
+    part[2 * sd.part] = acc.s;
+    part[2 * sd.part + 1] = acc.e;
   }
 }
 
@@ -640,7 +643,7 @@ __device__ __forceinline__ void dual_update(const DualView<T> &v, int i, const D
 
 // Dual half-step over the slices of A: the product, the pending weighted-sum
 // update of y and A x, the projected dual step and the partial sums of dy^2
-// and Adx*dy of each slice (part[4w..4w+3]).
+// and Adx*dy of each slice (part[4p..4p+3] for its partial index p).
 template <typename T>
 __global__ __launch_bounds__(kThreads) void k_dual(const SliceDesc *__restrict__ slices, int nsl,
                                                    const InstDev *__restrict__ insts,
@@ -650,6 +653,7 @@ __global__ __launch_bounds__(kThreads) void k_dual(const SliceDesc *__restrict__
   if (wv >= nsl) return;
   const int lane = threadIdx.x & 63;
   const SliceDesc sd = slices[wv];
+  if (sd.nrows < 0) return;
   const Ctl *C = &ctl[sd.inst];
   if (C->active == 0) return;
   const InstDev &I = insts[sd.inst];
@@ -665,10 +669,10 @@ __global__ __launch_bounds__(kThreads) void k_dual(const SliceDesc *__restrict__
   acc_dy = wave_dd(acc_dy);
   acc_int = wave_dd(acc_int);
   if (lane == 0) {
-    part[4 * wv] = acc_dy.s;
-    part[4 * wv + 1] = acc_dy.e;
-    part[4 * wv + 2] = acc_int.s;
-    part[4 * wv + 3] = acc_int.e;
+    part[4 * sd.part] = acc_dy.s;
+    part[4 * sd.part + 1] = acc_dy.e;
+    part[4 * sd.part + 2] = acc_int.s;
+    part[4 * sd.part + 3] = acc_int.e;
   }
 }
 
@@ -695,10 +699,10 @@ __global__ __launch_bounds__(kThreads) void k_dual_long(const SliceDesc *__restr
     double tot;
     if (longrow_publish(longs[sd.lid], sd.chunk, sd.lid, long_cnt, long_part, t, &tot))
       dual_update(v, sd.row0, dual_load(v, sd.row0), tot, acc_dy, acc_int);
-    part[4 * wv] = acc_dy.s;
-    part[4 * wv + 1] = acc_dy.e;
-    part[4 * wv + 2] = acc_int.s;
-    part[4 * wv + 3] = acc_int.e;
+    part[4 * sd.part] = acc_dy.s;
+    part[4 * sd.part + 1] = acc_dy.e;
+    part[4 * sd.part + 2] = acc_int.s;
+    part[4 * sd.part + 3] = acc_int.e;
   }
 }
 
@@ -1341,44 +1345,71 @@ static inline unsigned grid_of_slices(int nsl) {
   return (unsigned)(nsl > 0 ? (nsl + kThreads / 64 - 1) / (kThreads / 64) : 0);
 }
 
-// Normal slices [0, nslices) write part[0 .. 2 nslices); chunk slices
-// write the partials after them.
+// A launch that, when e0 / e1 are given, records them at the start / end of
+// the kernel's own execution (hipExtLaunchKernelGGL), so profiled durations
+// exclude the dispatch gaps between kernels.
+template <typename F, typename... Args>
+static void launch_ev(F kernel, dim3 g, dim3 b, hipStream_t s, hipEvent_t e0, hipEvent_t e1,
+                      Args... args) {
+  if (e0 || e1)
+    hipExtLaunchKernelGGL(kernel, g, b, 0, s, e0, e1, 0, args...);
+  else
+    hipLaunchKernelGGL(kernel, g, b, 0, s, args...);
+}
+
+// Normal slices, then the chunk slices of long rows; each slice writes its
+// partial sums at its own partial index.  e0 / e1 (optional) bracket both
+// launches.
 int launch_primal(hipStream_t s, int prec, int nslices, const SliceDesc *slices, int nlong,
                   const SliceDesc *lslices, const LongRow *longs, int32_t *long_cnt,
-                  double *long_part, const InstDev *insts, const Ctl *ctl, double *part) {
+                  double *long_part, const InstDev *insts, const Ctl *ctl, double *part,
+                  hipEvent_t e0, hipEvent_t e1) {
   const dim3 b(kThreads);
+  if (nslices <= 0 && nlong <= 0) {
+    hipError_t r = e0 ? hipEventRecord(e0, s) : hipSuccess;
+    if (r == hipSuccess && e1) r = hipEventRecord(e1, s);
+    return (int)r;
+  }
   if (nslices > 0) {
+    hipEvent_t stop = nlong > 0 ? nullptr : e1;
     if (prec == 0)
-      hipLaunchKernelGGL(k_primal<double>, dim3(grid_of_slices(nslices)), b, 0, s, slices, nslices, insts, ctl, part);
+      launch_ev(k_primal<double>, dim3(grid_of_slices(nslices)), b, s, e0, stop, slices, nslices, insts, ctl, part);
     else
-      hipLaunchKernelGGL(k_primal<float>, dim3(grid_of_slices(nslices)), b, 0, s, slices, nslices, insts, ctl, part);
+      launch_ev(k_primal<float>, dim3(grid_of_slices(nslices)), b, s, e0, stop, slices, nslices, insts, ctl, part);
   }
   if (nlong > 0) {
-    double *lp = part + 2 * (size_t)std::max(nslices, 0);
+    hipEvent_t start = nslices > 0 ? nullptr : e0;
     if (prec == 0)
-      hipLaunchKernelGGL(k_primal_long<double>, dim3(grid_of_slices(nlong)), b, 0, s, lslices, nlong, longs, long_cnt, long_part, insts, ctl, lp);
+      launch_ev(k_primal_long<double>, dim3(grid_of_slices(nlong)), b, s, start, e1, lslices, nlong, longs, long_cnt, long_part, insts, ctl, part);
     else
-      hipLaunchKernelGGL(k_primal_long<float>, dim3(grid_of_slices(nlong)), b, 0, s, lslices, nlong, longs, long_cnt, long_part, insts, ctl, lp);
+      launch_ev(k_primal_long<float>, dim3(grid_of_slices(nlong)), b, s, start, e1, lslices, nlong, longs, long_cnt, long_part, insts, ctl, part);
   }
   PDLP_CHECK_LAUNCH();
 }
 
 int launch_dual(hipStream_t s, int prec, int nslices, const SliceDesc *slices, int nlong,
                 const SliceDesc *lslices, const LongRow *longs, int32_t *long_cnt,
-                double *long_part, const InstDev *insts, const Ctl *ctl, double *part) {
+                double *long_part, const InstDev *insts, const Ctl *ctl, double *part,
+                hipEvent_t e0, hipEvent_t e1) {
   const dim3 b(kThreads);
+  if (nslices <= 0 && nlong <= 0) {
+    hipError_t r = e0 ? hipEventRecord(e0, s) : hipSuccess;
+    if (r == hipSuccess && e1) r = hipEventRecord(e1, s);
+    return (int)r;
+  }
   if (nslices > 0) {
+    hipEvent_t stop = nlong > 0 ? nullptr : e1;
     if (prec == 0)
-      hipLaunchKernelGGL(k_dual<double>, dim3(grid_of_slices(nslices)), b, 0, s, slices, nslices, insts, ctl, part);
+      launch_ev(k_dual<double>, dim3(grid_of_slices(nslices)), b, s, e0, stop, slices, nslices, insts, ctl, part);
     else
-      hipLaunchKernelGGL(k_dual<float>, dim3(grid_of_slices(nslices)), b, 0, s, slices, nslices, insts, ctl, part);
+      launch_ev(k_dual<float>, dim3(grid_of_slices(nslices)), b, s, e0, stop, slices, nslices, insts, ctl, part);
   }
   if (nlong > 0) {
-    double *lp = part + 4 * (size_t)std::max(nslices, 0);
+    hipEvent_t start = nslices > 0 ? nullptr : e0;
     if (prec == 0)
-      hipLaunchKernelGGL(k_dual_long<double>, dim3(grid_of_slices(nlong)), b, 0, s, lslices, nlong, longs, long_cnt, long_part, insts, ctl, lp);
+      launch_ev(k_dual_long<double>, dim3(grid_of_slices(nlong)), b, s, start, e1, lslices, nlong, longs, long_cnt, long_part, insts, ctl, part);
     else
-      hipLaunchKernelGGL(k_dual_long<float>, dim3(grid_of_slices(nlong)), b, 0, s, lslices, nlong, longs, long_cnt, long_part, insts, ctl, lp);
+      launch_ev(k_dual_long<float>, dim3(grid_of_slices(nlong)), b, s, start, e1, lslices, nlong, longs, long_cnt, long_part, insts, ctl, part);
   }
   PDLP_CHECK_LAUNCH();
 }
@@ -1395,6 +1426,7 @@ __global__ __launch_bounds__(kThreads) void k_slice_fill(const SliceDesc *__rest
   if (wv >= nsl) return;
   const int lane = threadIdx.x & 63;
   const SliceDesc sd = slices[wv];
+  if (sd.nrows < 0) return;
   int row, k, kstep, len;
   if (sd.nrows == 0) {
     row = sd.row0;

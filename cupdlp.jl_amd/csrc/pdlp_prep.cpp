@@ -195,7 +195,7 @@ void build_slices(const int32_t *ptr, int64_t R, SliceTable *out) {
       for (int32_t c = 0; c < nch; ++c) {
         const int64_t cl = std::min<int64_t>(kSellChunk, len0 - (int64_t)c * kSellChunk);
         const int32_t w = (int32_t)((cl + 63) / 64);
-        out->sl.push_back({0, (int32_t)r, (int32_t)off, w, 6, 0, lid, c});
+        out->sl.push_back({0, (int32_t)r, (int32_t)off, w, 6, 0, lid, c, 0});
         off += 64 * (int64_t)w;
       }
       ++r;
@@ -211,7 +211,7 @@ void build_slices(const int32_t *ptr, int64_t R, SliceTable *out) {
       w = std::max<int64_t>(w, (len + (1 << lg) - 1) >> lg);
       ++nr;
     }
-    out->sl.push_back({0, (int32_t)r, (int32_t)off, (int32_t)w, lg, (int32_t)nr, 0, 0});
+    out->sl.push_back({0, (int32_t)r, (int32_t)off, (int32_t)w, (int16_t)lg, (int16_t)nr, 0, 0, 0});
     off += 64 * w;
     r += nr;
   }
