@@ -38,6 +38,18 @@ sys.path.insert(0, os.path.join(REPO, "cupdlp.jl_amd"))
 
 HBM_PEAK_GBS = 8000.0
 
+# HBM bytes per launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE
+# (2 * FETCH + WRITE, factors from a plain 1 GiB copy on the same box), see
+# profiles/hbm_traffic_counters.md.  Keyed by (workload, precision, kernel)
+# and used only when this run's algorithmic bytes per launch match the
+# profiled run's (the same deterministic workload).
+MEASURED_TRAFFIC = {
+    ("batch", "fp64", "k_dual"): (109563308.65, 164.82e6),
+    ("batch", "fp64", "k_primal"): (135796999.84, 164.45e6),
+    ("c1", "fp64", "k_dual"): (195460496.0, 285.70e6),
+    ("c1", "fp64", "k_primal"): (229460496.0, 248.51e6),
+}
+
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
@@ -57,7 +69,14 @@ def batch_problems(count):
     return [random_lp(n=int(n), m=int(n) // 2, seed=i)[0] for i, n in enumerate(sizes)]
 
 
-def profile_roofline(prof, kernel_names, note):
+def measured_traffic(workload, precision, kernel, bytes_per_launch):
+    rec = MEASURED_TRAFFIC.get((workload, precision, kernel))
+    if rec is None or abs(rec[0] - bytes_per_launch) > 1e-6 * rec[0]:
+        return None
+    return rec[1]
+
+
+def profile_roofline(prof, kernel_names, note, workload=None, precision=None):
     pm, dm = prof.primal_ms, prof.dual_ms
     if prof.primal_launches == 0:
         return None
@@ -69,7 +88,8 @@ def profile_roofline(prof, kernel_names, note):
     bytes_per_launch = nbytes / launches
     achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
     return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": name,
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": measured_traffic(workload, precision, name, bytes_per_launch), "kernel": name,
             "avg_launch_us": round(avg_ms * 1e3, 2), "bytes_per_launch": bytes_per_launch,
             "launches": launches, "note": note,
             "other_kernel": {"primal_avg_us": round(pm / max(prof.primal_launches, 1) * 1e3, 2),
@@ -149,7 +169,8 @@ def run_batch(args, ctx, lib):
         "speedup_vs_fp64_serial": round(value / ser, 3),
     }
     roof = profile_roofline(prof, ("k_primal", "k_dual"),
-                            f"{args.count}-LP batch, {args.precision}, tol {args.tol:g}")
+                            f"{args.count}-LP batch, {args.precision}, tol {args.tol:g}",
+                            "batch", args.precision)
     cfg = {"workload": f"batch of {args.count} random LPs (seeds 0..{args.count - 1}, n log-uniform "
                        f"500..200000, m=n/2, 8 nnz/col)",
            "tolerance": args.tol, "precision": args.precision, "total_nnz": nnz}
@@ -246,7 +267,8 @@ def run_single(args, ctx, lib, which):
     lib.pdlp_set_profiling(ctx.handle, 0)
     prof = _abi.pdlp_kernel_profile()
     lib.pdlp_get_profile(ctx.handle, C.byref(prof))
-    roof = profile_roofline(prof, ("k_primal", "k_dual"), f"config {which}, {args.precision}")
+    roof = profile_roofline(prof, ("k_primal", "k_dual"), f"config {which}, {args.precision}",
+                            f"c{which}", args.precision)
     n, m, nnz = prob.num_variables, prob.num_constraints, prob.constraint_matrix.nnz
     cfg = {"workload": f"single LP n={n} m={m} nnz={nnz}", "iteration_limit": args.iters,
            "precision": args.precision}
