@@ -246,15 +246,21 @@ int Engine::upload(HostInst &h, std::string *err) {
       tw = (double *)d.t_val; aw = (double *)d.a_val;
       cw = (double *)d.c; lw = (double *)d.l; uw = (double *)d.u; bw = (double *)d.b;
     }
+    // the slice tables (A^T first, then A): the row kernels of the
+    // rescaling walk them, and they are reused to fill the sliced matrices
+    const size_t nps = P.ps.sl.size(), nds = P.ds.sl.size();
+    SliceDesc *tab = (SliceDesc *)scratch(sizeof(SliceDesc) * (nps + nds));
     double *rv = (double *)scratch(8 * m), *vv = (double *)scratch(8 * n);
     int32_t *cursor = (int32_t *)scratch(4 * m);
     double *npart = (double *)scratch(8 * 5 * 256);
-    if (!tw || !aw || !cw || !lw || !uw || !bw || !rv || !vv || !cursor || !npart) {
+    if (!tw || !aw || !cw || !lw || !uw || !bw || !tab || !rv || !vv || !cursor || !npart) {
       for (void *p : tmp) pool_->put(p);
       if (err) *err = "device allocation for preconditioning failed";
       return -2;
     }
     if (!norm_pinned_) HIPCHK(hipHostMalloc((void **)&norm_pinned_, 8 * 5 * 256, 0));
+    HIPCHK(put(tab, P.ps.sl.data(), sizeof(SliceDesc) * nps));
+    HIPCHK(put(tab + nps, P.ds.sl.data(), sizeof(SliceDesc) * nds));
     HIPCHK(put(tw, P.t_val.data(), 8 * nnz));
     HIPCHK(put(cw, P.oc.data(), 8 * n));
     HIPCHK(put(lw, P.ol.data(), 8 * n));
@@ -270,6 +276,8 @@ int Engine::upload(HostInst &h, std::string *err) {
     pa.c = cw; pa.l = lw; pa.u = uw; pa.b = bw;
     pa.D = (double *)d.D; pa.E = (double *)d.E; pa.r = rv; pa.v = vv;
     pa.norm_part = npart; pa.norm_host = norm_pinned_;
+    pa.ps = tab; pa.nps = (int)nps;
+    pa.ds = tab + nps; pa.nds = (int)nds;
     pa.ruiz_iterations = prm_.l_inf_ruiz_iterations;
     pa.has_pc = prm_.has_pock_chambolle_alpha != 0;
     pa.pc_alpha = prm_.pock_chambolle_alpha;
@@ -293,17 +301,20 @@ int Engine::upload(HostInst &h, std::string *err) {
     if (e == hipSuccess && n) e = hipMemcpyAsync(P.D.data(), d.D, 8 * n, hipMemcpyDeviceToHost, s_);
     if (e == hipSuccess && m) e = hipMemcpyAsync(P.E.data(), d.E, 8 * m, hipMemcpyDeviceToHost, s_);
     if (e == hipSuccess) e = hipStreamSynchronize(s_);
+    int rc = 0;
+    if (e == hipSuccess) rc = fill_slices(h, err, tab);
     for (void *p : tmp) pool_->put(p);
     if (e != hipSuccess) {
       if (err) *err = std::string("device preconditioning failed: ") + hipGetErrorString(e);
       return -(int)e;
     }
+    if (rc) return rc;
     P.max_abs_scaled = po.max_abs;
     P.norm_b_scaled = po.norm_b;
     P.norm_c_scaled = po.norm_c;
     // host copies of the scaled data are not needed any more
     std::vector<double>().swap(P.t_val);
-    return fill_slices(h, err);
+    return 0;
   }
 
   // host-prepared problem: upload the scaled data
@@ -340,20 +351,27 @@ int Engine::upload(HostInst &h, std::string *err) {
 }
 
 // Builds the sliced copies of the scaled A^T and A from the device CSR.
-int Engine::fill_slices(HostInst &h, std::string *err) {
+// dev_tab: the instance's slice tables already on the device (A^T's, then
+// A's), or null to upload them here.
+int Engine::fill_slices(HostInst &h, std::string *err, const SliceDesc *dev_tab) {
   const Prepared &P = h.P;
   const InstDev &d = h.dev;
   const size_t np = P.ps.sl.size(), nd = P.ds.sl.size();
   if (np + nd == 0) return 0;
-  SliceDesc *tab = (SliceDesc *)pool_->get(sizeof(SliceDesc) * (np + nd));
-  if (!tab) {
-    if (err) *err = "device allocation of the slice tables failed";
-    return -2;
-  }
+  SliceDesc *own = nullptr;
+  const SliceDesc *tab = dev_tab;
   hipError_t e = hipSuccess;
-  if (np) e = hipMemcpyAsync(tab, P.ps.sl.data(), sizeof(SliceDesc) * np, hipMemcpyHostToDevice, s_);
-  if (e == hipSuccess && nd)
-    e = hipMemcpyAsync(tab + np, P.ds.sl.data(), sizeof(SliceDesc) * nd, hipMemcpyHostToDevice, s_);
+  if (!tab) {
+    own = (SliceDesc *)pool_->get(sizeof(SliceDesc) * (np + nd));
+    if (!own) {
+      if (err) *err = "device allocation of the slice tables failed";
+      return -2;
+    }
+    if (np) e = hipMemcpyAsync(own, P.ps.sl.data(), sizeof(SliceDesc) * np, hipMemcpyHostToDevice, s_);
+    if (e == hipSuccess && nd)
+      e = hipMemcpyAsync(own + np, P.ds.sl.data(), sizeof(SliceDesc) * nd, hipMemcpyHostToDevice, s_);
+    tab = own;
+  }
   if (e == hipSuccess)
     e = (hipError_t)launch_slice_fill(s_, prec_, (int)np, tab, d.t_ptr, d.t_col, d.t_val,
                                       (int32_t *)d.ts_col, (void *)d.ts_val);
@@ -361,7 +379,7 @@ int Engine::fill_slices(HostInst &h, std::string *err) {
     e = (hipError_t)launch_slice_fill(s_, prec_, (int)nd, tab + np, d.a_ptr, d.a_col, d.a_val,
                                       (int32_t *)d.as_col, (void *)d.as_val);
   if (e == hipSuccess) e = hipStreamSynchronize(s_);
-  pool_->put(tab);
+  if (own) pool_->put(own);
   if (e != hipSuccess) {
     if (err) *err = std::string("building the sliced matrices failed: ") + hipGetErrorString(e);
     return -(int)e;

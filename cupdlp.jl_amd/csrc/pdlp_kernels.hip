@@ -407,14 +407,9 @@ __device__ __forceinline__ void primal_update(const PrimalView<T> &v, int j, con
 // One wave per slice; each row's operands are loaded before the gathers of
 // its sum.
 template <typename T>
-__global__ __launch_bounds__(kThreads) void k_primal(const SliceDesc *__restrict__ slices, int nsl,
-                                                     const InstDev *__restrict__ insts,
-                                                     const Ctl *__restrict__ ctl,
-                                                     double *__restrict__ part) {
-  const int wv = (int)blockIdx.x * (kThreads / 64) + ((int)threadIdx.x >> 6);
-  if (wv >= nsl) return;
-  const int lane = threadIdx.x & 63;
-  const SliceDesc sd = slices[wv];
+__device__ __forceinline__ void primal_slice(const SliceDesc &sd, int lane,
+                                             const InstDev *__restrict__ insts, const Ctl *ctl,
+                                             double *part) {
   if (sd.nrows < 0) return;
   const Ctl *C = &ctl[sd.inst];
   if (C->active == 0) return;
@@ -436,6 +431,16 @@ __global__ __launch_bounds__(kThreads) void k_primal(const SliceDesc *__restrict
     part[2 * sd.part] = acc.s;
     part[2 * sd.part + 1] = acc.e;
   }
+}
+
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_primal(const SliceDesc *__restrict__ slices, int nsl,
+                                                     const InstDev *__restrict__ insts,
+                                                     const Ctl *__restrict__ ctl,
+                                                     double *__restrict__ part) {
+  const int wv = (int)blockIdx.x * (kThreads / 64) + ((int)threadIdx.x >> 6);
+  if (wv >= nsl) return;
+  primal_slice<T>(slices[wv], threadIdx.x & 63, insts, ctl, part);
 }
 
 // The chunk slices of rows longer than kSellChunk entries: each wave sums one
@@ -645,14 +650,9 @@ __device__ __forceinline__ void dual_update(const DualView<T> &v, int i, const D
 // update of y and A x, the projected dual step and the partial sums of dy^2
 // and Adx*dy of each slice (part[4p..4p+3] for its partial index p).
 template <typename T>
-__global__ __launch_bounds__(kThreads) void k_dual(const SliceDesc *__restrict__ slices, int nsl,
-                                                   const InstDev *__restrict__ insts,
-                                                   const Ctl *__restrict__ ctl,
-                                                   double *__restrict__ part) {
-  const int wv = (int)blockIdx.x * (kThreads / 64) + ((int)threadIdx.x >> 6);
-  if (wv >= nsl) return;
-  const int lane = threadIdx.x & 63;
-  const SliceDesc sd = slices[wv];
+__device__ __forceinline__ void dual_slice(const SliceDesc &sd, int lane,
+                                           const InstDev *__restrict__ insts, const Ctl *ctl,
+                                           double *part) {
   if (sd.nrows < 0) return;
   const Ctl *C = &ctl[sd.inst];
   if (C->active == 0) return;
@@ -674,6 +674,16 @@ __global__ __launch_bounds__(kThreads) void k_dual(const SliceDesc *__restrict__
     part[4 * sd.part + 2] = acc_int.s;
     part[4 * sd.part + 3] = acc_int.e;
   }
+}
+
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_dual(const SliceDesc *__restrict__ slices, int nsl,
+                                                   const InstDev *__restrict__ insts,
+                                                   const Ctl *__restrict__ ctl,
+                                                   double *__restrict__ part) {
+  const int wv = (int)blockIdx.x * (kThreads / 64) + ((int)threadIdx.x >> 6);
+  if (wv >= nsl) return;
+  dual_slice<T>(slices[wv], threadIdx.x & 63, insts, ctl, part);
 }
 
 // Chunk slices of long rows of A (see k_primal_long).
@@ -710,16 +720,14 @@ __global__ __launch_bounds__(kThreads) void k_dual_long(const SliceDesc *__restr
 // work-group of kStepThreads per instance.
 constexpr int kStepThreads = 1024;
 
-__global__ __launch_bounds__(kStepThreads) void k_step(const InstDev *__restrict__ insts,
-                                                       Ctl *__restrict__ ctl,
-                                                       const double *__restrict__ ppart,
-                                                       const double *__restrict__ dpart,
-                                                       const double *__restrict__ red_tab,
-                                                       const double *__restrict__ grow_tab,
-                                                       int64_t tab_len, double red_exp,
-                                                       double grow_exp) {
-  __shared__ double sc[3][2 * kStepThreads / 64];
-  const int q = blockIdx.x;
+// The step of instance q by a work-group of kStepThreads threads; sc is
+// LDS scratch of 3 x 2 kStepThreads/64 doubles.  Contains a barrier.
+__device__ __forceinline__ void step_instance(int q, const InstDev *__restrict__ insts, Ctl *ctl,
+                                              const double *ppart, const double *dpart,
+                                              const double *__restrict__ red_tab,
+                                              const double *__restrict__ grow_tab,
+                                              int64_t tab_len, double red_exp, double grow_exp,
+                                              double (*sc)[2 * kStepThreads / 64]) {
   Ctl *C = &ctl[q];
   if (!C->active) return;
   const InstDev &I = insts[q];
@@ -754,6 +762,19 @@ __global__ __launch_bounds__(kStepThreads) void k_step(const InstDev *__restrict
   }
   step_rule(C, dd_val(r[0]), dd_val(r[1]), dd_val(r[2]), red_tab, grow_tab, tab_len, red_exp,
             grow_exp);
+}
+
+__global__ __launch_bounds__(kStepThreads) void k_step(const InstDev *__restrict__ insts,
+                                                       Ctl *__restrict__ ctl,
+                                                       const double *__restrict__ ppart,
+                                                       const double *__restrict__ dpart,
+                                                       const double *__restrict__ red_tab,
+                                                       const double *__restrict__ grow_tab,
+                                                       int64_t tab_len, double red_exp,
+                                                       double grow_exp) {
+  __shared__ double sc[3][2 * kStepThreads / 64];
+  step_instance(blockIdx.x, insts, ctl, ppart, dpart, red_tab, grow_tab, tab_len, red_exp,
+                grow_exp, sc);
 }
 
 // ---------------------------------------------------------------------------

@@ -3,6 +3,8 @@
 #include <hip/hip_runtime_api.h>
 #include <stdint.h>
 
+#include "pdlp_device.h"
+
 namespace pdlp {
 
 struct PrecondArgs {
@@ -17,6 +19,10 @@ struct PrecondArgs {
   double *r = nullptr, *v = nullptr;                 // scratch (m), (n)
   double *norm_part = nullptr;                       // scratch, 5 * 256 doubles
   double *norm_host = nullptr;                       // pinned host, 5 * 256 doubles
+  // instance-local slice tables of A^T (ps) and A (ds) on the device; the
+  // row kernels walk them
+  const SliceDesc *ps = nullptr, *ds = nullptr;
+  int nps = 0, nds = 0;
   int64_t ruiz_iterations = 10;
   bool has_pc = true;
   double pc_alpha = 1.0;

@@ -7,7 +7,9 @@
 // Every value equals the host/oracle computation bit for bit: maxima are
 // exact, square roots and divisions are correctly rounded, each entry is
 // rescaled as ((1/r_i) * a_ij) * (1/v_j) in both layouts, and the
-// Pock-Chambolle sums are compensated sums (order independent).  Column
+// Pock-Chambolle sums are compensated sums (order independent).  The row
+// kernels walk the slice tables of the step kernels, so a row gets as many
+// lanes as its length needs and its entries are read coalesced.  Column
 // order within a CSR row follows the scatter's arrival order; the iteration
 // kernels' row sums are order independent, so results do not depend on it.
 #include <hip/hip_runtime.h>
@@ -73,79 +75,154 @@ __global__ __launch_bounds__(kPT) void k_scatter(int n, const int32_t *__restric
   }
 }
 
-// sqrt(max |a|) over each segment, 0 -> 1 (Ruiz-inf, preprocess.jl:412-454).
-__global__ __launch_bounds__(kPT) void k_segmax_sqrt(int rows, const int32_t *__restrict__ ptr,
-                                                     const double *__restrict__ val,
-                                                     double *__restrict__ out) {
-  const int i = blockIdx.x * kPT + threadIdx.x;
-  if (i >= rows) return;
-  double mx = 0.0;
-  for (int k = ptr[i]; k < ptr[i + 1]; ++k) mx = pjl_max(mx, fabs(val[k]));
-  mx = pjl_max(mx, 0.0);  // column: max with the (zero) objective-matrix column
-  double s = sqrt(mx);
-  out[i] = s == 0.0 ? 1.0 : s;
+// The rows of a slice table (SliceDesc): lane `lane` of a normal slice
+// covers entries k = j, j + L, ... of row row0 + g (L = 2^lg lanes per row,
+// g = lane / L, j = lane mod L); a chunk slice covers its chunk of one long
+// row with all 64 lanes.  `head` marks the lane that owns the row's
+// per-row work (the group's first lane; in a chunk slice lane 0 of chunk 0).
+struct SliceRow {
+  int row, k, step, end;
+  bool valid, head;
+};
+
+__device__ __forceinline__ SliceRow slice_row(const SliceDesc &sd, int lane,
+                                              const int32_t *__restrict__ ptr, bool whole_row) {
+  SliceRow r;
+  if (sd.nrows == 0) {
+    r.row = sd.row0;
+    const int len = ptr[r.row + 1] - ptr[r.row];
+    r.step = 64;
+    r.head = sd.chunk == 0 && lane == 0;
+    if (whole_row) {  // reductions: chunk 0's wave takes the whole row
+      r.valid = sd.chunk == 0;
+      r.k = lane;
+      r.end = len;
+    } else {
+      r.valid = true;
+      r.k = sd.chunk * kSellChunk + lane;
+      r.end = min(len, (sd.chunk + 1) * kSellChunk);
+    }
+  } else {
+    const int g = lane >> sd.lg, L = 1 << sd.lg;
+    r.row = sd.row0 + g;
+    r.valid = sd.nrows > 0 && g < sd.nrows;
+    r.head = r.valid && (lane & (L - 1)) == 0;
+    r.k = lane & (L - 1);
+    r.step = L;
+    r.end = r.valid ? ptr[r.row + 1] - ptr[r.row] : 0;
+  }
+  return r;
 }
 
-// sqrt(sum |a|^e) over each segment (compensated), 0 -> 1; e == 0 counts
-// every entry of the dense dimension (0^0 = 1), as Julia's mapreduce does.
-__global__ __launch_bounds__(kPT) void k_segpow_sqrt(int rows, const int32_t *__restrict__ ptr,
+// Lanes of a row: a normal slice's group of 2^lg lanes, a chunk slice's wave.
+__device__ __forceinline__ int slice_group_lg(const SliceDesc &sd) {
+  return sd.nrows == 0 ? 6 : sd.lg;
+}
+
+// sqrt(max |a|) over each row, 0 -> 1 (Ruiz-inf, preprocess.jl:412-454).
+__global__ __launch_bounds__(kPT) void k_segmax_sqrt(const SliceDesc *__restrict__ sl, int nsl,
+                                                     const int32_t *__restrict__ ptr,
+                                                     const double *__restrict__ val,
+                                                     double *__restrict__ out) {
+  const int wv = (int)blockIdx.x * (kPT / 64) + ((int)threadIdx.x >> 6);
+  if (wv >= nsl) return;
+  const int lane = threadIdx.x & 63;
+  const SliceDesc sd = sl[wv];
+  const SliceRow r = slice_row(sd, lane, ptr, true);
+  double mx = 0.0;
+  if (r.valid) {
+    const double *v = val + ptr[r.row];
+    for (int k = r.k; k < r.end; k += r.step) mx = pjl_max(mx, fabs(v[k]));
+  }
+  for (int o = (1 << slice_group_lg(sd)) >> 1; o > 0; o >>= 1) mx = pjl_max(mx, __shfl_xor(mx, o, 64));
+  if (r.valid && r.head) {
+    mx = pjl_max(mx, 0.0);  // column: max with the (zero) objective-matrix column
+    const double q = sqrt(mx);
+    out[r.row] = q == 0.0 ? 1.0 : q;
+  }
+}
+
+// sqrt(sum |a|^e) over each row (compensated), 0 -> 1; e == 0 counts every
+// entry of the dense dimension (0^0 = 1), as Julia's mapreduce does.
+__global__ __launch_bounds__(kPT) void k_segpow_sqrt(const SliceDesc *__restrict__ sl, int nsl,
+                                                     const int32_t *__restrict__ ptr,
                                                      const double *__restrict__ val, double ex,
                                                      double dense_len, double *__restrict__ out) {
-  const int i = blockIdx.x * kPT + threadIdx.x;
-  if (i >= rows) return;
-  double s;
-  if (ex == 0.0) {
-    s = dense_len;
-  } else {
-    PDD acc{0.0, 0.0};
-    for (int k = ptr[i]; k < ptr[i + 1]; ++k) {
-      const double a = fabs(val[k]);
+  const int wv = (int)blockIdx.x * (kPT / 64) + ((int)threadIdx.x >> 6);
+  if (wv >= nsl) return;
+  const int lane = threadIdx.x & 63;
+  const SliceDesc sd = sl[wv];
+  const SliceRow r = slice_row(sd, lane, ptr, true);
+  PDD acc{0.0, 0.0};
+  if (r.valid && ex != 0.0) {
+    const double *v = val + ptr[r.row];
+    for (int k = r.k; k < r.end; k += r.step) {
+      const double a = fabs(v[k]);
       pdd_add(acc, ex == 1.0 ? a : pow(a, ex));
     }
-    s = acc.s + acc.e;
   }
-  const double r = sqrt(s);
-  out[i] = r == 0.0 ? 1.0 : r;
+  for (int o = (1 << slice_group_lg(sd)) >> 1; o > 0; o >>= 1)
+    pdd_merge(acc, PDD{__shfl_xor(acc.s, o, 64), __shfl_xor(acc.e, o, 64)});
+  if (r.valid && r.head) {
+    const double q = sqrt(ex == 0.0 ? dense_len : acc.s + acc.e);
+    out[r.row] = q == 0.0 ? 1.0 : q;
+  }
 }
 
 // scale_problem (preprocess.jl:541-561) over the columns: A^T rows, c, l, u, D.
-__global__ __launch_bounds__(kPT) void k_scale_cols(int n, const int32_t *__restrict__ t_ptr,
+__global__ __launch_bounds__(kPT) void k_scale_cols(const SliceDesc *__restrict__ sl, int nsl,
+                                                    const int32_t *__restrict__ t_ptr,
                                                     const int32_t *__restrict__ t_col,
                                                     double *__restrict__ t_val,
                                                     const double *__restrict__ r,
                                                     const double *__restrict__ v,
                                                     double *__restrict__ c, double *__restrict__ l,
                                                     double *__restrict__ u, double *__restrict__ D) {
-  const int j = blockIdx.x * kPT + threadIdx.x;
-  if (j >= n) return;
+  const int wv = (int)blockIdx.x * (kPT / 64) + ((int)threadIdx.x >> 6);
+  if (wv >= nsl) return;
+  const int lane = threadIdx.x & 63;
+  const SliceRow w = slice_row(sl[wv], lane, t_ptr, false);
+  if (!w.valid) return;
+  const int j = w.row;
   const double vj = v[j];
-  c[j] /= vj;
-  u[j] *= vj;
-  l[j] *= vj;
-  D[j] *= vj;
+  if (w.head) {
+    c[j] /= vj;
+    u[j] *= vj;
+    l[j] *= vj;
+    D[j] *= vj;
+  }
   const double iv = 1.0 / vj;
-  for (int k = t_ptr[j]; k < t_ptr[j + 1]; ++k) {
-    const double ir = 1.0 / r[t_col[k]];
-    t_val[k] = (ir * t_val[k]) * iv;
+  const int b = t_ptr[j];
+  for (int k = w.k; k < w.end; k += w.step) {
+    const double ir = 1.0 / r[t_col[b + k]];
+    t_val[b + k] = (ir * t_val[b + k]) * iv;
   }
 }
 
 // ... and over the rows: A rows, b, E.
-__global__ __launch_bounds__(kPT) void k_scale_rows(int m, const int32_t *__restrict__ a_ptr,
+__global__ __launch_bounds__(kPT) void k_scale_rows(const SliceDesc *__restrict__ sl, int nsl,
+                                                    const int32_t *__restrict__ a_ptr,
                                                     const int32_t *__restrict__ a_col,
                                                     double *__restrict__ a_val,
                                                     const double *__restrict__ r,
                                                     const double *__restrict__ v,
                                                     double *__restrict__ b, double *__restrict__ E) {
-  const int i = blockIdx.x * kPT + threadIdx.x;
-  if (i >= m) return;
+  const int wv = (int)blockIdx.x * (kPT / 64) + ((int)threadIdx.x >> 6);
+  if (wv >= nsl) return;
+  const int lane = threadIdx.x & 63;
+  const SliceRow w = slice_row(sl[wv], lane, a_ptr, false);
+  if (!w.valid) return;
+  const int i = w.row;
   const double ri = r[i];
-  b[i] /= ri;
-  E[i] *= ri;
+  if (w.head) {
+    b[i] /= ri;
+    E[i] *= ri;
+  }
   const double ir = 1.0 / ri;
-  for (int p = a_ptr[i]; p < a_ptr[i + 1]; ++p) {
-    const double iv = 1.0 / v[a_col[p]];
-    a_val[p] = (ir * a_val[p]) * iv;
+  const int p0 = a_ptr[i];
+  for (int p = w.k; p < w.end; p += w.step) {
+    const double iv = 1.0 / v[a_col[p0 + p]];
+    a_val[p0 + p] = (ir * a_val[p0 + p]) * iv;
   }
 }
 
@@ -193,6 +270,7 @@ __global__ __launch_bounds__(kPT) void k_to_float(int64_t len, const double *__r
 }
 
 inline unsigned nb(int64_t len) { return (unsigned)((len + kPT - 1) / kPT); }
+inline unsigned nbs(int nsl) { return (unsigned)((nsl + kPT / 64 - 1) / (kPT / 64)); }
 
 }  // namespace
 
@@ -210,32 +288,34 @@ hipError_t precond_run(hipStream_t s, const PrecondArgs &a, PrecondOut *out) {
   PCHK(hipGetLastError());
   if (a.after_scatter) PCHK(a.after_scatter(s, a.after_scatter_ctx));
   for (int64_t it = 0; it < a.ruiz_iterations; ++it) {
-    if (n > 0)
-      hipLaunchKernelGGL(k_segmax_sqrt, dim3(nb(n)), dim3(kPT), 0, s, n, a.t_ptr, a.t_val, a.v);
-    if (m > 0)
-      hipLaunchKernelGGL(k_segmax_sqrt, dim3(nb(m)), dim3(kPT), 0, s, m, a.a_ptr, a.a_val, a.r);
-    if (n > 0)
-      hipLaunchKernelGGL(k_scale_cols, dim3(nb(n)), dim3(kPT), 0, s, n, a.t_ptr, a.t_col, a.t_val,
-                         a.r, a.v, a.c, a.l, a.u, a.D);
-    if (m > 0)
-      hipLaunchKernelGGL(k_scale_rows, dim3(nb(m)), dim3(kPT), 0, s, m, a.a_ptr, a.a_col, a.a_val,
-                         a.r, a.v, a.b, a.E);
+    if (a.nps > 0)
+      hipLaunchKernelGGL(k_segmax_sqrt, dim3(nbs(a.nps)), dim3(kPT), 0, s, a.ps, a.nps, a.t_ptr,
+                         a.t_val, a.v);
+    if (a.nds > 0)
+      hipLaunchKernelGGL(k_segmax_sqrt, dim3(nbs(a.nds)), dim3(kPT), 0, s, a.ds, a.nds, a.a_ptr,
+                         a.a_val, a.r);
+    if (a.nps > 0)
+      hipLaunchKernelGGL(k_scale_cols, dim3(nbs(a.nps)), dim3(kPT), 0, s, a.ps, a.nps, a.t_ptr,
+                         a.t_col, a.t_val, a.r, a.v, a.c, a.l, a.u, a.D);
+    if (a.nds > 0)
+      hipLaunchKernelGGL(k_scale_rows, dim3(nbs(a.nds)), dim3(kPT), 0, s, a.ds, a.nds, a.a_ptr,
+                         a.a_col, a.a_val, a.r, a.v, a.b, a.E);
     PCHK(hipGetLastError());
   }
   if (a.has_pc) {
     const double ec = 2.0 - a.pc_alpha, er = a.pc_alpha;
-    if (n > 0)
-      hipLaunchKernelGGL(k_segpow_sqrt, dim3(nb(n)), dim3(kPT), 0, s, n, a.t_ptr, a.t_val, ec,
-                         (double)m, a.v);
-    if (m > 0)
-      hipLaunchKernelGGL(k_segpow_sqrt, dim3(nb(m)), dim3(kPT), 0, s, m, a.a_ptr, a.a_val, er,
-                         (double)n, a.r);
-    if (n > 0)
-      hipLaunchKernelGGL(k_scale_cols, dim3(nb(n)), dim3(kPT), 0, s, n, a.t_ptr, a.t_col, a.t_val,
-                         a.r, a.v, a.c, a.l, a.u, a.D);
-    if (m > 0)
-      hipLaunchKernelGGL(k_scale_rows, dim3(nb(m)), dim3(kPT), 0, s, m, a.a_ptr, a.a_col, a.a_val,
-                         a.r, a.v, a.b, a.E);
+    if (a.nps > 0)
+      hipLaunchKernelGGL(k_segpow_sqrt, dim3(nbs(a.nps)), dim3(kPT), 0, s, a.ps, a.nps, a.t_ptr,
+                         a.t_val, ec, (double)m, a.v);
+    if (a.nds > 0)
+      hipLaunchKernelGGL(k_segpow_sqrt, dim3(nbs(a.nds)), dim3(kPT), 0, s, a.ds, a.nds, a.a_ptr,
+                         a.a_val, er, (double)n, a.r);
+    if (a.nps > 0)
+      hipLaunchKernelGGL(k_scale_cols, dim3(nbs(a.nps)), dim3(kPT), 0, s, a.ps, a.nps, a.t_ptr,
+                         a.t_col, a.t_val, a.r, a.v, a.c, a.l, a.u, a.D);
+    if (a.nds > 0)
+      hipLaunchKernelGGL(k_scale_rows, dim3(nbs(a.nds)), dim3(kPT), 0, s, a.ds, a.nds, a.a_ptr,
+                         a.a_col, a.a_val, a.r, a.v, a.b, a.E);
     PCHK(hipGetLastError());
   }
   // norms: max |A~| and ||b~||, ||c~|| with a fixed grid so the combine order is fixed
