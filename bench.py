@@ -135,7 +135,7 @@ def run_batch(args, ctx, lib):
         ctx.batch_solve(problems, params)
     lib.pdlp_get_profile(ctx.handle, C.byref(_abi.pdlp_kernel_profile()))
     lib.pdlp_set_profiling(ctx.handle, 1)
-    lat, iters, bad = [], [], 0
+    lat, iters, bad, fallbacks = [], [], 0, 0
     t0 = time.time()
     for k in range(args.steps):
         outs = ctx.batch_solve(problems, params)
@@ -143,6 +143,7 @@ def run_batch(args, ctx, lib):
             lat.append(o.solve_time_sec)
             iters.append(o.iteration_count)
             bad += int(o.termination_string != "OPTIMAL")
+            fallbacks += int(o.fp64_fallback)
         log(f"step {k}: {time.time() - t0:.2f} s cumulative")
     dt = time.time() - t0
     lib.pdlp_set_profiling(ctx.handle, 0)
@@ -163,6 +164,7 @@ def run_batch(args, ctx, lib):
         "p50_latency_s": round(float(np.percentile(lat, 50)), 4),
         "p99_latency_s": round(float(np.percentile(lat, 99)), 4),
         "not_optimal": bad,
+        "fp64_fallbacks": fallbacks,
         "median_iterations": int(np.median(iters)),
         "fp64_serial_solves_per_s": round(ser, 3),
         "fp64_serial_sample": nser,

@@ -207,6 +207,64 @@ int pdlp::prepare_instance(const pdlp_lp *lp, const pdlp_params *prm, HostInst *
   return 0;
 }
 
+// FP32 + refinement: LPs whose refinement stalled are solved again, together,
+// in plain FP64, within what is left of the caller's iteration and time
+// limits.  Their results become the FP64 solves', with the FP32 iteration
+// and attempt counts added and fp64_fallback = 1.
+struct FallbackItem {
+  const pdlp_lp *lp;
+  pdlp_result *res;
+  pdlp_stats_cb cb;
+  void *user;
+};
+
+static int fp64_fallback(pdlp_ctx *ctx, const pdlp_params *params,
+                         const std::vector<FallbackItem> &todo, double t0) {
+  if (todo.empty()) return 0;
+  pdlp_params p = *params;
+  p.precision = PDLP_PRECISION_FP64;
+  struct Prior {
+    int64_t iters, attempts;
+    int32_t rounds;
+  };
+  std::vector<Prior> prior;
+  std::vector<pdlp_result *> ran;
+  Engine eng(ctx->stream, &ctx->pool, p, &ctx->prof);
+  const double elapsed = now_sec() - t0;
+  for (const FallbackItem &f : todo) {
+    // the FP64 solve gets what is left of the caller's iteration and time limits
+    const int64_t left_it = (int64_t)p.termination_criteria.iteration_limit - f.res->iteration_count;
+    const double left_t = p.termination_criteria.time_sec_limit - elapsed;
+    if (left_it <= 0 || !(left_t > 0)) continue;
+    prior.push_back({f.res->iteration_count, f.res->total_attempts, f.res->refinement_rounds});
+    ran.push_back(f.res);
+    auto h = std::make_unique<HostInst>();
+    int rc = prepare_instance(f.lp, &p, h.get(), &ctx->err);
+    if (rc) return rc;
+    h->res = f.res;
+    h->t_submit = t0;
+    h->cb = f.cb;
+    h->user = f.user;
+    h->iteration_limit_override = left_it;
+    h->time_limit_override = left_t;
+    rc = eng.add(std::move(h), &ctx->err);
+    if (rc) return rc;
+  }
+  if (ran.empty()) return 0;
+  int rc = eng.run(&ctx->err);
+  if (rc) return rc;
+  for (size_t k = 0; k < ran.size(); ++k) {
+    pdlp_result *r = ran[k];
+    const int64_t it = (int64_t)r->iteration_count + prior[k].iters;
+    r->iteration_count = (int32_t)std::min<int64_t>(it, INT32_MAX);
+    r->total_attempts += prior[k].attempts;
+    r->refinement_rounds = prior[k].rounds;
+    r->fp64_fallback = 1;
+    r->solve_time_sec = now_sec() - t0;
+  }
+  return 0;
+}
+
 extern "C" int pdlp_solve(pdlp_ctx *ctx, const pdlp_lp *lp, const pdlp_params *params,
                           pdlp_result *result, pdlp_stats_cb cb, void *user) {
   if (!ctx) return 1;
@@ -231,12 +289,21 @@ extern "C" int pdlp_solve(pdlp_ctx *ctx, const pdlp_lp *lp, const pdlp_params *p
   h->cb = cb;
   h->user = user;
   result->refinement_rounds = 0;
-  Engine eng(ctx->stream, &ctx->pool, *params, &ctx->prof);
-  rc = eng.add(std::move(h), &ctx->err);
-  if (rc) return rc;
-  result->preprocess_time_sec = now_sec() - t0;
-  rc = eng.run(&ctx->err);
-  if (rc) return rc;
+  result->fp64_fallback = 0;
+  bool stalled = false;
+  {
+    Engine eng(ctx->stream, &ctx->pool, *params, &ctx->prof);
+    rc = eng.add(std::move(h), &ctx->err);
+    if (rc) return rc;
+    result->preprocess_time_sec = now_sec() - t0;
+    rc = eng.run(&ctx->err);
+    if (rc) return rc;
+    stalled = eng.inst(0).stalled;
+  }
+  if (stalled) {
+    rc = fp64_fallback(ctx, params, {{lp, result, cb, user}}, t0);
+    if (rc) return rc;
+  }
   result->solve_time_sec = now_sec() - t0;
   return 0;
 }
@@ -253,6 +320,8 @@ extern "C" int pdlp_batch_solve(pdlp_ctx *ctx, int32_t count, const pdlp_lp *lps
   int rc = check_params(params, &ctx->err);
   if (rc) return rc;
   (void)hipSetDevice(ctx->device);
+  std::vector<FallbackItem> todo;
+  {
   Engine eng(ctx->stream, &ctx->pool, *params, &ctx->prof);
   for (int32_t i = 0; i < count; ++i) {
     if ((lps[i].n > 0 && !results[i].x) || (lps[i].m > 0 && !results[i].y)) {
@@ -267,14 +336,20 @@ extern "C" int pdlp_batch_solve(pdlp_ctx *ctx, int32_t count, const pdlp_lp *lps
     }
     h->res = &results[i];
     h->t_submit = t0;
+    results[i].refinement_rounds = 0;
+    results[i].fp64_fallback = 0;
     rc = eng.add(std::move(h), &ctx->err);
     if (rc) return rc;
   }
   const double t1 = now_sec();
   rc = eng.run(&ctx->err);
   if (rc) return rc;
-  for (int32_t i = 0; i < count; ++i) results[i].preprocess_time_sec = t1 - t0;
-  return 0;
+  for (int32_t i = 0; i < count; ++i) {
+    results[i].preprocess_time_sec = t1 - t0;
+    if (eng.inst((size_t)i).stalled) todo.push_back({&lps[i], &results[i], nullptr, nullptr});
+  }
+  }  // the FP32 engine's device memory is returned before the FP64 solves
+  return fp64_fallback(ctx, params, todo, t0);
 }
 
 // ------------------------------------------------------------------ serving

@@ -772,6 +772,9 @@ void Engine::host_eval_logic(size_t q, const double *o) {
     tc.eps_optimal_absolute = tol;
     tc.eps_optimal_relative = tol;
   }
+  if (h.iteration_limit_override >= 0)
+    tc.iteration_limit = (int32_t)std::min<int64_t>(tc.iteration_limit, h.iteration_limit_override);
+  if (h.time_limit_override >= 0) tc.time_sec_limit = std::min(tc.time_sec_limit, h.time_limit_override);
   const QpCache &qc = h.qc_cur;
   const double c0 = P.c0;
   C.kkt_passes += 2.0;
@@ -831,8 +834,10 @@ void Engine::host_eval_logic(size_t q, const double *o) {
   if (C.numerical_error && reason == 0) reason = PDLP_TERMINATION_REASON_NUMERICAL_ERROR;
   // a correction solve that has not met its tolerance within its cap is
   // refined anyway (the outer rounds are bounded by refine_max_outer)
-  if (reason == 0 && prec_ && h.inner_cap > 0 && (int64_t)st.iteration_number >= h.inner_cap)
+  if (reason == 0 && prec_ && h.inner_cap > 0 && (int64_t)st.iteration_number >= h.inner_cap) {
     reason = PDLP_TERMINATION_REASON_ITERATION_LIMIT;
+    ++h.capped_rounds;
+  }
   if (h.eval_iter < 10 && (reason == PDLP_TERMINATION_REASON_PRIMAL_INFEASIBLE ||
                            reason == PDLP_TERMINATION_REASON_DUAL_INFEASIBLE))
     reason = 0;
@@ -1174,8 +1179,14 @@ void Engine::refine_logic(size_t q, const double *o) {
     reason = PDLP_TERMINATION_REASON_OPTIMAL;
   else if (st.cumulative_time_sec >= tc.time_sec_limit)
     reason = PDLP_TERMINATION_REASON_TIME_LIMIT;
-  else if (h.ref_round >= prm_.refine_max_outer || h.iters_total >= (int64_t)tc.iteration_limit)
+  else if (h.iters_total >= (int64_t)tc.iteration_limit)
     reason = PDLP_TERMINATION_REASON_ITERATION_LIMIT;
+  else if (h.capped_rounds >= 2 || h.ref_round >= prm_.refine_max_outer) {
+    // refinement stalled (two correction solves stopped by their cap, or the
+    // round limit): the caller may solve the LP again in FP64
+    reason = PDLP_TERMINATION_REASON_ITERATION_LIMIT;
+    h.stalled = true;
+  }
   if (reason) {
     h.done = true;
     h.reason = reason;

@@ -66,6 +66,8 @@ struct HostInst {
   QpCache qc_cur{};                  // norms of the LP the current solve terminates on
   double tol_cur = -1.0;             // tolerance of the current (inner) solve, -1 = user's
   int32_t ref_round = 0;             // correction LPs solved so far
+  int32_t capped_rounds = 0;         // correction solves stopped by their iteration cap
+  bool stalled = false;              // refinement stalled: finished for an FP64 solve
   bool refine_now = false;           // inner solve finished in this evaluation
   bool refined = false;              // (xk, yk) holds a refined point
   double dp = 1.0, dd = 1.0;         // scale factors of the current correction LP
@@ -76,6 +78,8 @@ struct HostInst {
   double last_err = INFINITY;        // KKT error of (xk, yk)
   int64_t iters_total = 0, attempts_total = 0;
   double tolerance_override = -1.0;  // serving: per-request tolerance
+  int64_t iteration_limit_override = -1;  // >= 0: tighter iteration limit for this instance
+  double time_limit_override = -1.0;      // >= 0: tighter time limit (s) for this instance
   int64_t tag = -1;                  // serving: request id
 };
 

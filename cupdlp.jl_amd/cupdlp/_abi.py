@@ -115,7 +115,7 @@ class pdlp_result(C.Structure):
         ("final_stats", pdlp_iteration_stats),
         ("solve_time_sec", C.c_double), ("preprocess_time_sec", C.c_double),
         ("total_attempts", C.c_int64),
-        ("refinement_rounds", C.c_int32), ("pad_", C.c_int32),
+        ("refinement_rounds", C.c_int32), ("fp64_fallback", C.c_int32),
     ]
 
 

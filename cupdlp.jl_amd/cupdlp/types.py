@@ -276,5 +276,6 @@ class SaddlePointOutput:
     solve_time_sec: float = 0.0
     total_attempts: int = 0
     refinement_rounds: int = 0
+    fp64_fallback: bool = False
     primal_current: Optional[np.ndarray] = None
     dual_current: Optional[np.ndarray] = None

@@ -204,7 +204,7 @@ mutable struct CResult
     preprocess_time_sec::Float64
     total_attempts::Int64
     refinement_rounds::Int32
-    pad::Int32
+    fp64_fallback::Int32
     CResult() = new()
 end
 

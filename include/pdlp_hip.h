@@ -211,7 +211,10 @@ typedef struct pdlp_result {
   double preprocess_time_sec;    /* validation + rescaling + upload */
   int64_t total_attempts;        /* total_number_iterations of the reference */
   int32_t refinement_rounds;     /* FP32_REFINE: outer rounds performed */
-  int32_t pad_;
+  int32_t fp64_fallback;         /* FP32_REFINE: 1 if refinement stalled and the
+                                    LP was solved again in FP64 (pdlp_solve,
+                                    pdlp_batch_solve; x, y, the statistics and
+                                    the reason are then the FP64 solve's) */
 } pdlp_result;
 
 /* Receives each recorded IterationStats (replaces Vector{IterationStats}). */

@@ -73,15 +73,20 @@ def test_refined_transportation(gpu_ctx):
 
 
 def test_inner_iteration_cap_and_best_point(gpu_ctx):
-    """Correction solves capped at 64 iterations: the solve still ends, and the
-    returned point is the one its final statistics describe."""
-    say("case: n=2000 LP, FP32 + refinement, inner cap 64, 6 rounds")
+    """Correction solves capped at 64 iterations and a total iteration limit:
+    the solve ends at the limit (no FP64 fallback: the limit is the
+    caller's), and the returned point is the one its final statistics
+    describe."""
+    say("case: n=2000 LP, FP32 + refinement, inner cap 64, iteration limit 400")
     prob, cert = config_lp(0)
-    out = gpu_ctx.solve(prob, params(1e-8, refine_inner_iteration_limit=64, refine_max_outer=6))
+    p = params(1e-8, refine_inner_iteration_limit=64, refine_max_outer=6)
+    p.termination_criteria.iteration_limit = 400
+    out = gpu_ctx.solve(prob, p)
     say(f"  {out.termination_string}, iterations {out.iteration_count}, "
         f"rounds {out.refinement_rounds}")
     assert out.termination_reason in (TR.TERMINATION_REASON_OPTIMAL,
                                       TR.TERMINATION_REASON_ITERATION_LIMIT)
+    assert not out.fp64_fallback
     assert out.refinement_rounds <= 6
     ci, _ = O.point_stats(prob, out.primal_solution, out.dual_solution, 1e-8, 1e-8)
     assert out.iteration_stats
@@ -89,6 +94,29 @@ def test_inner_iteration_cap_and_best_point(gpu_ctx):
     for k in ("l2_primal_residual", "l2_dual_residual", "primal_objective", "dual_objective"):
         a, b = getattr(fs, k), getattr(ci, k)
         assert abs(a - b) <= 1e-9 * (1 + abs(b)), (k, a, b)
+
+
+def test_stalled_refinement_falls_back_to_fp64(gpu_ctx):
+    """Correction solves capped at 64 iterations stall the refinement; the LP
+    is then solved again in FP64, whose result is returned (bitwise the plain
+    FP64 solve's), alone and in a batch."""
+    say("case: n=2000 LP and a transportation LP, FP32 + refinement with inner cap 64 -> FP64 fallback")
+    probs = [config_lp(0)[0], transportation_lp(40, 50, seed=6)]
+    p = params(1e-8, refine_inner_iteration_limit=64)
+    p64 = params(1e-8)
+    p64.precision = Precision.FP64
+    plain = [gpu_ctx.solve(q, p64) for q in probs]
+    alone = [gpu_ctx.solve(q, p) for q in probs]
+    mixed = gpu_ctx.batch_solve(probs, p)
+    for prob, f, a, b in zip(probs, plain, alone, mixed):
+        for o in (a, b):
+            assert o.fp64_fallback
+            assert o.termination_reason == TR.TERMINATION_REASON_OPTIMAL
+            assert o.refinement_rounds >= 2
+            assert o.iteration_count > f.iteration_count
+            assert np.array_equal(o.primal_solution, f.primal_solution)
+            assert np.array_equal(o.dual_solution, f.dual_solution)
+        check_kkt(prob, a, 1e-8)
 
 
 def test_refined_small_cases(gpu_ctx):
