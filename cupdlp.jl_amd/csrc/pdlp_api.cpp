@@ -19,6 +19,7 @@ using namespace pdlp;
 struct pdlp_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
+  hipStream_t stream2 = nullptr;  // FP64 solves of stalled refinements (created on first use)
   DevicePool pool;
   Profiler prof;
   std::string err;
@@ -126,6 +127,10 @@ void pdlp_ctx_destroy(pdlp_ctx *ctx) {
   for (hipEvent_t ev : ctx->prof.events) (void)hipEventDestroy(ev);
   ctx->pool.release_all();
   (void)hipStreamDestroy(ctx->stream);
+  if (ctx->stream2) {
+    (void)hipStreamSynchronize(ctx->stream2);
+    (void)hipStreamDestroy(ctx->stream2);
+  }
   delete ctx;
 }
 
@@ -207,60 +212,105 @@ int pdlp::prepare_instance(const pdlp_lp *lp, const pdlp_params *prm, HostInst *
   return 0;
 }
 
-// FP32 + refinement: LPs whose refinement stalled are solved again, together,
-// in plain FP64, within what is left of the caller's iteration and time
-// limits.  Their results become the FP64 solves', with the FP32 iteration
-// and attempt counts added and fp64_fallback = 1.
-struct FallbackItem {
-  const pdlp_lp *lp;
-  pdlp_result *res;
-  pdlp_stats_cb cb;
-  void *user;
-};
+// Solves `count` prepared-on-demand LPs together and fills results[i].
+// FP32 + refinement: an instance whose refinement stalls is moved, as soon as
+// it stalls, to a plain FP64 engine on the context's second stream, within
+// what is left of the caller's iteration and time limits; the two engines'
+// rounds are enqueued back to back so their kernels overlap on the GPU.  A
+// moved instance's result is the FP64 solve's, with the FP32 iteration and
+// attempt counts added and fp64_fallback = 1.
+static int solve_instances(pdlp_ctx *ctx, int32_t count, const pdlp_lp *lps,
+                           const pdlp_params *params, pdlp_result *results, pdlp_stats_cb cb,
+                           void *user, double t0) {
+  std::string *err = &ctx->err;
+  Engine main(ctx->stream, &ctx->pool, *params, &ctx->prof);
+  for (int32_t i = 0; i < count; ++i) {
+    auto h = std::make_unique<HostInst>();
+    int rc = prepare_instance(&lps[i], params, h.get(), err);
+    if (rc) {
+      if (count > 1) *err = "instance " + std::to_string(i) + ": " + *err;
+      return rc;
+    }
+    h->res = &results[i];
+    h->t_submit = t0;
+    h->cb = cb;
+    h->user = user;
+    results[i].refinement_rounds = 0;
+    results[i].fp64_fallback = 0;
+    rc = main.add(std::move(h), err);
+    if (rc) return rc;
+  }
+  const double t1 = now_sec();
+  for (int32_t i = 0; i < count; ++i) results[i].preprocess_time_sec = t1 - t0;
+  if (params->precision != PDLP_PRECISION_FP32_REFINE) return main.run(err);
 
-static int fp64_fallback(pdlp_ctx *ctx, const pdlp_params *params,
-                         const std::vector<FallbackItem> &todo, double t0) {
-  if (todo.empty()) return 0;
-  pdlp_params p = *params;
-  p.precision = PDLP_PRECISION_FP64;
+  pdlp_params p64 = *params;
+  p64.precision = PDLP_PRECISION_FP64;
+  std::unique_ptr<Engine> fb;  // kernel profiling covers the FP32 engine only
   struct Prior {
     int64_t iters, attempts;
     int32_t rounds;
   };
-  std::vector<Prior> prior;
-  std::vector<pdlp_result *> ran;
-  Engine eng(ctx->stream, &ctx->pool, p, &ctx->prof);
-  const double elapsed = now_sec() - t0;
-  for (const FallbackItem &f : todo) {
-    // the FP64 solve gets what is left of the caller's iteration and time limits
-    const int64_t left_it = (int64_t)p.termination_criteria.iteration_limit - f.res->iteration_count;
-    const double left_t = p.termination_criteria.time_sec_limit - elapsed;
-    if (left_it <= 0 || !(left_t > 0)) continue;
-    prior.push_back({f.res->iteration_count, f.res->total_attempts, f.res->refinement_rounds});
-    ran.push_back(f.res);
-    auto h = std::make_unique<HostInst>();
-    int rc = prepare_instance(f.lp, &p, h.get(), &ctx->err);
-    if (rc) return rc;
-    h->res = f.res;
-    h->t_submit = t0;
-    h->cb = f.cb;
-    h->user = f.user;
-    h->iteration_limit_override = left_it;
-    h->time_limit_override = left_t;
-    rc = eng.add(std::move(h), &ctx->err);
-    if (rc) return rc;
+  std::vector<Prior> prior(count);
+  std::vector<char> moved(count, 0);
+  bool main_done = false;
+  for (;;) {
+    const int r1 = main_done ? 1 : main.begin_round(err);
+    if (r1 < 0) return r1;
+    // the FP64 engine runs its (short) rounds while the FP32 chunk is in flight
+    if (fb && !fb->all_done()) {
+      do {
+        int rc = fb->step_round(err);
+        if (rc < 0) return rc;
+      } while (r1 == 0 && !main.round_ready() && !fb->all_done());
+    }
+    if (!main_done) {
+      int rc = main.end_round(err);
+      if (rc < 0) return rc;
+    }
+    if (r1 == 1) main_done = true;
+    for (int32_t q = 0; q < count; ++q) {
+      const HostInst &h = main.inst((size_t)q);
+      if (!h.done || !h.stalled || moved[q]) continue;
+      moved[q] = 1;
+      pdlp_result *r = &results[q];
+      const int64_t left_it = (int64_t)p64.termination_criteria.iteration_limit - r->iteration_count;
+      const double left_t = p64.termination_criteria.time_sec_limit - (now_sec() - t0);
+      if (left_it <= 0 || !(left_t > 0)) continue;
+      prior[q] = {r->iteration_count, r->total_attempts, r->refinement_rounds};
+      if (!fb) {
+        if (!ctx->stream2) {
+          hipError_t e = hipStreamCreateWithFlags(&ctx->stream2, hipStreamNonBlocking);
+          if (e != hipSuccess) {
+            *err = std::string("hipStreamCreate: ") + hipGetErrorString(e);
+            return -(int)e;
+          }
+        }
+        fb = std::make_unique<Engine>(ctx->stream2, &ctx->pool, p64);
+      }
+      auto g = std::make_unique<HostInst>();
+      int rc = prepare_instance(&lps[q], &p64, g.get(), err);
+      if (rc) return rc;
+      g->res = r;
+      g->t_submit = t0;
+      g->cb = cb;
+      g->user = user;
+      g->iteration_limit_override = left_it;
+      g->time_limit_override = left_t;
+      rc = fb->add(std::move(g), err);
+      if (rc) return rc;
+      r->fp64_fallback = 2;  // being solved in FP64
+    }
+    if (main_done && (!fb || fb->all_done())) break;
   }
-  if (ran.empty()) return 0;
-  int rc = eng.run(&ctx->err);
-  if (rc) return rc;
-  for (size_t k = 0; k < ran.size(); ++k) {
-    pdlp_result *r = ran[k];
-    const int64_t it = (int64_t)r->iteration_count + prior[k].iters;
+  for (int32_t q = 0; q < count; ++q) {
+    pdlp_result *r = &results[q];
+    if (r->fp64_fallback != 2) continue;
+    const int64_t it = (int64_t)r->iteration_count + prior[q].iters;
     r->iteration_count = (int32_t)std::min<int64_t>(it, INT32_MAX);
-    r->total_attempts += prior[k].attempts;
-    r->refinement_rounds = prior[k].rounds;
+    r->total_attempts += prior[q].attempts;
+    r->refinement_rounds = prior[q].rounds;
     r->fp64_fallback = 1;
-    r->solve_time_sec = now_sec() - t0;
   }
   return 0;
 }
@@ -281,29 +331,8 @@ extern "C" int pdlp_solve(pdlp_ctx *ctx, const pdlp_lp *lp, const pdlp_params *p
   int rc = check_params(params, &ctx->err);
   if (rc) return rc;
   (void)hipSetDevice(ctx->device);
-  auto h = std::make_unique<HostInst>();
-  rc = prepare_instance(lp, params, h.get(), &ctx->err);
+  rc = solve_instances(ctx, 1, lp, params, result, cb, user, t0);
   if (rc) return rc;
-  h->res = result;
-  h->t_submit = t0;
-  h->cb = cb;
-  h->user = user;
-  result->refinement_rounds = 0;
-  result->fp64_fallback = 0;
-  bool stalled = false;
-  {
-    Engine eng(ctx->stream, &ctx->pool, *params, &ctx->prof);
-    rc = eng.add(std::move(h), &ctx->err);
-    if (rc) return rc;
-    result->preprocess_time_sec = now_sec() - t0;
-    rc = eng.run(&ctx->err);
-    if (rc) return rc;
-    stalled = eng.inst(0).stalled;
-  }
-  if (stalled) {
-    rc = fp64_fallback(ctx, params, {{lp, result, cb, user}}, t0);
-    if (rc) return rc;
-  }
   result->solve_time_sec = now_sec() - t0;
   return 0;
 }
@@ -320,36 +349,13 @@ extern "C" int pdlp_batch_solve(pdlp_ctx *ctx, int32_t count, const pdlp_lp *lps
   int rc = check_params(params, &ctx->err);
   if (rc) return rc;
   (void)hipSetDevice(ctx->device);
-  std::vector<FallbackItem> todo;
-  {
-  Engine eng(ctx->stream, &ctx->pool, *params, &ctx->prof);
   for (int32_t i = 0; i < count; ++i) {
     if ((lps[i].n > 0 && !results[i].x) || (lps[i].m > 0 && !results[i].y)) {
       ctx->err = "result vectors must be allocated by the caller (instance " + std::to_string(i) + ")";
       return 1;
     }
-    auto h = std::make_unique<HostInst>();
-    rc = prepare_instance(&lps[i], params, h.get(), &ctx->err);
-    if (rc) {
-      ctx->err = "instance " + std::to_string(i) + ": " + ctx->err;
-      return rc;
-    }
-    h->res = &results[i];
-    h->t_submit = t0;
-    results[i].refinement_rounds = 0;
-    results[i].fp64_fallback = 0;
-    rc = eng.add(std::move(h), &ctx->err);
-    if (rc) return rc;
   }
-  const double t1 = now_sec();
-  rc = eng.run(&ctx->err);
-  if (rc) return rc;
-  for (int32_t i = 0; i < count; ++i) {
-    results[i].preprocess_time_sec = t1 - t0;
-    if (eng.inst((size_t)i).stalled) todo.push_back({&lps[i], &results[i], nullptr, nullptr});
-  }
-  }  // the FP32 engine's device memory is returned before the FP64 solves
-  return fp64_fallback(ctx, params, todo, t0);
+  return solve_instances(ctx, count, lps, params, results, nullptr, nullptr, t0);
 }
 
 // ------------------------------------------------------------------ serving

@@ -635,12 +635,19 @@ int Engine::ensure_step_table(int64_t need, std::string *err) {
   return 0;
 }
 
-int Engine::run_attempts(int64_t count, std::string *err) {
+// Enqueues `count` step attempts of every running instance and the copy of
+// the control blocks back to the host; complete_attempts() waits for them.
+int Engine::launch_attempts(int64_t count, std::string *err) {
   int64_t max_att = 0;
   for (size_t q = 0; q < inst_.size(); ++q) max_att = std::max(max_att, ctl_[q].attempts);
   int rc = ensure_step_table(max_att + count + 2, err);
   if (rc) return rc;
   const bool prof = prof_ && prof_->enabled;
+  std::vector<double> &pbytes = pend_pbytes_, &dbytes = pend_dbytes_;
+  std::vector<int64_t> &remaining = pend_remaining_;
+  pbytes.clear();
+  dbytes.clear();
+  remaining.clear();
   // Algorithmic bytes per launch of each instance: the matrix with its
   // indices, the row pointers, the gathered vector read once, and nine fused
   // vector streams per row (primal: x c l u in, x' out, the two weighted sums
@@ -648,8 +655,6 @@ int Engine::run_attempts(int64_t count, std::string *err) {
   //   N (s_A + 4) + 4 (rows + 1) + s_v (gathered) + 9 s_v rows,
   // with s_A = s_v = 8 (FP64) or 4 (FP32).  Traffic the engine adds beyond
   // this (A^T y and dx stored, FP64 sums on the FP32 path) is not counted.
-  std::vector<double> pbytes, dbytes;
-  std::vector<int64_t> remaining;
   if (prof) {
     const size_t need = 4 * (size_t)count;
     while (prof_->events.size() < need) {
@@ -686,9 +691,23 @@ int Engine::run_attempts(int64_t count, std::string *err) {
   }
   const size_t ninst = inst_.size();
   HIPCHK(hipMemcpyAsync(ctl_pinned_, d_ctl_, sizeof(Ctl) * ninst, hipMemcpyDeviceToHost, s_));
+  pend_count_ = count;
+  pend_t0_ = t0;
+  pend_prof_ = prof;
+  return 0;
+}
+
+int Engine::complete_attempts(std::string *err) {
+  const int64_t count = pend_count_;
+  if (count <= 0) return 0;
+  pend_count_ = 0;
+  const bool prof = pend_prof_;
+  const std::vector<double> &pbytes = pend_pbytes_, &dbytes = pend_dbytes_;
+  const std::vector<int64_t> &remaining = pend_remaining_;
+  const size_t ninst = inst_.size();
   HIPCHK(hipStreamSynchronize(s_));
   memcpy(ctl_.data(), ctl_pinned_, sizeof(Ctl) * ninst);
-  const double dt = now_sec() - t0;
+  const double dt = now_sec() - pend_t0_;
   if (prof) {
     for (int64_t a = 0; a < count; ++a) {
       float ms_p = 0.f, ms_d = 0.f;
@@ -1260,7 +1279,7 @@ void Engine::reset_inner(size_t q, double omega0) {
   h.tol_cur = prm_.refine_inner_tolerance;
 }
 
-int Engine::step_round(std::string *err) {
+int Engine::begin_round(std::string *err) {
   // Rebuild the block tables once finished instances hold a quarter of them.
   int64_t dead = 0;
   for (auto &h : inst_)
@@ -1287,8 +1306,29 @@ int Engine::step_round(std::string *err) {
   if (all_done) return 1;
   if (R <= 0) return 0;
   R = std::min<int64_t>(R, std::max<int32_t>(prm_.termination_evaluation_frequency, 10));
-  int rc = run_attempts(R, err);
+  int rc = launch_attempts(R, err);
   return rc < 0 ? rc : 0;
+}
+
+int Engine::end_round(std::string *err) {
+  int rc = complete_attempts(err);
+  return rc < 0 ? rc : 0;
+}
+
+int Engine::step_round(std::string *err) {
+  int r = begin_round(err);
+  if (r != 0) return r;
+  return end_round(err);
+}
+
+bool Engine::round_ready() const {
+  return pend_count_ <= 0 || hipStreamQuery(s_) == hipSuccess;
+}
+
+bool Engine::all_done() const {
+  for (auto &h : inst_)
+    if (!h->done) return false;
+  return true;
 }
 
 int Engine::run(std::string *err) {

@@ -102,6 +102,14 @@ class Engine {
   // Runs one round: pending evaluations, then one chunk of attempts.  Returns
   // 0 while work remains, 1 when all instances are done, < 0 on error.
   int step_round(std::string *err);
+  // The same in two halves, so that engines on different streams overlap:
+  // begin_round evaluates and enqueues the chunk (same return values),
+  // end_round waits for it.
+  int begin_round(std::string *err);
+  int end_round(std::string *err);
+  bool all_done() const;
+  // True when the chunk enqueued by begin_round has finished on the device.
+  bool round_ready() const;
   size_t count() const { return inst_.size(); }
   HostInst &inst(size_t i) { return *inst_[i]; }
   void remove_done();  // drops finished instances (serving)
@@ -113,7 +121,8 @@ class Engine {
   hipError_t push_ctl();
   int ensure_step_table(int64_t need, std::string *err);
   int evaluate(std::string *err);
-  int run_attempts(int64_t count, std::string *err);
+  int launch_attempts(int64_t count, std::string *err);
+  int complete_attempts(std::string *err);
   int finish_instance(size_t q, std::string *err);
   void host_eval_logic(size_t q, const double *out);
   int refine(std::string *err);
@@ -150,6 +159,12 @@ class Engine {
   double *d_red_ = nullptr, *d_grow_ = nullptr;
   int64_t tab_len_ = 0;
   std::vector<void *> table_allocs_;
+  // the chunk in flight (launch_attempts -> complete_attempts)
+  int64_t pend_count_ = 0;
+  double pend_t0_ = 0.0;
+  bool pend_prof_ = false;
+  std::vector<double> pend_pbytes_, pend_dbytes_;
+  std::vector<int64_t> pend_remaining_;
 };
 
 double now_sec();
