@@ -387,9 +387,92 @@ int Engine::fill_slices(HostInst &h, std::string *err, const SliceDesc *dev_tab)
   return 0;
 }
 
+// power_method_failure_probability — src/primal_dual_hybrid_gradient_gpu.jl:139-148
+static double power_failure_probability(int64_t dim, double eps, int64_t k) {
+  if (k < 2 || eps <= 0.0) return 1.0;
+  const double a = 0.354 / sqrt(eps * (double)(k - 1));
+  return (a < 0.824 ? a : 0.824) * sqrt((double)dim) * pow(1.0 - eps, (double)k - 0.5);
+}
+
+// SplitMix64 (the published generator of Steele, Lea and Flood) and
+// Box-Muller: the power method's start vector, x_j ~ N(0, 1) for the input
+// columns j = 0, 1, ... in order (the reference draws randn from
+// MersenneTwister(1), which is not reproduced).
+static uint64_t splitmix64(uint64_t *s) {
+  uint64_t z = (*s += 0x9E3779B97F4A7C15ULL);
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+  return z ^ (z >> 31);
+}
+static double unit_uniform(uint64_t *s) {
+  return ((double)(splitmix64(s) >> 11) + 0.5) * (1.0 / 9007199254740992.0);
+}
+
+// The constant step policy's step size (1 - 0.2) / sigma_max(A~), with
+// sigma_max from the power method on the device
+// (src/primal_dual_hybrid_gradient_gpu.jl:573-582); *passes gets the number
+// of power iterations (cumulative_kkt_passes).
+int Engine::constant_step(HostInst &h, double *eta, double *passes, std::string *err) {
+  const Prepared &P = h.P;
+  const int64_t n = P.n, m = P.m;
+  const double rel = 0.2, eps = 1.0 - (1.0 - rel) * (1.0 - rel);
+  int64_t iters = 0;
+  while (power_failure_probability(n, eps, iters) > 0.001) ++iters;
+  std::vector<double> x0(n), xd(std::max<int64_t>(n, 1));
+  uint64_t st = 1;
+  for (int64_t j = 0; j < n; ++j) {
+    const double u1 = unit_uniform(&st), u2 = unit_uniform(&st);
+    x0[j] = sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2);
+  }
+  for (int64_t j = 0; j < n; ++j) xd[j] = x0[P.col_perm[j]];  // device numbering
+  const size_t nps = P.ps.sl.size(), nds = P.ds.sl.size();
+  std::vector<void *> tmp;
+  auto get = [&](size_t bytes) -> void * {
+    void *p = pool_->get(std::max<size_t>(bytes, 256));
+    if (p) tmp.push_back(p);
+    return p;
+  };
+  SliceDesc *tab = (SliceDesc *)get(sizeof(SliceDesc) * (nps + nds));
+  double *x = (double *)get(8 * n), *t = (double *)get(8 * m), *z = (double *)get(8 * n);
+  double *part = (double *)get(8 * 512);
+  hipError_t e = hipSuccess;
+  if (!tab || !x || !t || !z || !part) {
+    for (void *p : tmp) pool_->put(p);
+    if (err) *err = "device allocation for the power method failed";
+    return -2;
+  }
+  if (!norm_pinned_) e = hipHostMalloc((void **)&norm_pinned_, 8 * 5 * 256, 0);
+  if (e == hipSuccess && nps)
+    e = hipMemcpyAsync(tab, P.ps.sl.data(), sizeof(SliceDesc) * nps, hipMemcpyHostToDevice, s_);
+  if (e == hipSuccess && nds)
+    e = hipMemcpyAsync(tab + nps, P.ds.sl.data(), sizeof(SliceDesc) * nds, hipMemcpyHostToDevice, s_);
+  if (e == hipSuccess && n) e = hipMemcpyAsync(x, xd.data(), 8 * n, hipMemcpyHostToDevice, s_);
+  double sv = 0.0;
+  if (e == hipSuccess) {
+    PowerArgs a;
+    a.n = n; a.m = m;
+    a.ps = tab; a.nps = (int)nps; a.ds = tab + nps; a.nds = (int)nds;
+    a.t_ptr = h.dev.t_ptr; a.t_col = h.dev.t_col; a.t_val = (const double *)h.dev.t_val;
+    a.a_ptr = h.dev.a_ptr; a.a_col = h.dev.a_col; a.a_val = (const double *)h.dev.a_val;
+    a.x = x; a.t = t; a.z = z; a.part = part; a.part_host = norm_pinned_;
+    a.iterations = iters;
+    e = power_method(s_, a, &sv);
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(s_);
+  for (void *p : tmp) pool_->put(p);
+  if (e != hipSuccess) {
+    if (err) *err = std::string("power method failed: ") + hipGetErrorString(e);
+    return -(int)e;
+  }
+  *eta = (1.0 - rel) / sv;
+  *passes = (double)iters;
+  return 0;
+}
+
 int Engine::add(std::unique_ptr<HostInst> h, std::string *err) {
-  if (prm_.step_size_policy != PDLP_STEP_ADAPTIVE) {
-    if (err) *err = "ConstantStepsizeParams is not supported by the device engine";
+  const bool constant = prm_.step_size_policy == PDLP_STEP_CONSTANT;
+  if (constant && prec_) {
+    if (err) *err = "ConstantStepsizeParams needs PDLP_PRECISION_FP64";
     return 4;
   }
   int rc = upload(*h, err);
@@ -399,7 +482,12 @@ int Engine::add(std::unique_ptr<HostInst> h, std::string *err) {
   const Prepared &P = h->P;
   // 1 / norm(A, Inf) with the elementwise norm (src/primal_dual_hybrid_gradient_gpu.jl:569-571);
   // a matrix with no entries (m = 0) uses 1.0, where the reference fails.
-  const double eta0 = P.max_abs_scaled > 0.0 ? 1.0 / P.max_abs_scaled : 1.0;
+  double eta0 = P.max_abs_scaled > 0.0 ? 1.0 / P.max_abs_scaled : 1.0;
+  double passes0 = 0.5;
+  if (constant) {
+    rc = constant_step(*h, &eta0, &passes0, err);
+    if (rc) return rc;
+  }
   c.eta = eta0;
   c.eta_entry = eta0;
   double omega = prm_.primal_importance;
@@ -411,7 +499,7 @@ int Engine::add(std::unique_ptr<HostInst> h, std::string *err) {
     if (obj > 0.0 && rhs > 0.0) omega = prm_.primal_importance * (obj / rhs);
   }
   c.omega = omega;
-  c.kkt_passes = 0.5;
+  c.kkt_passes = passes0;
   c.aty_valid = 1;
   h->qc_cur = P.qc;
   if (prec_) h->tol_cur = prm_.refine_inner_tolerance;
@@ -683,7 +771,8 @@ int Engine::launch_attempts(int64_t count, std::string *err) {
                       ev ? ev[2] : nullptr, ev ? ev[3] : nullptr);
     if (!e)
       e = launch_step(s_, (int)inst_.size(), d_insts_, d_ctl_, d_ppart_, d_dpart_, d_red_, d_grow_,
-                      tab_len_, prm_.reduction_exponent, prm_.growth_exponent);
+                      tab_len_, prm_.reduction_exponent, prm_.growth_exponent,
+                      prm_.step_size_policy == PDLP_STEP_CONSTANT ? 1 : 0);
     if (e) {
       if (err) *err = std::string("kernel launch failed: ") + hipGetErrorString((hipError_t)e);
       return -e;

@@ -117,6 +117,7 @@ class Engine {
  private:
   int upload(HostInst &h, std::string *err);
   int fill_slices(HostInst &h, std::string *err, const SliceDesc *dev_tab = nullptr);
+  int constant_step(HostInst &h, double *eta, double *passes, std::string *err);
   int rebuild_tables(std::string *err);
   hipError_t push_ctl();
   int ensure_step_table(int64_t need, std::string *err);

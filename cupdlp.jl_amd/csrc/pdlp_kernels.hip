@@ -720,6 +720,21 @@ __global__ __launch_bounds__(kThreads) void k_dual_long(const SliceDesc *__restr
 // work-group of kStepThreads per instance.
 constexpr int kStepThreads = 1024;
 
+// take_step! with ConstantStepsizeParams (src/primal_dual_hybrid_gradient_gpu.jl:418-452):
+// the attempt is always taken with the fixed step size; one KKT pass, no
+// attempt count (the reference counts attempts in the adaptive rule only).
+__device__ __forceinline__ void step_constant(Ctl *C) {
+  C->kkt_passes += 1.0;
+  C->cur ^= 1;
+  C->pending = 1;
+  C->w_pending = C->eta_entry;
+  C->wsum += C->eta_entry;
+  C->count += 1;
+  C->accepted += 1;
+  C->aty_valid = 0;
+  if (C->accepted >= C->target) C->active = 0;
+}
+
 // The step of instance q by a work-group of kStepThreads threads; sc is
 // LDS scratch of 3 x 2 kStepThreads/64 doubles.  Contains a barrier.
 __device__ __forceinline__ void step_instance(int q, const InstDev *__restrict__ insts, Ctl *ctl,
@@ -727,9 +742,13 @@ __device__ __forceinline__ void step_instance(int q, const InstDev *__restrict__
                                               const double *__restrict__ red_tab,
                                               const double *__restrict__ grow_tab,
                                               int64_t tab_len, double red_exp, double grow_exp,
-                                              double (*sc)[2 * kStepThreads / 64]) {
+                                              int constant, double (*sc)[2 * kStepThreads / 64]) {
   Ctl *C = &ctl[q];
   if (!C->active) return;
+  if (constant) {
+    if (threadIdx.x == 0) step_constant(C);
+    return;
+  }
   const InstDev &I = insts[q];
   DD sx = dd_zero(), sy = dd_zero(), sxy = dd_zero();
   for (int b = I.ps0 + (int)threadIdx.x; b < I.ps1; b += kStepThreads)
@@ -771,10 +790,10 @@ __global__ __launch_bounds__(kStepThreads) void k_step(const InstDev *__restrict
                                                        const double *__restrict__ red_tab,
                                                        const double *__restrict__ grow_tab,
                                                        int64_t tab_len, double red_exp,
-                                                       double grow_exp) {
+                                                       double grow_exp, int constant) {
   __shared__ double sc[3][2 * kStepThreads / 64];
   step_instance(blockIdx.x, insts, ctl, ppart, dpart, red_tab, grow_tab, tab_len, red_exp,
-                grow_exp, sc);
+                grow_exp, constant, sc);
 }
 
 // ---------------------------------------------------------------------------
@@ -1486,10 +1505,10 @@ int launch_slice_fill(hipStream_t s, int prec, int nslices, const SliceDesc *sli
 
 int launch_step(hipStream_t s, int ninst, const InstDev *insts, Ctl *ctl, const double *ppart,
                 const double *dpart, const double *red_tab, const double *grow_tab,
-                int64_t tab_len, double red_exp, double grow_exp) {
+                int64_t tab_len, double red_exp, double grow_exp, int constant) {
   if (ninst <= 0) return 0;
   hipLaunchKernelGGL(k_step, dim3(grid_of(ninst)), dim3(kStepThreads), 0, s, insts, ctl, ppart,
-                     dpart, red_tab, grow_tab, tab_len, red_exp, grow_exp);
+                     dpart, red_tab, grow_tab, tab_len, red_exp, grow_exp, constant);
   PDLP_CHECK_LAUNCH();
 }
 

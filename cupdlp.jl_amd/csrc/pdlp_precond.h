@@ -39,6 +39,22 @@ struct PrecondOut {
 };
 
 hipError_t precond_run(hipStream_t s, const PrecondArgs &a, PrecondOut *out);
+
+// estimate_maximum_singular_value (src/primal_dual_hybrid_gradient_gpu.jl:150-172)
+// on the device: `iterations` rounds of x <- A^T (A (x / ||x||)) from the
+// start vector in x, then sqrt(x'A^T A x / ||x||^2), every sum exactly
+// rounded.  x (n), t (m), z (n) device scratch; part 512 doubles; part_host
+// pinned, 512 doubles.
+struct PowerArgs {
+  int64_t n = 0, m = 0;
+  const SliceDesc *ps = nullptr, *ds = nullptr;
+  int nps = 0, nds = 0;
+  const int32_t *t_ptr = nullptr, *t_col = nullptr, *a_ptr = nullptr, *a_col = nullptr;
+  const double *t_val = nullptr, *a_val = nullptr;
+  double *x = nullptr, *t = nullptr, *z = nullptr, *part = nullptr, *part_host = nullptr;
+  int64_t iterations = 0;
+};
+hipError_t power_method(hipStream_t s, const PowerArgs &a, double *sv);
 hipError_t to_float(hipStream_t s, int64_t len, const double *src, float *dst);
 
 }  // namespace pdlp

@@ -263,6 +263,55 @@ __global__ __launch_bounds__(kPT) void k_norms(int64_t nnz, const double *__rest
   }
 }
 
+// Power method of the constant step policy: out = M x over the rows of a
+// slice table (whole rows, exactly rounded sums; a chunked row by the wave of
+// its first chunk).
+__global__ __launch_bounds__(kPT) void k_row_spmv(const SliceDesc *__restrict__ sl, int nsl,
+                                                  const int32_t *__restrict__ ptr,
+                                                  const int32_t *__restrict__ col,
+                                                  const double *__restrict__ val,
+                                                  const double *__restrict__ x,
+                                                  double *__restrict__ out) {
+  const int wv = (int)blockIdx.x * (kPT / 64) + ((int)threadIdx.x >> 6);
+  if (wv >= nsl) return;
+  const int lane = threadIdx.x & 63;
+  const SliceDesc sd = sl[wv];
+  const SliceRow r = slice_row(sd, lane, ptr, true);
+  PDD acc{0.0, 0.0};
+  if (r.valid) {
+    const int b = ptr[r.row];
+    for (int k = r.k; k < r.end; k += r.step) pdd_add_prod(acc, val[b + k], x[col[b + k]]);
+  }
+  for (int o = (1 << slice_group_lg(sd)) >> 1; o > 0; o >>= 1)
+    pdd_merge(acc, PDD{__shfl_xor(acc.s, o, 64), __shfl_xor(acc.e, o, 64)});
+  if (r.valid && r.head) out[r.row] = acc.s + acc.e;
+}
+
+// Block partials (s, e) of the compensated sum of a[i] * b[i].
+__global__ __launch_bounds__(kPT) void k_dot(int64_t len, const double *__restrict__ a,
+                                             const double *__restrict__ b,
+                                             double *__restrict__ part) {
+  __shared__ double sm[2][kPT];
+  PDD acc{0.0, 0.0};
+  const int64_t stride = (int64_t)gridDim.x * kPT;
+  for (int64_t i = (int64_t)blockIdx.x * kPT + threadIdx.x; i < len; i += stride)
+    pdd_add_prod(acc, a[i], b[i]);
+  sm[0][threadIdx.x] = acc.s;
+  sm[1][threadIdx.x] = acc.e;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    PDD t{sm[0][0], sm[1][0]};
+    for (int k = 1; k < kPT; ++k) pdd_merge(t, PDD{sm[0][k], sm[1][k]});
+    part[2 * blockIdx.x] = t.s;
+    part[2 * blockIdx.x + 1] = t.e;
+  }
+}
+
+__global__ __launch_bounds__(kPT) void k_div(int64_t len, double *__restrict__ x, double d) {
+  const int64_t i = (int64_t)blockIdx.x * kPT + threadIdx.x;
+  if (i < len) x[i] = x[i] / d;
+}
+
 __global__ __launch_bounds__(kPT) void k_to_float(int64_t len, const double *__restrict__ src,
                                                   float *__restrict__ dst) {
   const int64_t i = (int64_t)blockIdx.x * kPT + threadIdx.x;
@@ -339,6 +388,58 @@ hipError_t precond_run(hipStream_t s, const PrecondArgs &a, PrecondOut *out) {
   out->max_abs = mx;
   out->norm_b = sqrt(sb.s + sb.e);
   out->norm_c = sqrt(scc.s + scc.e);
+  return hipSuccess;
+}
+
+static hipError_t dot_host(hipStream_t s, int64_t len, const double *a, const double *b,
+                           double *part, double *part_host, double *out) {
+  constexpr int grid = 256;
+  hipLaunchKernelGGL(k_dot, dim3(grid), dim3(kPT), 0, s, len, a, b, part);
+  PCHK(hipGetLastError());
+  PCHK(hipMemcpyAsync(part_host, part, sizeof(double) * 2 * grid, hipMemcpyDeviceToHost, s));
+  PCHK(hipStreamSynchronize(s));
+  PDD t{0.0, 0.0};
+  for (int g = 0; g < grid; ++g) {  // host-side merge, same arithmetic as pdd_merge
+    const double s1 = t.s + part_host[2 * g], bb = s1 - t.s;
+    const double e1 = (t.s - (s1 - bb)) + (part_host[2 * g] - bb);
+    t.s = s1;
+    t.e = (t.e + part_host[2 * g + 1]) + e1;
+  }
+  *out = t.s + t.e;
+  return hipSuccess;
+}
+
+hipError_t power_method(hipStream_t s, const PowerArgs &a, double *sv) {
+  const int64_t n = a.n, m = a.m;
+  auto spmv_A = [&](const double *x, double *t) -> hipError_t {
+    if (a.nds > 0)
+      hipLaunchKernelGGL(k_row_spmv, dim3(nbs(a.nds)), dim3(kPT), 0, s, a.ds, a.nds, a.a_ptr,
+                         a.a_col, a.a_val, x, t);
+    return hipGetLastError();
+  };
+  auto spmv_AT = [&](const double *t, double *z) -> hipError_t {
+    if (a.nps > 0)
+      hipLaunchKernelGGL(k_row_spmv, dim3(nbs(a.nps)), dim3(kPT), 0, s, a.ps, a.nps, a.t_ptr,
+                         a.t_col, a.t_val, t, z);
+    return hipGetLastError();
+  };
+  for (int64_t it = 0; it < a.iterations; ++it) {
+    double nn = 0.0;
+    PCHK(dot_host(s, n, a.x, a.x, a.part, a.part_host, &nn));
+    const double nx = sqrt(nn);
+    if (n > 0) hipLaunchKernelGGL(k_div, dim3(nb(n)), dim3(kPT), 0, s, n, a.x, nx);
+    PCHK(hipGetLastError());
+    PCHK(spmv_A(a.x, a.t));
+    PCHK(spmv_AT(a.t, a.x));
+  }
+  PCHK(spmv_A(a.x, a.t));
+  PCHK(spmv_AT(a.t, a.z));
+  double nn = 0.0, xz = 0.0;
+  PCHK(dot_host(s, n, a.x, a.x, a.part, a.part_host, &nn));
+  PCHK(dot_host(s, n, a.x, a.z, a.part, a.part_host, &xz));
+  const double nx = sqrt(nn);
+  *sv = sqrt(xz / (nx * nx));
+  (void)m;
   return hipSuccess;
 }
 

@@ -232,6 +232,28 @@ def test_batch_alone_vs_mixed_bitwise(gpu_ctx):
         assert np.array_equal(a.dual_solution, b.dual_solution)
 
 
+def test_constant_step_policy(gpu_ctx):
+    """ConstantStepsizeParams: step (1 - 0.2) / sigma_max from the power method
+    (start vector SplitMix64 + Box-Muller on both sides), every attempt taken;
+    bitwise equal to the oracle, with device and with host rescaling."""
+    from cupdlp import ConstantStepsizeParams
+    say("case: n=2000 LP and a transportation LP, constant step policy, 600 iterations")
+    for prob in (config_lp(0)[0], transportation_lp(30, 40, seed=4)):
+        for dev in (True, False):
+            p = params(1e-12, iteration_limit=600, step_size_policy_params=ConstantStepsizeParams(),
+                       device_preconditioning=dev)
+            g = gpu_ctx.solve(prob, p)
+            o = O.solve(prob, p)
+            compare_runs(g, o)
+            assert g.total_attempts == 0
+    prob, opt = lp_cases.tiny_two_var()
+    p = params(1e-8, step_size_policy_params=ConstantStepsizeParams())
+    g = gpu_ctx.solve(prob, p)
+    o = O.solve(prob, p)
+    compare_runs(g, o)
+    assert g.termination_reason == TR.TERMINATION_REASON_OPTIMAL
+
+
 def test_invalid_input_raises(gpu_ctx):
     say("case: NaN objective")
     from cupdlp.solver import PdlpError
