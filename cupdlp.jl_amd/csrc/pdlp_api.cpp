@@ -359,17 +359,49 @@ extern "C" int pdlp_batch_solve(pdlp_ctx *ctx, int32_t count, const pdlp_lp *lps
 }
 
 // ------------------------------------------------------------------ serving
+// A host copy of a request's LP (FP32 + refinement servers keep one, for the
+// FP64 solve of a request whose refinement stalls).
+struct LpCopy {
+  pdlp_lp lp{};
+  std::vector<int64_t> colptr, rowval;
+  std::vector<double> nzval, c, lb, ub, b;
+  explicit LpCopy(const pdlp_lp &in) {
+    lp = in;
+    const int64_t n = in.n, m = in.m, nnz = in.nnz;
+    colptr.assign(in.colptr, in.colptr + n + 1);
+    rowval.assign(in.rowval, in.rowval + nnz);
+    nzval.assign(in.nzval, in.nzval + nnz);
+    c.assign(in.c, in.c + n);
+    lb.assign(in.lb, in.lb + n);
+    ub.assign(in.ub, in.ub + n);
+    b.assign(in.b, in.b + m);
+    lp.colptr = colptr.data();
+    lp.rowval = rowval.data();
+    lp.nzval = nzval.data();
+    lp.c = c.data();
+    lp.lb = lb.data();
+    lp.ub = ub.data();
+    lp.b = b.data();
+  }
+};
+
 struct ServeReq {
   int64_t id;
   std::unique_ptr<HostInst> inst;
   double tol;
   size_t bytes;
+  std::unique_ptr<LpCopy> lp;
 };
 
 struct ServeDone {
   std::vector<double> x, y;
   int64_t n = 0, m = 0;
   pdlp_result res;
+  std::unique_ptr<LpCopy> lp;  // FP32 + refinement servers
+  double tol = 0.0, t_submit = 0.0;
+  int64_t prior_iters = 0, prior_attempts = 0;
+  int32_t prior_rounds = 0;
+  size_t fb_bytes = 0;         // footprint of the FP64 solve, while it runs
 };
 
 struct pdlp_server {
@@ -378,6 +410,7 @@ struct pdlp_server {
   double budget = 0;
   int32_t max_slots = 0;
   std::unique_ptr<Engine> eng;
+  std::unique_ptr<Engine> fb;  // FP64 solves of stalled FP32 requests (context's second stream)
   std::deque<ServeReq> queue;
   std::map<int64_t, std::unique_ptr<ServeDone>> running;  // result storage of admitted ids
   std::map<int64_t, std::unique_ptr<ServeDone>> done;
@@ -430,6 +463,7 @@ extern "C" int pdlp_serve_submit(pdlp_server *s, const pdlp_lp *lp, double toler
   }
   h->tag = r.id;
   r.inst = std::move(h);
+  if (s->prm.precision == PDLP_PRECISION_FP32_REFINE) r.lp = std::make_unique<LpCopy>(*lp);
   *id = r.id;
   s->queue.push_back(std::move(r));
   return 0;
@@ -452,12 +486,16 @@ extern "C" int pdlp_serve_poll(pdlp_server *s, int64_t *ids, int32_t max_ids, in
     memset(&d->res, 0, sizeof d->res);
     d->res.x = d->x.data();
     d->res.y = d->y.data();
+    d->lp = std::move(r.lp);
+    d->tol = r.tol;
+    d->t_submit = r.inst->t_submit;
     r.inst->res = &d->res;
     s->in_use += (double)r.bytes;
     s->running[r.id] = std::move(d);
     int rc = s->eng->add(std::move(r.inst), err);
     if (rc) return rc;
   }
+  const bool refine = s->prm.precision == PDLP_PRECISION_FP32_REFINE;
   if (s->eng->count() > 0) {
     int rc = s->eng->step_round(err);
     if (rc < 0) return rc;
@@ -465,14 +503,73 @@ extern "C" int pdlp_serve_poll(pdlp_server *s, int64_t *ids, int32_t max_ids, in
       HostInst &h = s->eng->inst(q);
       if (!h.done) continue;
       auto it = s->running.find(h.tag);
-      if (it != s->running.end()) {
-        s->in_use -= (double)footprint(h.P, s->prm.precision == PDLP_PRECISION_FP32_REFINE);
-        s->done[h.tag] = std::move(it->second);
-        s->running.erase(it);
-        s->finished.push_back(h.tag);
+      if (it == s->running.end()) continue;
+      ServeDone &d = *it->second;
+      s->in_use -= (double)footprint(h.P, refine);
+      if (refine && h.stalled && d.lp) {
+        // refinement stalled: the request continues as a plain FP64 solve
+        // within what is left of its iteration and time limits
+        pdlp_params p64 = s->prm;
+        p64.precision = PDLP_PRECISION_FP64;
+        const int64_t left_it = (int64_t)p64.termination_criteria.iteration_limit - d.res.iteration_count;
+        const double left_t = p64.termination_criteria.time_sec_limit - (now_sec() - d.t_submit);
+        if (left_it > 0 && left_t > 0) {
+          auto g = std::make_unique<HostInst>();
+          rc = pdlp::prepare_instance(&d.lp->lp, &p64, g.get(), err);
+          if (rc) return rc;
+          g->tolerance_override = d.tol;
+          g->t_submit = d.t_submit;
+          g->res = &d.res;
+          g->tag = h.tag;
+          g->iteration_limit_override = left_it;
+          g->time_limit_override = left_t;
+          d.prior_iters = d.res.iteration_count;
+          d.prior_attempts = d.res.total_attempts;
+          d.prior_rounds = d.res.refinement_rounds;
+          d.fb_bytes = footprint(g->P, 0);
+          if (!s->fb) {
+            if (!s->ctx->stream2) {
+              hipError_t e = hipStreamCreateWithFlags(&s->ctx->stream2, hipStreamNonBlocking);
+              if (e != hipSuccess) {
+                *err = std::string("hipStreamCreate: ") + hipGetErrorString(e);
+                return -(int)e;
+              }
+            }
+            s->fb = std::make_unique<Engine>(s->ctx->stream2, &s->ctx->pool, p64);
+          }
+          s->in_use += (double)d.fb_bytes;
+          rc = s->fb->add(std::move(g), err);
+          if (rc) return rc;
+          continue;
+        }
       }
+      s->done[h.tag] = std::move(it->second);
+      s->running.erase(it);
+      s->finished.push_back(h.tag);
     }
     s->eng->remove_done();
+  }
+  if (s->fb && s->fb->count() > 0) {
+    int rc = s->fb->step_round(err);
+    if (rc < 0) return rc;
+    for (size_t q = 0; q < s->fb->count(); ++q) {
+      HostInst &h = s->fb->inst(q);
+      if (!h.done) continue;
+      auto it = s->running.find(h.tag);
+      if (it == s->running.end()) continue;
+      ServeDone &d = *it->second;
+      pdlp_result &r = d.res;
+      const int64_t total = (int64_t)r.iteration_count + d.prior_iters;
+      r.iteration_count = (int32_t)std::min<int64_t>(total, INT32_MAX);
+      r.total_attempts += d.prior_attempts;
+      r.refinement_rounds = d.prior_rounds;
+      r.fp64_fallback = 1;
+      s->in_use -= (double)d.fb_bytes;
+      s->done[h.tag] = std::move(it->second);
+      s->running.erase(it);
+      s->finished.push_back(h.tag);
+    }
+    s->fb->remove_done();
   }
   int32_t k = 0;
   while (k < max_ids && !s->finished.empty()) {
@@ -506,6 +603,7 @@ extern "C" int pdlp_serve_result(pdlp_server *s, int64_t id, pdlp_result *result
 
 extern "C" int pdlp_serve_close(pdlp_server *s) {
   if (!s) return 1;
+  s->fb.reset();
   s->eng.reset();
   delete s;
   return 0;

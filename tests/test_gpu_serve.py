@@ -119,3 +119,31 @@ def test_serve_fp32_refine_alone_vs_mixed(gpu_ctx):
     for k, rid in enumerate(ids):
         assert out[rid].termination_string == "OPTIMAL"
         same_result(out[rid], alone[k])
+
+
+def test_serve_fp32_stalled_falls_back_to_fp64(gpu_ctx):
+    print("case: FP32 + refinement server with inner cap 64: stalled requests finish in FP64, "
+          "bitwise the plain FP64 solve", flush=True)
+    from cupdlp.generators import config_lp
+    probs = [config_lp(0)[0], random_lp(n=900, m=450, seed=70)[0], transportation_lp(30, 40, seed=3)]
+    plain = [gpu_ctx.solve(p, params(1e-8)) for p in probs]
+    sp = params(1e-4, Precision.FP32_REFINE)
+    sp.refine_inner_iteration_limit = 64
+    srv = Server(gpu_ctx, sp)
+    try:
+        out = {}
+        ids = [srv.submit(probs[0], 1e-8)]
+        for _ in range(4):
+            for f in srv.poll():
+                out[f] = srv.result(f)
+        ids += [srv.submit(p, 1e-8) for p in probs[1:]]
+        drain(srv, ids, out)
+    finally:
+        srv.close()
+    for k, rid in enumerate(ids):
+        o = out[rid]
+        assert o.termination_string == "OPTIMAL"
+        assert o.fp64_fallback
+        assert o.iteration_count > plain[k].iteration_count
+        assert np.array_equal(o.primal_solution, plain[k].primal_solution)
+        assert np.array_equal(o.dual_solution, plain[k].dual_solution)
