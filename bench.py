@@ -48,7 +48,7 @@ HBM_PEAK_GBS = 8000.0
 MEASURED_TRAFFIC = {
     ("batch", "fp64", "k_dual"): (109563308.65, 164.82e6),
     ("batch", "fp64", "k_primal"): (135796999.84, 164.45e6),
-    ("batch", "fp32", "k_dual"): (297774904.60, 438.71e6),
+    ("batch", "fp32", "k_dual"): (454888827.88, 670.05e6),
     ("c1", "fp64", "k_dual"): (195460496.0, 285.70e6),
     ("c1", "fp64", "k_primal"): (229460496.0, 248.51e6),
 }
@@ -70,6 +70,12 @@ def batch_problems(count):
     from cupdlp.generators import batch_sizes, random_lp
     sizes = batch_sizes(count)
     return [random_lp(n=int(n), m=int(n) // 2, seed=i)[0] for i, n in enumerate(sizes)]
+
+
+# Optimal objectives of generated LPs without a certificate, from HiGHS
+# (scipy.optimize.linprog, method="highs") in FP64 on the host; see
+# tests/golden/highs_objectives.json.
+REFERENCE_OBJECTIVES = {2: 1219334.0106920893}  # 2,000 x 2,000 transportation LP, seed 2
 
 
 def measured_traffic(workload, precision, kernel, bytes_per_launch):
@@ -269,7 +275,8 @@ def run_single(args, ctx, lib, which):
     from cupdlp.generators import config_lp
     log(f"generating config {which} ...")
     prob, cert = config_lp(which)
-    params = make_params(args.tol, args.precision, iteration_limit=args.iters)
+    params = make_params(args.tol, args.precision, iteration_limit=args.iters,
+                         time_limit=args.time_limit)
     for w in range(args.warmup):
         ctx.solve(prob, params)
     lib.pdlp_set_profiling(ctx.handle, 1)
@@ -286,8 +293,19 @@ def run_single(args, ctx, lib, which):
                             f"c{which}", args.precision)
     n, m, nnz = prob.num_variables, prob.num_constraints, prob.constraint_matrix.nnz
     cfg = {"workload": f"single LP n={n} m={m} nnz={nnz}", "iteration_limit": args.iters,
-           "precision": args.precision}
-    extra = {"iterations_per_s": round(its / dt, 1), "iterations": its}
+           "tolerance": args.tol, "precision": args.precision}
+    ci = out.iteration_stats[-1].convergence_information[0]
+    extra = {"iterations_per_s": round(its / dt, 1), "iterations": its,
+             "termination": out.termination_string, "solve_time_s": round(out.solve_time_sec, 3),
+             "relative_l2_primal_residual": ci.relative_l2_primal_residual,
+             "relative_l2_dual_residual": ci.relative_l2_dual_residual,
+             "relative_optimality_gap": ci.relative_optimality_gap,
+             "primal_objective": ci.primal_objective,
+             "refinement_rounds": out.refinement_rounds, "fp64_fallback": out.fp64_fallback}
+    ref = cert.objective if cert is not None else REFERENCE_OBJECTIVES.get(which)
+    if ref is not None:
+        extra["objective_reference"] = ref
+        extra["objective_rel_error"] = abs(ci.primal_objective - ref) / max(1.0, abs(ref))
     return args.steps / dt, dt, cfg, extra, roof, [prob]
 
 

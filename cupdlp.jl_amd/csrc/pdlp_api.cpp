@@ -212,13 +212,60 @@ int pdlp::prepare_instance(const pdlp_lp *lp, const pdlp_params *prm, HostInst *
   return 0;
 }
 
+// The refined point a stalled FP32 solve ended with, kept while the LP is
+// solved again in FP64.  Should that solve stop at the caller's iteration,
+// time or KKT-pass limit with a larger relative KKT error than the refined
+// point (a tight limit leaves it few iterations), the refined point and its
+// statistics are returned instead; the reason and counts stay the whole
+// solve's.
+struct RefinedPoint {
+  std::vector<double> x, y, xc, yc;
+  pdlp_iteration_stats stats{};
+
+  void save(const pdlp_result &r, int64_t n, int64_t m) {
+    x.assign(r.x, r.x + n);
+    y.assign(r.y, r.y + m);
+    if (r.x_current) xc.assign(r.x_current, r.x_current + n);
+    if (r.y_current) yc.assign(r.y_current, r.y_current + m);
+    stats = r.final_stats;
+  }
+
+  // max of the relative primal residual, dual residual and gap in the
+  // termination norm: each is the absolute figure over (eps_abs / eps_rel +
+  // its norm), so the max orders points as the optimality test does
+  static double rel_kkt(const pdlp_iteration_stats &s, int32_t norm) {
+    const pdlp_convergence_information &c = s.convergence_information;
+    const bool linf = norm == PDLP_L_INF;
+    const double p = linf ? c.relative_l_inf_primal_residual : c.relative_l2_primal_residual;
+    const double d = linf ? c.relative_l_inf_dual_residual : c.relative_l2_dual_residual;
+    const double g = c.relative_optimality_gap;
+    if (isnan(p) || isnan(d) || isnan(g)) return INFINITY;
+    return fmax(fmax(p, d), g);
+  }
+
+  void restore_if_better(pdlp_result *r, int32_t norm) const {
+    const int32_t why = r->termination_reason;
+    if (why != PDLP_TERMINATION_REASON_ITERATION_LIMIT &&
+        why != PDLP_TERMINATION_REASON_TIME_LIMIT &&
+        why != PDLP_TERMINATION_REASON_KKT_MATRIX_PASS_LIMIT)
+      return;
+    if (!(rel_kkt(stats, norm) < rel_kkt(r->final_stats, norm))) return;
+    if (!x.empty()) memcpy(r->x, x.data(), 8 * x.size());
+    if (!y.empty()) memcpy(r->y, y.data(), 8 * y.size());
+    if (r->x_current && !xc.empty()) memcpy(r->x_current, xc.data(), 8 * xc.size());
+    if (r->y_current && !yc.empty()) memcpy(r->y_current, yc.data(), 8 * yc.size());
+    r->final_stats = stats;
+  }
+};
+
 // Solves `count` prepared-on-demand LPs together and fills results[i].
 // FP32 + refinement: an instance whose refinement stalls is moved, as soon as
 // it stalls, to a plain FP64 engine on the context's second stream, within
 // what is left of the caller's iteration and time limits; the two engines'
 // rounds are enqueued back to back so their kernels overlap on the GPU.  A
-// moved instance's result is the FP64 solve's, with the FP32 iteration and
-// attempt counts added and fp64_fallback = 1.
+// moved instance's result is the FP64 solve's (or, if that solve stops at a
+// caller's limit further from optimal, the refined point: RefinedPoint), with
+// the FP32 iteration and attempt counts added and fp64_fallback = 1.
 static int solve_instances(pdlp_ctx *ctx, int32_t count, const pdlp_lp *lps,
                            const pdlp_params *params, pdlp_result *results, pdlp_stats_cb cb,
                            void *user, double t0) {
@@ -252,6 +299,7 @@ static int solve_instances(pdlp_ctx *ctx, int32_t count, const pdlp_lp *lps,
     int32_t rounds;
   };
   std::vector<Prior> prior(count);
+  std::vector<RefinedPoint> refined(count);
   std::vector<char> moved(count, 0);
   bool main_done = false;
   for (;;) {
@@ -278,6 +326,7 @@ static int solve_instances(pdlp_ctx *ctx, int32_t count, const pdlp_lp *lps,
       const double left_t = p64.termination_criteria.time_sec_limit - (now_sec() - t0);
       if (left_it <= 0 || !(left_t > 0)) continue;
       prior[q] = {r->iteration_count, r->total_attempts, r->refinement_rounds};
+      refined[q].save(*r, lps[q].n, lps[q].m);
       if (!fb) {
         if (!ctx->stream2) {
           hipError_t e = hipStreamCreateWithFlags(&ctx->stream2, hipStreamNonBlocking);
@@ -311,6 +360,7 @@ static int solve_instances(pdlp_ctx *ctx, int32_t count, const pdlp_lp *lps,
     r->total_attempts += prior[q].attempts;
     r->refinement_rounds = prior[q].rounds;
     r->fp64_fallback = 1;
+    refined[q].restore_if_better(r, params->termination_criteria.optimality_norm);
   }
   return 0;
 }
@@ -402,6 +452,7 @@ struct ServeDone {
   int64_t prior_iters = 0, prior_attempts = 0;
   int32_t prior_rounds = 0;
   size_t fb_bytes = 0;         // footprint of the FP64 solve, while it runs
+  RefinedPoint refined;        // the stalled FP32 solve's point
 };
 
 struct pdlp_server {
@@ -526,6 +577,7 @@ extern "C" int pdlp_serve_poll(pdlp_server *s, int64_t *ids, int32_t max_ids, in
           d.prior_iters = d.res.iteration_count;
           d.prior_attempts = d.res.total_attempts;
           d.prior_rounds = d.res.refinement_rounds;
+          d.refined.save(d.res, d.n, d.m);
           d.fb_bytes = footprint(g->P, 0);
           if (!s->fb) {
             if (!s->ctx->stream2) {
@@ -564,6 +616,8 @@ extern "C" int pdlp_serve_poll(pdlp_server *s, int64_t *ids, int32_t max_ids, in
       r.total_attempts += d.prior_attempts;
       r.refinement_rounds = d.prior_rounds;
       r.fp64_fallback = 1;
+      d.refined.restore_if_better(&r, s->prm.termination_criteria.optimality_norm);
+      d.refined = RefinedPoint();
       s->in_use -= (double)d.fb_bytes;
       s->done[h.tag] = std::move(it->second);
       s->running.erase(it);

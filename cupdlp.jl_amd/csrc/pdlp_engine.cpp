@@ -1289,8 +1289,8 @@ void Engine::refine_logic(size_t q, const double *o) {
     reason = PDLP_TERMINATION_REASON_TIME_LIMIT;
   else if (h.iters_total >= (int64_t)tc.iteration_limit)
     reason = PDLP_TERMINATION_REASON_ITERATION_LIMIT;
-  else if (h.capped_rounds >= 2 || h.ref_round >= prm_.refine_max_outer) {
-    // refinement stalled (two correction solves stopped by their cap, or the
+  else if (h.capped_rounds >= 1 || h.ref_round >= prm_.refine_max_outer) {
+    // refinement stalled (a correction solve stopped by its cap, or the
     // round limit): the caller may solve the LP again in FP64
     reason = PDLP_TERMINATION_REASON_ITERATION_LIMIT;
     h.stalled = true;

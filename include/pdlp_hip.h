@@ -213,8 +213,12 @@ typedef struct pdlp_result {
   int32_t refinement_rounds;     /* FP32_REFINE: outer rounds performed */
   int32_t fp64_fallback;         /* FP32_REFINE: 1 if refinement stalled and the
                                     LP was solved again in FP64 (pdlp_solve,
-                                    pdlp_batch_solve; x, y, the statistics and
-                                    the reason are then the FP64 solve's) */
+                                    pdlp_batch_solve, the serving loop; x, y,
+                                    the statistics and the reason are then the
+                                    FP64 solve's, except that a solve stopped
+                                    by the caller's limit further from optimal
+                                    than the refined point returns that point
+                                    and its statistics) */
 } pdlp_result;
 
 /* Receives each recorded IterationStats (replaces Vector{IterationStats}). */

@@ -47,6 +47,8 @@ CASES = {
     "transport_200x200_s2": lambda: transportation_lp(200, 200, seed=2),
     "long_row_6000": lambda: lp_cases.long_row_lp(6000)[0],
     "zipf_scaled_n20000": lambda: config_lp(1, scale=0.02)[0],
+    # the 2,000 x 2,000 transportation LP (n = 4,000,000); about 2 minutes
+    "transport_2000x2000_s2": lambda: config_lp(2)[0],
 }
 
 
