@@ -212,6 +212,32 @@ int pdlp::prepare_instance(const pdlp_lp *lp, const pdlp_params *prm, HostInst *
   return 0;
 }
 
+// A host copy of an LP: FP32 + refinement with host rescaling keeps one per
+// instance, for the FP64 solve of an instance whose refinement stalls.
+struct LpCopy {
+  pdlp_lp lp{};
+  std::vector<int64_t> colptr, rowval;
+  std::vector<double> nzval, c, lb, ub, b;
+  explicit LpCopy(const pdlp_lp &in) {
+    lp = in;
+    const int64_t n = in.n, m = in.m, nnz = in.nnz;
+    colptr.assign(in.colptr, in.colptr + n + 1);
+    rowval.assign(in.rowval, in.rowval + nnz);
+    nzval.assign(in.nzval, in.nzval + nnz);
+    c.assign(in.c, in.c + n);
+    lb.assign(in.lb, in.lb + n);
+    ub.assign(in.ub, in.ub + n);
+    b.assign(in.b, in.b + m);
+    lp.colptr = colptr.data();
+    lp.rowval = rowval.data();
+    lp.nzval = nzval.data();
+    lp.c = c.data();
+    lp.lb = lb.data();
+    lp.ub = ub.data();
+    lp.b = b.data();
+  }
+};
+
 // The refined point a stalled FP32 solve ended with, kept while the LP is
 // solved again in FP64.  Should that solve stop at the caller's iteration,
 // time or KKT-pass limit with a larger relative KKT error than the refined
@@ -258,37 +284,116 @@ struct RefinedPoint {
   }
 };
 
-// Solves `count` prepared-on-demand LPs together and fills results[i].
-// FP32 + refinement: an instance whose refinement stalls is moved, as soon as
-// it stalls, to a plain FP64 engine on the context's second stream, within
-// what is left of the caller's iteration and time limits; the two engines'
-// rounds are enqueued back to back so their kernels overlap on the GPU.  A
-// moved instance's result is the FP64 solve's (or, if that solve stops at a
-// caller's limit further from optimal, the refined point: RefinedPoint), with
-// the FP32 iteration and attempt counts added and fp64_fallback = 1.
-static int solve_instances(pdlp_ctx *ctx, int32_t count, const pdlp_lp *lps,
-                           const pdlp_params *params, pdlp_result *results, pdlp_stats_cb cb,
-                           void *user, double t0) {
+// The stream of a context's FP64 solves of stalled refinements, created on
+// first use.
+static int second_stream(pdlp_ctx *ctx, std::string *err) {
+  if (ctx->stream2) return 0;
+  hipError_t e = hipStreamCreateWithFlags(&ctx->stream2, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    *err = std::string("hipStreamCreate: ") + hipGetErrorString(e);
+    return -(int)e;
+  }
+  return 0;
+}
+
+// Stages in `fb` the plain FP64 solve of an FP32 instance whose refinement
+// stalled.  With device rescaling it starts from the FP32 instance's unscaled
+// FP64 copies on the device and its host layout (the same numbering and
+// slices), so no host data of the LP is needed; with host rescaling, from a
+// host copy of the LP.  g carries the result pointer and limits.
+static int stage_fp64(Engine *fb, const HostInst &h32, const LpCopy *lp, const pdlp_params &p64,
+                      std::unique_ptr<HostInst> g, std::string *err) {
+  if (lp) {
+    int rc = prepare_instance(&lp->lp, &p64, g.get(), err);
+    if (rc) return rc;
+    return fb->stage(std::move(g), err);
+  }
+  g->P = h32.P;
+  DevSource src;
+  src.t_ptr = h32.dev.t_ptr;
+  src.t_col = h32.dev.t_col;
+  src.a_ptr = h32.dev.a_ptr;
+  src.t_val = h32.dev.t_orig;
+  // the true problem's vectors (oc, ol, ou, ob hold the correction LP's)
+  src.c = h32.dev.rc;
+  src.l = h32.dev.rl;
+  src.u = h32.dev.ru;
+  src.b = h32.dev.rb;
+  return fb->stage(std::move(g), err, &src);
+}
+
+// A loaded batch: its LPs prepared on the host (validation, numbering,
+// slices) and copied to the device, not yet rescaled or solved.
+struct pdlp_batch {
+  pdlp_ctx *ctx = nullptr;
+  pdlp_params prm{};
+  std::unique_ptr<Engine> main;
+  int32_t count = 0;
+  std::vector<std::unique_ptr<LpCopy>> lps;  // FP32 + refinement with host rescaling only
+  double load_sec = 0.0;
+  bool ran = false;
+};
+
+static int load_batch(pdlp_ctx *ctx, int32_t count, const pdlp_lp *lps, const pdlp_params *params,
+                      std::unique_ptr<pdlp_batch> *out) {
   std::string *err = &ctx->err;
-  Engine main(ctx->stream, &ctx->pool, *params, &ctx->prof);
+  const double t0 = now_sec();
+  auto b = std::make_unique<pdlp_batch>();
+  b->ctx = ctx;
+  b->prm = *params;
+  b->count = count;
+  b->main = std::make_unique<Engine>(ctx->stream, &ctx->pool, *params, &ctx->prof);
+  const bool keep = params->precision == PDLP_PRECISION_FP32_REFINE &&
+                    !(params->device_preconditioning && !params->l2_norm_rescaling);
   for (int32_t i = 0; i < count; ++i) {
     auto h = std::make_unique<HostInst>();
     int rc = prepare_instance(&lps[i], params, h.get(), err);
+    if (rc == 0) rc = b->main->stage(std::move(h), err);
     if (rc) {
       if (count > 1) *err = "instance " + std::to_string(i) + ": " + *err;
       return rc;
     }
-    h->res = &results[i];
-    h->t_submit = t0;
-    h->cb = cb;
-    h->user = user;
+    if (keep) b->lps.push_back(std::make_unique<LpCopy>(lps[i]));
+  }
+  b->load_sec = now_sec() - t0;
+  *out = std::move(b);
+  return 0;
+}
+
+// Solves a loaded batch and fills results[i]; t0 is the start of the timed
+// call (solve_time_sec counts from it).  FP32 + refinement: an instance whose
+// refinement stalls is moved, as soon as it stalls, to a plain FP64 engine on
+// the context's second stream, within what is left of the caller's iteration
+// and time limits; the two engines' rounds are enqueued back to back so their
+// kernels overlap on the GPU.  A moved instance's result is the FP64 solve's
+// (or, if that solve stops at a caller's limit further from optimal, the
+// refined point: RefinedPoint), with the FP32 iteration and attempt counts
+// added and fp64_fallback = 1.
+static int run_batch(pdlp_batch *b, pdlp_result *results, pdlp_stats_cb cb, void *user,
+                     double t0) {
+  pdlp_ctx *ctx = b->ctx;
+  std::string *err = &ctx->err;
+  if (b->ran) {
+    *err = "a loaded batch is solved once";
+    return 1;
+  }
+  b->ran = true;
+  Engine &main = *b->main;
+  const int32_t count = b->count;
+  for (int32_t i = 0; i < count; ++i) {
+    HostInst &h = main.inst((size_t)i);
+    h.res = &results[i];
+    h.t_submit = t0;
+    h.cb = cb;
+    h.user = user;
     results[i].refinement_rounds = 0;
     results[i].fp64_fallback = 0;
-    rc = main.add(std::move(h), err);
-    if (rc) return rc;
   }
-  const double t1 = now_sec();
-  for (int32_t i = 0; i < count; ++i) results[i].preprocess_time_sec = t1 - t0;
+  int rc = main.activate_staged(err);  // rescaling and slicing on the device
+  if (rc) return rc;
+  const double prep = b->load_sec + (now_sec() - t0);
+  for (int32_t i = 0; i < count; ++i) results[i].preprocess_time_sec = prep;
+  const pdlp_params *params = &b->prm;
   if (params->precision != PDLP_PRECISION_FP32_REFINE) return main.run(err);
 
   pdlp_params p64 = *params;
@@ -308,12 +413,12 @@ static int solve_instances(pdlp_ctx *ctx, int32_t count, const pdlp_lp *lps,
     // the FP64 engine runs its (short) rounds while the FP32 chunk is in flight
     if (fb && !fb->all_done()) {
       do {
-        int rc = fb->step_round(err);
+        rc = fb->step_round(err);
         if (rc < 0) return rc;
       } while (r1 == 0 && !main.round_ready() && !fb->all_done());
     }
     if (!main_done) {
-      int rc = main.end_round(err);
+      rc = main.end_round(err);
       if (rc < 0) return rc;
     }
     if (r1 == 1) main_done = true;
@@ -326,27 +431,20 @@ static int solve_instances(pdlp_ctx *ctx, int32_t count, const pdlp_lp *lps,
       const double left_t = p64.termination_criteria.time_sec_limit - (now_sec() - t0);
       if (left_it <= 0 || !(left_t > 0)) continue;
       prior[q] = {r->iteration_count, r->total_attempts, r->refinement_rounds};
-      refined[q].save(*r, lps[q].n, lps[q].m);
+      refined[q].save(*r, h.P.n, h.P.m);
       if (!fb) {
-        if (!ctx->stream2) {
-          hipError_t e = hipStreamCreateWithFlags(&ctx->stream2, hipStreamNonBlocking);
-          if (e != hipSuccess) {
-            *err = std::string("hipStreamCreate: ") + hipGetErrorString(e);
-            return -(int)e;
-          }
-        }
+        rc = second_stream(ctx, err);
+        if (rc) return rc;
         fb = std::make_unique<Engine>(ctx->stream2, &ctx->pool, p64);
       }
       auto g = std::make_unique<HostInst>();
-      int rc = prepare_instance(&lps[q], &p64, g.get(), err);
-      if (rc) return rc;
       g->res = r;
       g->t_submit = t0;
       g->cb = cb;
       g->user = user;
       g->iteration_limit_override = left_it;
       g->time_limit_override = left_t;
-      rc = fb->add(std::move(g), err);
+      rc = stage_fp64(fb.get(), h, b->lps.empty() ? nullptr : b->lps[q].get(), p64, std::move(g), err);
       if (rc) return rc;
       r->fp64_fallback = 2;  // being solved in FP64
     }
@@ -381,7 +479,9 @@ extern "C" int pdlp_solve(pdlp_ctx *ctx, const pdlp_lp *lp, const pdlp_params *p
   int rc = check_params(params, &ctx->err);
   if (rc) return rc;
   (void)hipSetDevice(ctx->device);
-  rc = solve_instances(ctx, 1, lp, params, result, cb, user, t0);
+  std::unique_ptr<pdlp_batch> b;
+  rc = load_batch(ctx, 1, lp, params, &b);
+  if (rc == 0) rc = run_batch(b.get(), result, cb, user, t0);
   if (rc) return rc;
   result->solve_time_sec = now_sec() - t0;
   return 0;
@@ -405,36 +505,57 @@ extern "C" int pdlp_batch_solve(pdlp_ctx *ctx, int32_t count, const pdlp_lp *lps
       return 1;
     }
   }
-  return solve_instances(ctx, count, lps, params, results, nullptr, nullptr, t0);
+  std::unique_ptr<pdlp_batch> b;
+  rc = load_batch(ctx, count, lps, params, &b);
+  if (rc == 0) rc = run_batch(b.get(), results, nullptr, nullptr, t0);
+  return rc;
+}
+
+extern "C" int pdlp_batch_load(pdlp_ctx *ctx, int32_t count, const pdlp_lp *lps,
+                               const pdlp_params *params, pdlp_batch **out) {
+  if (!ctx) return 1;
+  ctx->err.clear();
+  if (!out || count < 0 || (count > 0 && !lps) || !params) {
+    ctx->err = "invalid batch arguments";
+    return 1;
+  }
+  *out = nullptr;
+  int rc = check_params(params, &ctx->err);
+  if (rc) return rc;
+  (void)hipSetDevice(ctx->device);
+  std::unique_ptr<pdlp_batch> b;
+  rc = load_batch(ctx, count, lps, params, &b);
+  if (rc) return rc;
+  *out = b.release();
+  return 0;
+}
+
+extern "C" int pdlp_batch_run(pdlp_batch *batch, pdlp_result *results) {
+  if (!batch) return 1;
+  pdlp_ctx *ctx = batch->ctx;
+  ctx->err.clear();
+  if (batch->count > 0 && !results) {
+    ctx->err = "invalid batch arguments";
+    return 1;
+  }
+  for (int32_t i = 0; i < batch->count; ++i) {
+    const Prepared &P = batch->main->inst((size_t)i).P;
+    if ((P.n > 0 && !results[i].x) || (P.m > 0 && !results[i].y)) {
+      ctx->err = "result vectors must be allocated by the caller (instance " + std::to_string(i) + ")";
+      return 1;
+    }
+  }
+  (void)hipSetDevice(ctx->device);
+  return run_batch(batch, results, nullptr, nullptr, now_sec());
+}
+
+extern "C" void pdlp_batch_free(pdlp_batch *batch) {
+  if (!batch) return;
+  (void)hipSetDevice(batch->ctx->device);
+  delete batch;
 }
 
 // ------------------------------------------------------------------ serving
-// A host copy of a request's LP (FP32 + refinement servers keep one, for the
-// FP64 solve of a request whose refinement stalls).
-struct LpCopy {
-  pdlp_lp lp{};
-  std::vector<int64_t> colptr, rowval;
-  std::vector<double> nzval, c, lb, ub, b;
-  explicit LpCopy(const pdlp_lp &in) {
-    lp = in;
-    const int64_t n = in.n, m = in.m, nnz = in.nnz;
-    colptr.assign(in.colptr, in.colptr + n + 1);
-    rowval.assign(in.rowval, in.rowval + nnz);
-    nzval.assign(in.nzval, in.nzval + nnz);
-    c.assign(in.c, in.c + n);
-    lb.assign(in.lb, in.lb + n);
-    ub.assign(in.ub, in.ub + n);
-    b.assign(in.b, in.b + m);
-    lp.colptr = colptr.data();
-    lp.rowval = rowval.data();
-    lp.nzval = nzval.data();
-    lp.c = c.data();
-    lp.lb = lb.data();
-    lp.ub = ub.data();
-    lp.b = b.data();
-  }
-};
-
 struct ServeReq {
   int64_t id;
   std::unique_ptr<HostInst> inst;
@@ -447,7 +568,7 @@ struct ServeDone {
   std::vector<double> x, y;
   int64_t n = 0, m = 0;
   pdlp_result res;
-  std::unique_ptr<LpCopy> lp;  // FP32 + refinement servers
+  std::unique_ptr<LpCopy> lp;  // FP32 + refinement servers with host rescaling
   double tol = 0.0, t_submit = 0.0;
   int64_t prior_iters = 0, prior_attempts = 0;
   int32_t prior_rounds = 0;
@@ -514,7 +635,12 @@ extern "C" int pdlp_serve_submit(pdlp_server *s, const pdlp_lp *lp, double toler
   }
   h->tag = r.id;
   r.inst = std::move(h);
-  if (s->prm.precision == PDLP_PRECISION_FP32_REFINE) r.lp = std::make_unique<LpCopy>(*lp);
+  // an FP32 request whose refinement stalls is solved again in FP64: from
+  // its own unscaled device copies with device rescaling, else from a host
+  // copy of its LP
+  if (s->prm.precision == PDLP_PRECISION_FP32_REFINE &&
+      !(s->prm.device_preconditioning && !s->prm.l2_norm_rescaling))
+    r.lp = std::make_unique<LpCopy>(*lp);
   *id = r.id;
   s->queue.push_back(std::move(r));
   return 0;
@@ -557,7 +683,7 @@ extern "C" int pdlp_serve_poll(pdlp_server *s, int64_t *ids, int32_t max_ids, in
       if (it == s->running.end()) continue;
       ServeDone &d = *it->second;
       s->in_use -= (double)footprint(h.P, refine);
-      if (refine && h.stalled && d.lp) {
+      if (refine && h.stalled) {
         // refinement stalled: the request continues as a plain FP64 solve
         // within what is left of its iteration and time limits
         pdlp_params p64 = s->prm;
@@ -566,8 +692,6 @@ extern "C" int pdlp_serve_poll(pdlp_server *s, int64_t *ids, int32_t max_ids, in
         const double left_t = p64.termination_criteria.time_sec_limit - (now_sec() - d.t_submit);
         if (left_it > 0 && left_t > 0) {
           auto g = std::make_unique<HostInst>();
-          rc = pdlp::prepare_instance(&d.lp->lp, &p64, g.get(), err);
-          if (rc) return rc;
           g->tolerance_override = d.tol;
           g->t_submit = d.t_submit;
           g->res = &d.res;
@@ -578,19 +702,16 @@ extern "C" int pdlp_serve_poll(pdlp_server *s, int64_t *ids, int32_t max_ids, in
           d.prior_attempts = d.res.total_attempts;
           d.prior_rounds = d.res.refinement_rounds;
           d.refined.save(d.res, d.n, d.m);
-          d.fb_bytes = footprint(g->P, 0);
+          d.fb_bytes = footprint(h.P, 0);
           if (!s->fb) {
-            if (!s->ctx->stream2) {
-              hipError_t e = hipStreamCreateWithFlags(&s->ctx->stream2, hipStreamNonBlocking);
-              if (e != hipSuccess) {
-                *err = std::string("hipStreamCreate: ") + hipGetErrorString(e);
-                return -(int)e;
-              }
-            }
+            rc = second_stream(s->ctx, err);
+            if (rc) return rc;
             s->fb = std::make_unique<Engine>(s->ctx->stream2, &s->ctx->pool, p64);
           }
           s->in_use += (double)d.fb_bytes;
-          rc = s->fb->add(std::move(g), err);
+          // staging copies on the device and waits for them, so the FP32
+          // instance's arena can be recycled below
+          rc = stage_fp64(s->fb.get(), h, d.lp.get(), p64, std::move(g), err);
           if (rc) return rc;
           continue;
         }

@@ -98,8 +98,10 @@ Engine::Engine(hipStream_t s, DevicePool *pool, const pdlp_params &prm, Profiler
 
 Engine::~Engine() {
   (void)hipStreamSynchronize(s_);
-  for (auto &h : inst_)
+  for (auto &h : inst_) {
     if (h->arena) pool_->put(h->arena);
+    for (void *p : h->stage.bufs) pool_->put(p);
+  }
   for (void *p : table_allocs_) pool_->put(p);
   pool_->put(d_red_);
   pool_->put(d_grow_);
@@ -135,7 +137,11 @@ hipError_t keep_originals(hipStream_t s, void *p) {
 }
 }  // namespace
 
-int Engine::upload(HostInst &h, std::string *err) {
+// Allocates the instance's arena and copies the problem into it: from the
+// host (h.P) or, with src, from another instance's originals on this device.
+// With device preparation the rescaling inputs stay staged for prepare(); a
+// host-prepared problem is uploaded scaled and sliced here.
+int Engine::transfer(HostInst &h, const DevSource *src, std::string *err) {
   Prepared &P = h.P;
   const size_t n = (size_t)P.n, m = (size_t)P.m, nnz = (size_t)P.nnz;
   const size_t ts = prec_ ? sizeof(float) : sizeof(double);
@@ -178,9 +184,19 @@ int Engine::upload(HostInst &h, std::string *err) {
   }
   char *base = (char *)h.arena;
   HIPCHK(hipMemsetAsync(base, 0, cv.off, s_));
-  auto put = [&](void *dst, const void *src, size_t bytes) -> hipError_t {
+  if (src && !P.device_prep) {
+    if (err) *err = "internal: staging from the device needs device preparation";
+    return 8;
+  }
+  // host -> device, or device -> device from src's arrays
+  auto put = [&](void *dst, const void *host, const void *dev, size_t bytes) -> hipError_t {
     if (bytes == 0) return hipSuccess;
-    return hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s_);
+    if (src) return hipMemcpyAsync(dst, dev, bytes, hipMemcpyDeviceToDevice, s_);
+    return hipMemcpyAsync(dst, host, bytes, hipMemcpyHostToDevice, s_);
+  };
+  auto hput = [&](void *dst, const void *host, size_t bytes) -> hipError_t {
+    if (bytes == 0) return hipSuccess;
+    return hipMemcpyAsync(dst, host, bytes, hipMemcpyHostToDevice, s_);
   };
   InstDev &d = h.dev;
   d.n = (int32_t)n; d.m = (int32_t)m; d.neq = (int32_t)P.neq;
@@ -215,104 +231,57 @@ int Engine::upload(HostInst &h, std::string *err) {
   d.rb = prec_ ? (const double *)(base + o_ref[8]) : nullptr;
   d.yb = prec_ ? (double *)(base + o_ref[9]) : nullptr;
   if (prec_) {
-    HIPCHK(put((void *)d.rc, P.oc.data(), 8 * n));
-    HIPCHK(put((void *)d.rl, P.ol.data(), 8 * n));
-    HIPCHK(put((void *)d.ru, P.ou.data(), 8 * n));
-    HIPCHK(put((void *)d.rb, P.ob.data(), 8 * m));
+    HIPCHK(put((void *)d.rc, P.oc.data(), src ? src->c : nullptr, 8 * n));
+    HIPCHK(put((void *)d.rl, P.ol.data(), src ? src->l : nullptr, 8 * n));
+    HIPCHK(put((void *)d.ru, P.ou.data(), src ? src->u : nullptr, 8 * n));
+    HIPCHK(put((void *)d.rb, P.ob.data(), src ? src->b : nullptr, 8 * m));
   }
-  HIPCHK(put(base + o_aptr, P.a_ptr.data(), 4 * (m + 1)));
-  HIPCHK(put(base + o_tptr, P.t_ptr.data(), 4 * (n + 1)));
-  HIPCHK(put(base + o_tcol, P.t_col.data(), 4 * nnz));
-  HIPCHK(put((void *)d.oc, P.oc.data(), 8 * n));
-  HIPCHK(put((void *)d.ol, P.ol.data(), 8 * n));
-  HIPCHK(put((void *)d.ou, P.ou.data(), 8 * n));
-  HIPCHK(put((void *)d.ob, P.ob.data(), 8 * m));
+  HIPCHK(put(base + o_aptr, P.a_ptr.data(), src ? src->a_ptr : nullptr, 4 * (m + 1)));
+  HIPCHK(put(base + o_tptr, P.t_ptr.data(), src ? src->t_ptr : nullptr, 4 * (n + 1)));
+  HIPCHK(put(base + o_tcol, P.t_col.data(), src ? src->t_col : nullptr, 4 * nnz));
+  HIPCHK(put((void *)d.oc, P.oc.data(), src ? src->c : nullptr, 8 * n));
+  HIPCHK(put((void *)d.ol, P.ol.data(), src ? src->l : nullptr, 8 * n));
+  HIPCHK(put((void *)d.ou, P.ou.data(), src ? src->u : nullptr, 8 * n));
+  HIPCHK(put((void *)d.ob, P.ob.data(), src ? src->b : nullptr, 8 * m));
 
   if (P.device_prep) {
-    // FP64 working copies: the arena's own arrays on the FP64 path, pool
-    // buffers on the FP32 path (converted at the end).
-    std::vector<void *> tmp;
-    auto scratch = [&](size_t bytes) -> void * {
+    // FP64 working copies the rescaling starts from: the arena's own arrays
+    // on the FP64 path, pool buffers on the FP32 path (converted at the end)
+    HostInst::Stage &sg = h.stage;
+    auto scratch = [&](size_t bytes) -> double * {
       void *p = pool_->get(std::max<size_t>(bytes, 256));
-      if (p) tmp.push_back(p);
-      return p;
+      if (p) sg.bufs.push_back(p);
+      return (double *)p;
     };
-    double *tw, *aw, *cw, *lw, *uw, *bw;
     if (prec_) {
-      tw = (double *)scratch(8 * nnz); aw = (double *)scratch(8 * nnz);
-      cw = (double *)scratch(8 * n); lw = (double *)scratch(8 * n); uw = (double *)scratch(8 * n);
-      bw = (double *)scratch(8 * m);
+      sg.t_val = scratch(8 * nnz);
+      sg.c = scratch(8 * n); sg.l = scratch(8 * n); sg.u = scratch(8 * n);
+      sg.b = scratch(8 * m);
     } else {
-      tw = (double *)d.t_val; aw = (double *)d.a_val;
-      cw = (double *)d.c; lw = (double *)d.l; uw = (double *)d.u; bw = (double *)d.b;
+      sg.t_val = (double *)d.t_val;
+      sg.c = (double *)d.c; sg.l = (double *)d.l; sg.u = (double *)d.u; sg.b = (double *)d.b;
     }
     // the slice tables (A^T first, then A): the row kernels of the
     // rescaling walk them, and they are reused to fill the sliced matrices
     const size_t nps = P.ps.sl.size(), nds = P.ds.sl.size();
-    SliceDesc *tab = (SliceDesc *)scratch(sizeof(SliceDesc) * (nps + nds));
-    double *rv = (double *)scratch(8 * m), *vv = (double *)scratch(8 * n);
-    int32_t *cursor = (int32_t *)scratch(4 * m);
-    double *npart = (double *)scratch(8 * 5 * 256);
-    if (!tw || !aw || !cw || !lw || !uw || !bw || !tab || !rv || !vv || !cursor || !npart) {
-      for (void *p : tmp) pool_->put(p);
+    sg.tab = (SliceDesc *)scratch(sizeof(SliceDesc) * (nps + nds));
+    if (!sg.t_val || !sg.c || !sg.l || !sg.u || !sg.b || !sg.tab) {
       if (err) *err = "device allocation for preconditioning failed";
       return -2;
     }
-    if (!norm_pinned_) HIPCHK(hipHostMalloc((void **)&norm_pinned_, 8 * 5 * 256, 0));
-    HIPCHK(put(tab, P.ps.sl.data(), sizeof(SliceDesc) * nps));
-    HIPCHK(put(tab + nps, P.ds.sl.data(), sizeof(SliceDesc) * nds));
-    HIPCHK(put(tw, P.t_val.data(), 8 * nnz));
-    HIPCHK(put(cw, P.oc.data(), 8 * n));
-    HIPCHK(put(lw, P.ol.data(), 8 * n));
-    HIPCHK(put(uw, P.ou.data(), 8 * n));
-    HIPCHK(put(bw, P.ob.data(), 8 * m));
-    HIPCHK(put((void *)d.D, P.D.data(), 8 * n));  // ones
-    HIPCHK(put((void *)d.E, P.E.data(), 8 * m));  // ones
-    if (m) HIPCHK(hipMemsetAsync(cursor, 0, 4 * m, s_));
-    PrecondArgs pa;
-    pa.n = (int64_t)n; pa.m = (int64_t)m; pa.nnz = (int64_t)nnz;
-    pa.t_ptr = d.t_ptr; pa.t_col = d.t_col; pa.t_val = tw;
-    pa.a_ptr = d.a_ptr; pa.a_col = (int32_t *)d.a_col; pa.cursor = cursor; pa.a_val = aw;
-    pa.c = cw; pa.l = lw; pa.u = uw; pa.b = bw;
-    pa.D = (double *)d.D; pa.E = (double *)d.E; pa.r = rv; pa.v = vv;
-    pa.norm_part = npart; pa.norm_host = norm_pinned_;
-    pa.ps = tab; pa.nps = (int)nps;
-    pa.ds = tab + nps; pa.nds = (int)nds;
-    pa.ruiz_iterations = prm_.l_inf_ruiz_iterations;
-    pa.has_pc = prm_.has_pock_chambolle_alpha != 0;
-    pa.pc_alpha = prm_.pock_chambolle_alpha;
-    AfterScatterCtx ac{aw, (double *)d.a_orig, tw, (double *)d.t_orig, (int64_t)nnz};
-    if (prec_) {
-      pa.after_scatter = keep_originals;
-      pa.after_scatter_ctx = &ac;
-    }
-    PrecondOut po;
-    hipError_t e = precond_run(s_, pa, &po);
-    if (e == hipSuccess && prec_) {
-      e = to_float(s_, (int64_t)nnz, tw, (float *)d.t_val);
-      if (e == hipSuccess) e = to_float(s_, (int64_t)nnz, aw, (float *)d.a_val);
-      if (e == hipSuccess) e = to_float(s_, (int64_t)n, cw, (float *)d.c);
-      if (e == hipSuccess) e = to_float(s_, (int64_t)n, lw, (float *)d.l);
-      if (e == hipSuccess) e = to_float(s_, (int64_t)n, uw, (float *)d.u);
-      if (e == hipSuccess) e = to_float(s_, (int64_t)m, bw, (float *)d.b);
-    }
-    if (e == hipSuccess && n) e = hipMemcpyAsync(d.gbuf, d.c, ts * n, hipMemcpyDeviceToDevice, s_);
-    if (e == hipSuccess && n) e = hipMemcpyAsync(d.lg, d.c, ts * n, hipMemcpyDeviceToDevice, s_);
-    if (e == hipSuccess && n) e = hipMemcpyAsync(P.D.data(), d.D, 8 * n, hipMemcpyDeviceToHost, s_);
-    if (e == hipSuccess && m) e = hipMemcpyAsync(P.E.data(), d.E, 8 * m, hipMemcpyDeviceToHost, s_);
-    if (e == hipSuccess) e = hipStreamSynchronize(s_);
-    int rc = 0;
-    if (e == hipSuccess) rc = fill_slices(h, err, tab);
-    for (void *p : tmp) pool_->put(p);
-    if (e != hipSuccess) {
-      if (err) *err = std::string("device preconditioning failed: ") + hipGetErrorString(e);
-      return -(int)e;
-    }
-    if (rc) return rc;
-    P.max_abs_scaled = po.max_abs;
-    P.norm_b_scaled = po.norm_b;
-    P.norm_c_scaled = po.norm_c;
-    // host copies of the scaled data are not needed any more
+    HIPCHK(hput(sg.tab, P.ps.sl.data(), sizeof(SliceDesc) * nps));
+    HIPCHK(hput(sg.tab + nps, P.ds.sl.data(), sizeof(SliceDesc) * nds));
+    HIPCHK(put(sg.t_val, P.t_val.data(), src ? src->t_val : nullptr, 8 * nnz));
+    HIPCHK(put(sg.c, P.oc.data(), src ? src->c : nullptr, 8 * n));
+    HIPCHK(put(sg.l, P.ol.data(), src ? src->l : nullptr, 8 * n));
+    HIPCHK(put(sg.u, P.ou.data(), src ? src->u : nullptr, 8 * n));
+    HIPCHK(put(sg.b, P.ob.data(), src ? src->b : nullptr, 8 * m));
+    P.D.assign(n, 1.0);  // the scaling starts from ones
+    P.E.assign(m, 1.0);
+    HIPCHK(hput((void *)d.D, P.D.data(), 8 * n));
+    HIPCHK(hput((void *)d.E, P.E.data(), 8 * m));
+    HIPCHK(hipStreamSynchronize(s_));
+    // host copy of the values not needed any more
     std::vector<double>().swap(P.t_val);
     return 0;
   }
@@ -321,19 +290,19 @@ int Engine::upload(HostInst &h, std::string *err) {
   std::vector<float> tmpf;
   auto putT = [&](void *dst, const std::vector<double> &v) -> hipError_t {
     if (v.empty()) return hipSuccess;
-    if (!prec_) return put(dst, v.data(), 8 * v.size());
+    if (!prec_) return hput(dst, v.data(), 8 * v.size());
     tmpf.assign(v.begin(), v.end());
-    hipError_t e = put(dst, tmpf.data(), 4 * v.size());
+    hipError_t e = hput(dst, tmpf.data(), 4 * v.size());
     if (e == hipSuccess) e = hipStreamSynchronize(s_);
     return e;
   };
-  HIPCHK(put(base + o_acol, P.a_col.data(), 4 * nnz));
+  HIPCHK(hput(base + o_acol, P.a_col.data(), 4 * nnz));
   if (prec_) {  // unscaled FP64 values in both layouts
     std::vector<double> ao(nnz), to(nnz);
     for (size_t k = 0; k < nnz; ++k) to[k] = P.t_orig_host.empty() ? 0.0 : P.t_orig_host[k];
     for (size_t p = 0; p < nnz; ++p) ao[p] = to[P.csr_from_csc[p]];
-    HIPCHK(put((void *)d.t_orig, to.data(), 8 * nnz));
-    HIPCHK(put((void *)d.a_orig, ao.data(), 8 * nnz));
+    HIPCHK(hput((void *)d.t_orig, to.data(), 8 * nnz));
+    HIPCHK(hput((void *)d.a_orig, ao.data(), 8 * nnz));
     HIPCHK(hipStreamSynchronize(s_));
   }
   HIPCHK(putT(base + o_aval, P.a_val));
@@ -344,10 +313,88 @@ int Engine::upload(HostInst &h, std::string *err) {
   HIPCHK(putT(d.gbuf, P.c));  // gbuf = c - A^T 0
   HIPCHK(putT(d.lg, P.c));    // last restart gradient
   HIPCHK(putT((void *)d.b, P.b));
-  HIPCHK(put((void *)d.D, P.D.data(), 8 * n));
-  HIPCHK(put((void *)d.E, P.E.data(), 8 * m));
+  HIPCHK(hput((void *)d.D, P.D.data(), 8 * n));
+  HIPCHK(hput((void *)d.E, P.E.data(), 8 * m));
   HIPCHK(hipStreamSynchronize(s_));
   return fill_slices(h, err);
+}
+
+// Device preparation of a staged instance: rescaling (Ruiz, Pock-Chambolle)
+// and the CSR scatter on the device, the FP32 conversions, the sliced
+// matrices.  Returns the staged buffers to the pool.
+int Engine::prepare(HostInst &h, std::string *err) {
+  Prepared &P = h.P;
+  if (!P.device_prep) return 0;
+  const size_t n = (size_t)P.n, m = (size_t)P.m, nnz = (size_t)P.nnz;
+  const size_t ts = prec_ ? sizeof(float) : sizeof(double);
+  InstDev &d = h.dev;
+  HostInst::Stage &sg = h.stage;
+  if (!norm_pinned_) HIPCHK(hipHostMalloc((void **)&norm_pinned_, 8 * 5 * 256, 0));
+  std::vector<void *> tmp;
+  auto scratch = [&](size_t bytes) -> void * {
+    void *p = pool_->get(std::max<size_t>(bytes, 256));
+    if (p) tmp.push_back(p);
+    return p;
+  };
+  double *aw = prec_ ? (double *)scratch(8 * nnz) : (double *)d.a_val;
+  double *rv = (double *)scratch(8 * m), *vv = (double *)scratch(8 * n);
+  int32_t *cursor = (int32_t *)scratch(4 * m);
+  double *npart = (double *)scratch(8 * 5 * 256);
+  auto release = [&]() {
+    for (void *p : tmp) pool_->put(p);
+    for (void *p : sg.bufs) pool_->put(p);
+    sg = HostInst::Stage();
+  };
+  if (!aw || !rv || !vv || !cursor || !npart) {
+    release();
+    if (err) *err = "device allocation for preconditioning failed";
+    return -2;
+  }
+  hipError_t e = m ? hipMemsetAsync(cursor, 0, 4 * m, s_) : hipSuccess;
+  PrecondArgs pa;
+  pa.n = (int64_t)n; pa.m = (int64_t)m; pa.nnz = (int64_t)nnz;
+  pa.t_ptr = d.t_ptr; pa.t_col = d.t_col; pa.t_val = sg.t_val;
+  pa.a_ptr = d.a_ptr; pa.a_col = (int32_t *)d.a_col; pa.cursor = cursor; pa.a_val = aw;
+  pa.c = sg.c; pa.l = sg.l; pa.u = sg.u; pa.b = sg.b;
+  pa.D = (double *)d.D; pa.E = (double *)d.E; pa.r = rv; pa.v = vv;
+  pa.norm_part = npart; pa.norm_host = norm_pinned_;
+  pa.ps = sg.tab; pa.nps = (int)P.ps.sl.size();
+  pa.ds = sg.tab + P.ps.sl.size(); pa.nds = (int)P.ds.sl.size();
+  pa.ruiz_iterations = prm_.l_inf_ruiz_iterations;
+  pa.has_pc = prm_.has_pock_chambolle_alpha != 0;
+  pa.pc_alpha = prm_.pock_chambolle_alpha;
+  AfterScatterCtx ac{aw, (double *)d.a_orig, sg.t_val, (double *)d.t_orig, (int64_t)nnz};
+  if (prec_) {
+    pa.after_scatter = keep_originals;
+    pa.after_scatter_ctx = &ac;
+  }
+  PrecondOut po;
+  if (e == hipSuccess) e = precond_run(s_, pa, &po);
+  if (e == hipSuccess && prec_) {
+    e = to_float(s_, (int64_t)nnz, sg.t_val, (float *)d.t_val);
+    if (e == hipSuccess) e = to_float(s_, (int64_t)nnz, aw, (float *)d.a_val);
+    if (e == hipSuccess) e = to_float(s_, (int64_t)n, sg.c, (float *)d.c);
+    if (e == hipSuccess) e = to_float(s_, (int64_t)n, sg.l, (float *)d.l);
+    if (e == hipSuccess) e = to_float(s_, (int64_t)n, sg.u, (float *)d.u);
+    if (e == hipSuccess) e = to_float(s_, (int64_t)m, sg.b, (float *)d.b);
+  }
+  if (e == hipSuccess && n) e = hipMemcpyAsync(d.gbuf, d.c, ts * n, hipMemcpyDeviceToDevice, s_);
+  if (e == hipSuccess && n) e = hipMemcpyAsync(d.lg, d.c, ts * n, hipMemcpyDeviceToDevice, s_);
+  if (e == hipSuccess && n) e = hipMemcpyAsync(P.D.data(), d.D, 8 * n, hipMemcpyDeviceToHost, s_);
+  if (e == hipSuccess && m) e = hipMemcpyAsync(P.E.data(), d.E, 8 * m, hipMemcpyDeviceToHost, s_);
+  if (e == hipSuccess) e = hipStreamSynchronize(s_);
+  int rc = 0;
+  if (e == hipSuccess) rc = fill_slices(h, err, sg.tab);
+  release();
+  if (e != hipSuccess) {
+    if (err) *err = std::string("device preconditioning failed: ") + hipGetErrorString(e);
+    return -(int)e;
+  }
+  if (rc) return rc;
+  P.max_abs_scaled = po.max_abs;
+  P.norm_b_scaled = po.norm_b;
+  P.norm_c_scaled = po.norm_c;
+  return 0;
 }
 
 // Builds the sliced copies of the scaled A^T and A from the device CSR.
@@ -469,14 +516,49 @@ int Engine::constant_step(HostInst &h, double *eta, double *passes, std::string 
   return 0;
 }
 
-int Engine::add(std::unique_ptr<HostInst> h, std::string *err) {
-  const bool constant = prm_.step_size_policy == PDLP_STEP_CONSTANT;
-  if (constant && prec_) {
+int Engine::stage(std::unique_ptr<HostInst> h, std::string *err, const DevSource *src) {
+  if (prm_.step_size_policy == PDLP_STEP_CONSTANT && prec_) {
     if (err) *err = "ConstantStepsizeParams needs PDLP_PRECISION_FP64";
     return 4;
   }
-  int rc = upload(*h, err);
+  int rc = transfer(*h, src, err);
+  if (rc) {
+    (void)hipStreamSynchronize(s_);
+    for (void *p : h->stage.bufs) pool_->put(p);
+    if (h->arena) pool_->put(h->arena);
+    return rc;
+  }
+  h->staged = true;
+  Ctl c;
+  memset(&c, 0, sizeof c);
+  inst_.push_back(std::move(h));
+  ctl_.push_back(c);
+  tables_dirty_ = true;
+  return 0;
+}
+
+int Engine::add(std::unique_ptr<HostInst> h, std::string *err) {
+  int rc = stage(std::move(h), err);
   if (rc) return rc;
+  return activate(inst_.size() - 1, err);
+}
+
+int Engine::activate_staged(std::string *err) {
+  for (size_t q = 0; q < inst_.size(); ++q) {
+    if (!inst_[q]->staged) continue;
+    int rc = activate(q, err);
+    if (rc) return rc;
+  }
+  return 0;
+}
+
+// Prepares a staged instance on the device and sets its initial state.
+int Engine::activate(size_t q, std::string *err) {
+  HostInst *h = inst_[q].get();
+  int rc = prepare(*h, err);
+  if (rc) return rc;
+  h->staged = false;
+  const bool constant = prm_.step_size_policy == PDLP_STEP_CONSTANT;
   Ctl c;
   memset(&c, 0, sizeof c);
   const Prepared &P = h->P;
@@ -506,8 +588,7 @@ int Engine::add(std::unique_ptr<HostInst> h, std::string *err) {
   h->needs_eval = true;
   h->eval_iter = 1;
   h->start_time = now_sec();
-  inst_.push_back(std::move(h));
-  ctl_.push_back(c);
+  ctl_[q] = c;
   tables_dirty_ = true;
   return 0;
 }
@@ -569,13 +650,13 @@ int Engine::rebuild_tables(std::string *err) {
   for (size_t q = 0; q < inst_.size(); ++q) {
     HostInst &h = *inst_[q];
     h.dev.pb0 = (int32_t)pb.size();
-    if (!h.done) partition_rows(h.P.t_ptr, h.P.n, (int)q, pb, longs, &nslots);
+    if (!h.done && !h.staged) partition_rows(h.P.t_ptr, h.P.n, (int)q, pb, longs, &nslots);
     h.dev.pb1 = (int32_t)pb.size();
   }
   for (size_t q = 0; q < inst_.size(); ++q) {
     HostInst &h = *inst_[q];
     h.dev.db0 = (int32_t)db.size();
-    if (!h.done) partition_rows(h.P.a_ptr, h.P.m, (int)q, db, longs, &nslots);
+    if (!h.done && !h.staged) partition_rows(h.P.a_ptr, h.P.m, (int)q, db, longs, &nslots);
     h.dev.db1 = (int32_t)db.size();
   }
   // slice tables of the step kernels: normal slices, then the chunk slices
@@ -605,7 +686,7 @@ int Engine::rebuild_tables(std::string *err) {
     h.dev.pl0 = (int32_t)pls.size();
     h.dev.ds0 = (int32_t)ds.size();
     h.dev.dl0 = (int32_t)dls.size();
-    if (!h.done) {
+    if (!h.done && !h.staged) {
       add_slices(h.P.ps, (int)q, ps, pls);
       add_slices(h.P.ds, (int)q, ds, dls);
     }
@@ -1369,6 +1450,8 @@ void Engine::reset_inner(size_t q, double omega0) {
 }
 
 int Engine::begin_round(std::string *err) {
+  int rc0 = activate_staged(err);
+  if (rc0) return rc0;
   // Rebuild the block tables once finished instances hold a quarter of them.
   int64_t dead = 0;
   for (auto &h : inst_)

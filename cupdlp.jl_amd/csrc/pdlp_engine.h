@@ -43,6 +43,16 @@ struct HipError {
   std::string where;
 };
 
+// The original problem data of an instance already resident on this device
+// (an FP32 + refinement instance keeps its unscaled FP64 matrix, t_orig, and
+// the true problem's vectors, rc rl ru rb): staging another instance of the
+// same LP from it copies on the device.
+struct DevSource {
+  const int32_t *t_ptr = nullptr, *t_col = nullptr, *a_ptr = nullptr;
+  const double *t_val = nullptr;  // unscaled CSC values
+  const double *c = nullptr, *l = nullptr, *u = nullptr, *b = nullptr;
+};
+
 // Per-instance host state of the optimize() loop.
 struct HostInst {
   Prepared P;
@@ -81,6 +91,15 @@ struct HostInst {
   int64_t iteration_limit_override = -1;  // >= 0: tighter iteration limit for this instance
   double time_limit_override = -1.0;      // >= 0: tighter time limit (s) for this instance
   int64_t tag = -1;                  // serving: request id
+  // Between Engine::stage and activation: the uploaded originals the device
+  // preparation starts from (pool buffers on the FP32 path, the arena's own
+  // arrays on the FP64 path) and the slice tables.
+  bool staged = false;
+  struct Stage {
+    double *t_val = nullptr, *c = nullptr, *l = nullptr, *u = nullptr, *b = nullptr;
+    SliceDesc *tab = nullptr;
+    std::vector<void *> bufs;  // pool buffers to return
+  } stage;
 };
 
 // Event-based timing of the iteration kernels (see pdlp_kernel_profile).
@@ -95,8 +114,13 @@ class Engine {
   Engine(hipStream_t s, DevicePool *pool, const pdlp_params &prm, Profiler *prof = nullptr);
   ~Engine();
 
-  // Adds an instance (already prepared/rescaled on the host).
+  // Adds an instance (already prepared/rescaled on the host): stage + activation.
   int add(std::unique_ptr<HostInst> inst, std::string *err);
+  // Copies an instance to the device (from the host, or from `src` on this
+  // device) without rescaling or solving it; the next round, or
+  // activate_staged, prepares it on the device and starts it.
+  int stage(std::unique_ptr<HostInst> inst, std::string *err, const DevSource *src = nullptr);
+  int activate_staged(std::string *err);
   // Runs all instances to termination.  Returns 0 or a negative HIP error.
   int run(std::string *err);
   // Runs one round: pending evaluations, then one chunk of attempts.  Returns
@@ -115,7 +139,9 @@ class Engine {
   void remove_done();  // drops finished instances (serving)
 
  private:
-  int upload(HostInst &h, std::string *err);
+  int transfer(HostInst &h, const DevSource *src, std::string *err);
+  int prepare(HostInst &h, std::string *err);
+  int activate(size_t q, std::string *err);
   int fill_slices(HostInst &h, std::string *err, const SliceDesc *dev_tab = nullptr);
   int constant_step(HostInst &h, double *eta, double *passes, std::string *err);
   int rebuild_tables(std::string *err);

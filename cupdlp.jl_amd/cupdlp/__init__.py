@@ -11,5 +11,6 @@ from .types import (AdaptiveStepsizeParams, ConstantStepsizeParams, PdhgParamete
                     RestartParameters, construct_restart_parameters,
                     construct_termination_criteria, linear_programming_problem,
                     termination_reason_to_string)
-from .solver import Context, PdlpError, Server, default_context, default_params, optimize  # noqa: F401
+from .solver import (Context, LoadedBatch, PdlpError, Server, default_context,  # noqa: F401
+                     default_params, optimize)
 from .moi import Optimizer  # noqa: F401

@@ -138,6 +138,7 @@ EXPORTED = (
     "pdlp_solve", "pdlp_batch_solve", "pdlp_serve_open", "pdlp_serve_submit",
     "pdlp_serve_poll", "pdlp_serve_result", "pdlp_serve_close", "pdlp_abi_version",
     "pdlp_abi_layout", "pdlp_set_profiling", "pdlp_get_profile",
+    "pdlp_batch_load", "pdlp_batch_run", "pdlp_batch_free",
 )
 
 
@@ -157,6 +158,13 @@ def declare(lib):
     lib.pdlp_batch_solve.argtypes = [C.c_void_p, C.c_int32, C.POINTER(pdlp_lp),
                                      C.POINTER(pdlp_params), C.POINTER(pdlp_result)]
     lib.pdlp_batch_solve.restype = C.c_int
+    lib.pdlp_batch_load.argtypes = [C.c_void_p, C.c_int32, C.POINTER(pdlp_lp),
+                                    C.POINTER(pdlp_params), C.POINTER(C.c_void_p)]
+    lib.pdlp_batch_load.restype = C.c_int
+    lib.pdlp_batch_run.argtypes = [C.c_void_p, C.POINTER(pdlp_result)]
+    lib.pdlp_batch_run.restype = C.c_int
+    lib.pdlp_batch_free.argtypes = [C.c_void_p]
+    lib.pdlp_batch_free.restype = None
     lib.pdlp_serve_open.argtypes = [C.c_void_p, C.POINTER(pdlp_params), C.c_double, C.c_int32,
                                     C.POINTER(C.c_void_p)]
     lib.pdlp_serve_open.restype = C.c_int

@@ -209,6 +209,62 @@ class Context:
             outs.append(_output_from_result(results[i], x, y, p.num_variables, p.num_constraints))
         return outs
 
+    def batch_load(self, problems, params):
+        """Copies the LPs to the device without solving them (pdlp_batch_load);
+        ``run()`` on the result solves them as ``batch_solve`` would."""
+        return LoadedBatch(self, problems, params)
+
+
+class LoadedBatch:
+    """LPs resident on the device (pdlp_batch_load), solved once by ``run``
+    (pdlp_batch_run); ``solve_time_sec`` of its results counts from the start
+    of the run.  No counterpart in the reference."""
+
+    def __init__(self, ctx, problems, params):
+        self.ctx = ctx
+        self.lib = ctx.lib
+        self.handle = None
+        k = len(problems)
+        lps = (_abi.pdlp_lp * k)()
+        keeps = []
+        for i, p in enumerate(problems):
+            lp, keep = to_c_lp(p)
+            lps[i] = lp
+            keeps.append(keep)
+        cp = to_c_params(params)
+        h = C.c_void_p()
+        rc = self.lib.pdlp_batch_load(ctx.handle, k, lps, C.byref(cp), C.byref(h))
+        del keeps  # the library keeps no pointer into the caller's arrays
+        if rc != 0:
+            raise PdlpError(rc, ctx.error())
+        self.handle = h
+        self.sizes = [(p.num_variables, p.num_constraints) for p in problems]
+
+    def run(self):
+        k = len(self.sizes)
+        results = (_abi.pdlp_result * k)()
+        bufs = []
+        for i, (n, m) in enumerate(self.sizes):
+            x = np.zeros(max(n, 1)); y = np.zeros(max(m, 1))
+            bufs.append((x, y))
+            results[i].x = _ptr(x, C.c_double); results[i].y = _ptr(y, C.c_double)
+        rc = self.lib.pdlp_batch_run(self.handle, results)
+        if rc != 0:
+            raise PdlpError(rc, self.ctx.error())
+        return [_output_from_result(results[i], x, y, n, m)
+                for i, ((x, y), (n, m)) in enumerate(zip(bufs, self.sizes))]
+
+    def free(self):
+        if self.handle:
+            self.lib.pdlp_batch_free(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
 
 def _output_from_result(r, x, y, n, m):
     reason = TerminationReason(r.termination_reason)

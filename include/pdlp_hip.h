@@ -207,8 +207,9 @@ typedef struct pdlp_result {
   int32_t termination_reason;
   int32_t iteration_count;
   pdlp_iteration_stats final_stats;
-  double solve_time_sec;         /* wall time of the whole call */
-  double preprocess_time_sec;    /* validation + rescaling + upload */
+  double solve_time_sec;         /* wall time of the whole call (pdlp_batch_run:
+                                    from the start of the run) */
+  double preprocess_time_sec;    /* validation + layout + upload + rescaling */
   int64_t total_attempts;        /* total_number_iterations of the reference */
   int32_t refinement_rounds;     /* FP32_REFINE: outer rounds performed */
   int32_t fp64_fallback;         /* FP32_REFINE: 1 if refinement stalled and the
@@ -250,6 +251,23 @@ int pdlp_solve(pdlp_ctx *ctx, const pdlp_lp *lp, const pdlp_params *params,
  * termination (extension; each instance follows optimize() exactly). */
 int pdlp_batch_solve(pdlp_ctx *ctx, int32_t count, const pdlp_lp *lps,
                      const pdlp_params *params, pdlp_result *results);
+
+/* pdlp_batch_solve in two halves (extension), so that the LPs can be made
+ * resident in HBM before the solve is timed or scheduled.  optimize() copies
+ * the problem to the GPU (src/primal_dual_hybrid_gradient_gpu.jl:488-490)
+ * before its solve clock starts (:602); here the rescaling, done on the
+ * device, belongs to the run.
+ * pdlp_batch_load validates the LPs, builds their device layout on the host
+ * and copies them to the device; the caller's arrays may be freed on return.
+ * pdlp_batch_run rescales on the device, solves and writes `results` exactly
+ * as pdlp_batch_solve would (bitwise); solve_time_sec counts from the start
+ * of the run.  A loaded batch is solved once; free it with pdlp_batch_free
+ * before its context is destroyed.  Runs on one context do not overlap. */
+typedef struct pdlp_batch pdlp_batch;
+int pdlp_batch_load(pdlp_ctx *ctx, int32_t count, const pdlp_lp *lps,
+                    const pdlp_params *params, pdlp_batch **out);
+int pdlp_batch_run(pdlp_batch *batch, pdlp_result *results);
+void pdlp_batch_free(pdlp_batch *batch);
 
 /* Continuous-batching request loop (extension). */
 typedef struct pdlp_server pdlp_server;

@@ -14,7 +14,7 @@ import pytest
 
 import lp_cases
 import oracle as O
-from cupdlp import (PdhgParameters, Precision, TerminationReason as TR,
+from cupdlp import (PdhgParameters, PdlpError, Precision, TerminationReason as TR,
                    construct_termination_criteria)
 from cupdlp.generators import config_lp, random_lp, transportation_lp
 
@@ -139,6 +139,32 @@ def test_stalled_refinement_falls_back_to_fp64(gpu_ctx):
             assert np.array_equal(o.primal_solution, f.primal_solution)
             assert np.array_equal(o.dual_solution, f.dual_solution)
         check_kkt(prob, a, 1e-8)
+
+
+def test_batch_load_then_run_matches_batch_solve(gpu_ctx):
+    """pdlp_batch_load + pdlp_batch_run return bitwise what pdlp_batch_solve
+    returns, in FP64 and in FP32 + refinement with correction solves capped
+    at 64 iterations (the stalled instances' FP64 solves then start from the
+    FP32 instances' device copies); a loaded batch is solved once."""
+    say("case: 3 LPs, load + run vs batch_solve, FP64 and FP32 + refinement (inner cap 64)")
+    probs = [config_lp(0)[0], transportation_lp(40, 50, seed=6), lp_cases.tiny_two_var()[0]]
+    p64 = params(1e-8)
+    p64.precision = Precision.FP64
+    for p in (p64, params(1e-8, refine_inner_iteration_limit=64)):
+        ref = gpu_ctx.batch_solve(probs, p)
+        b = gpu_ctx.batch_load(probs, p)
+        outs = b.run()
+        with pytest.raises(PdlpError):
+            b.run()
+        b.free()
+        for o, r in zip(outs, ref):
+            assert o.termination_reason == r.termination_reason
+            assert o.iteration_count == r.iteration_count
+            assert o.fp64_fallback == r.fp64_fallback
+            assert np.array_equal(o.primal_solution, r.primal_solution)
+            assert np.array_equal(o.dual_solution, r.dual_solution)
+        if p.precision == Precision.FP32_REFINE:
+            assert any(o.fp64_fallback for o in outs)
 
 
 def test_refined_small_cases(gpu_ctx):
