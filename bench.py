@@ -114,9 +114,7 @@ def profile_roofline(prof, kernel_names, note, workload=None, precision=None):
 def cpu_baseline_batch(problems, tol, budget_s=20.0):
     """The CPU oracle on a bounded sample: instances in batch order until the
     time budget is spent."""
-    sys.path.insert(0, os.path.join(REPO, "oracle"))
-    import oracle as O
-    O.build()
+    O = _oracle()
     threads = O.lib().orc_num_threads()
     p = make_params(tol, "fp64")
     O.set_exact(False)  # an ordinary FP64 CPU implementation for timing
@@ -132,6 +130,54 @@ def cpu_baseline_batch(problems, tol, budget_s=20.0):
     return {"value": round(done / dt, 4), "unit": "solves/s", "cores": threads, "kind": "port",
             "sample": f"first {done} of the {len(problems)} batch LPs, one at a time, "
                       f"tol {tol:g}, {dt:.1f} s, plain FP64 sums, OpenMP"}
+
+
+def _oracle():
+    sys.path.insert(0, os.path.join(REPO, "oracle"))
+    import oracle as O
+    O.build()
+    return O
+
+
+def cpu_baseline_single(prob, tol, iteration_limit, gpu_iters_per_step, budget_s=20.0):
+    """The CPU oracle on the same LP with the same limits, its time limit set
+    to the budget: iterations/s, and the solves/s that rate gives for the GPU
+    step's iteration count (the unit of `value`)."""
+    O = _oracle()
+    threads = O.lib().orc_num_threads()
+    p = make_params(tol, "fp64", iteration_limit=iteration_limit, time_limit=budget_s)
+    O.set_exact(False)
+    t0 = time.time()
+    out = O.solve(prob, p)
+    dt = time.time() - t0
+    O.set_exact(True)
+    its = out.iteration_count
+    ips = its / dt
+    return {"value": round(ips / max(gpu_iters_per_step, 1), 6), "unit": "solves/s",
+            "cores": threads, "kind": "port", "iterations_per_s": round(ips, 2),
+            "sample": f"the same LP, {its} iterations ({out.termination_string}) in {dt:.1f} s "
+                      f"including rescaling, plain FP64 sums, OpenMP; value = iterations/s / "
+                      f"{gpu_iters_per_step:g} iterations per GPU step"}
+
+
+def cpu_baseline_stream(problems, tols, budget_s=20.0):
+    """The CPU oracle on the requests of the stream in arrival order, one at a
+    time at each request's tolerance, until the time budget is spent."""
+    O = _oracle()
+    threads = O.lib().orc_num_threads()
+    O.set_exact(False)
+    t0 = time.time()
+    done = 0
+    for prob, tol in zip(problems, tols):
+        O.solve(prob, make_params(tol, "fp64"))
+        done += 1
+        if time.time() - t0 > budget_s:
+            break
+    dt = time.time() - t0
+    O.set_exact(True)
+    return {"value": round(done / dt, 4), "unit": "solves/s", "cores": threads, "kind": "port",
+            "sample": f"first {done} of the {len(problems)} requests, one at a time at their "
+                      f"tolerances, {dt:.1f} s, plain FP64 sums, OpenMP"}
 
 
 def solve_resident(ctx, problems, params):
@@ -285,6 +331,7 @@ def run_serve(args, ctx, lib):
                        f"1000..{args.serve_nmax}, 10% transportation 200..1000 per side; tolerance "
                        f"1e-4 (70%) / 1e-8 (30%)), Poisson arrivals at {args.rate:g}/s",
            "precision": args.precision, "hbm_budget_GB": args.hbm_budget}
+    extra["_tols"] = tols
     return value, dt, cfg, extra, None, probs
 
 
@@ -393,8 +440,15 @@ def main():
         value, dt, cfg, extra, roof, problems = run_single(args, ctx, lib, int(args.workload[1]))
         metric_unit = "solves/s"
     cpu = None
+    tols = extra.pop("_tols", None)
     if not args.no_cpu_baseline:
-        cpu = cpu_baseline_batch(problems, args.tol, args.cpu_budget)
+        if args.workload == "batch":
+            cpu = cpu_baseline_batch(problems, args.tol, args.cpu_budget)
+        elif args.workload == "serve":
+            cpu = cpu_baseline_stream(problems, tols, args.cpu_budget)
+        else:
+            cpu = cpu_baseline_single(problems[0], args.tol, args.iters,
+                                      extra["iterations"] / args.steps, args.cpu_budget)
     with open(os.path.join(REPO, "BASELINE.json")) as f:
         metric = json.load(f)["metric"]
     line = {
