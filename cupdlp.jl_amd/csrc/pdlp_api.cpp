@@ -3,11 +3,13 @@
 #include <math.h>
 #include <string.h>
 
+#include <algorithm>
 #include <deque>
 #include <map>
 #include <memory>
 #include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "pdlp_engine.h"
@@ -20,6 +22,7 @@ struct pdlp_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
   hipStream_t stream2 = nullptr;  // FP64 solves of stalled refinements (created on first use)
+  std::vector<hipStream_t> extra; // streams of a batch's further groups (created on first use)
   DevicePool pool;
   Profiler prof;
   std::string err;
@@ -130,6 +133,10 @@ void pdlp_ctx_destroy(pdlp_ctx *ctx) {
   if (ctx->stream2) {
     (void)hipStreamSynchronize(ctx->stream2);
     (void)hipStreamDestroy(ctx->stream2);
+  }
+  for (hipStream_t s : ctx->extra) {
+    (void)hipStreamSynchronize(s);
+    (void)hipStreamDestroy(s);
   }
   delete ctx;
 }
@@ -284,15 +291,28 @@ struct RefinedPoint {
   }
 };
 
-// The stream of a context's FP64 solves of stalled refinements, created on
-// first use.
-static int second_stream(pdlp_ctx *ctx, std::string *err) {
-  if (ctx->stream2) return 0;
-  hipError_t e = hipStreamCreateWithFlags(&ctx->stream2, hipStreamNonBlocking);
-  if (e != hipSuccess) {
-    *err = std::string("hipStreamCreate: ") + hipGetErrorString(e);
-    return -(int)e;
+// Stream k of a context, created on first use: 0 is the context's stream,
+// 1 the stream of the FP64 solves of stalled refinements, 2g and 2g + 1 the
+// same pair for group g of a batch.  Default priority, non-blocking.
+static int ctx_stream(pdlp_ctx *ctx, int k, hipStream_t *out, std::string *err) {
+  hipStream_t *slot;
+  if (k == 0) {
+    slot = &ctx->stream;
+  } else if (k == 1) {
+    slot = &ctx->stream2;
+  } else {
+    if ((int)ctx->extra.size() < k - 1) ctx->extra.resize(k - 1, nullptr);
+    slot = &ctx->extra[k - 2];
   }
+  if (!*slot) {
+    hipError_t e = hipStreamCreateWithFlags(slot, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+      *slot = nullptr;
+      *err = std::string("hipStreamCreate: ") + hipGetErrorString(e);
+      return -(int)e;
+    }
+  }
+  *out = *slot;
   return 0;
 }
 
@@ -322,17 +342,42 @@ static int stage_fp64(Engine *fb, const HostInst &h32, const LpCopy *lp, const p
   return fb->stage(std::move(g), err, &src);
 }
 
+// One group of a batch: its instances packed into one engine's launches on
+// its own stream, with its own FP64 engine for stalled refinements.
+struct BatchGroup {
+  std::unique_ptr<Engine> main;
+  std::vector<int32_t> idx;  // the caller's index of each engine instance
+  std::vector<std::unique_ptr<LpCopy>> lps;  // FP32 + refinement with host rescaling only
+  int stream_main = 0, stream_fb = 1;        // ctx_stream numbers
+  Profiler prof;
+  std::string err;
+};
+
 // A loaded batch: its LPs prepared on the host (validation, numbering,
-// slices) and copied to the device, not yet rescaled or solved.
+// slices) and copied to the device, not yet rescaled or solved.  Instances
+// are split by size into groups that run side by side: every launch of an
+// engine carries one attempt of each of its running instances, so a small
+// instance that needs many iterations would otherwise advance at the pace
+// of the large instances' launches.
 struct pdlp_batch {
   pdlp_ctx *ctx = nullptr;
   pdlp_params prm{};
-  std::unique_ptr<Engine> main;
   int32_t count = 0;
-  std::vector<std::unique_ptr<LpCopy>> lps;  // FP32 + refinement with host rescaling only
+  std::vector<std::unique_ptr<BatchGroup>> groups;
   double load_sec = 0.0;
   bool ran = false;
+  ~pdlp_batch() {
+    for (auto &g : groups) {
+      g->main.reset();
+      for (hipEvent_t ev : g->prof.events) (void)hipEventDestroy(ev);
+    }
+  }
 };
+
+static int batch_group_count(int32_t count, const pdlp_params *params) {
+  int g = params->batch_groups > 0 ? params->batch_groups : (count >= 24 ? 3 : 1);
+  return std::max(1, std::min(g, std::min<int>(count, 8)));
+}
 
 static int load_batch(pdlp_ctx *ctx, int32_t count, const pdlp_lp *lps, const pdlp_params *params,
                       std::unique_ptr<pdlp_batch> *out) {
@@ -342,62 +387,81 @@ static int load_batch(pdlp_ctx *ctx, int32_t count, const pdlp_lp *lps, const pd
   b->ctx = ctx;
   b->prm = *params;
   b->count = count;
-  b->main = std::make_unique<Engine>(ctx->stream, &ctx->pool, *params, &ctx->prof);
-  const bool keep = params->precision == PDLP_PRECISION_FP32_REFINE &&
-                    !(params->device_preconditioning && !params->l2_norm_rescaling);
-  for (int32_t i = 0; i < count; ++i) {
-    auto h = std::make_unique<HostInst>();
-    int rc = prepare_instance(&lps[i], params, h.get(), err);
-    if (rc == 0) rc = b->main->stage(std::move(h), err);
-    if (rc) {
-      if (count > 1) *err = "instance " + std::to_string(i) + ": " + *err;
-      return rc;
+  const bool refine = params->precision == PDLP_PRECISION_FP32_REFINE;
+  const bool keep = refine && !(params->device_preconditioning && !params->l2_norm_rescaling);
+  // groups of equal instance counts, in order of nonzeros
+  const int G = batch_group_count(count, params);
+  std::vector<int32_t> order(count);
+  for (int32_t i = 0; i < count; ++i) order[i] = i;
+  std::stable_sort(order.begin(), order.end(),
+                   [&](int32_t a, int32_t c) { return lps[a].nnz < lps[c].nnz; });
+  for (int g = 0; g < G; ++g) {
+    auto grp = std::make_unique<BatchGroup>();
+    grp->stream_main = 2 * g;
+    grp->stream_fb = 2 * g + 1;
+    hipStream_t s;
+    int rc = ctx_stream(ctx, grp->stream_main, &s, err);
+    if (rc) return rc;
+    grp->main = std::make_unique<Engine>(s, &ctx->pool, *params, &grp->prof);
+    const int32_t lo = (int32_t)((int64_t)count * g / G), hi = (int32_t)((int64_t)count * (g + 1) / G);
+    for (int32_t k = lo; k < hi; ++k) grp->idx.push_back(order[k]);
+    std::sort(grp->idx.begin(), grp->idx.end());
+    b->groups.push_back(std::move(grp));
+  }
+  for (auto &grp : b->groups) {
+    for (int32_t i : grp->idx) {
+      auto h = std::make_unique<HostInst>();
+      int rc = prepare_instance(&lps[i], params, h.get(), err);
+      if (rc == 0) rc = grp->main->stage(std::move(h), err);
+      if (rc) {
+        if (count > 1) *err = "instance " + std::to_string(i) + ": " + *err;
+        return rc;
+      }
+      if (keep) grp->lps.push_back(std::make_unique<LpCopy>(lps[i]));
     }
-    if (keep) b->lps.push_back(std::make_unique<LpCopy>(lps[i]));
   }
   b->load_sec = now_sec() - t0;
   *out = std::move(b);
   return 0;
 }
 
-// Solves a loaded batch and fills results[i]; t0 is the start of the timed
-// call (solve_time_sec counts from it).  FP32 + refinement: an instance whose
-// refinement stalls is moved, as soon as it stalls, to a plain FP64 engine on
-// the context's second stream, within what is left of the caller's iteration
-// and time limits; the two engines' rounds are enqueued back to back so their
-// kernels overlap on the GPU.  A moved instance's result is the FP64 solve's
-// (or, if that solve stops at a caller's limit further from optimal, the
-// refined point: RefinedPoint), with the FP32 iteration and attempt counts
-// added and fp64_fallback = 1.
-static int run_batch(pdlp_batch *b, pdlp_result *results, pdlp_stats_cb cb, void *user,
-                     double t0) {
+// Solves one group of a loaded batch, filling results[idx[q]]; t0 is the
+// start of the timed call (solve_time_sec counts from it).  FP32 +
+// refinement: an instance whose refinement stalls is moved, as soon as it
+// stalls, to a plain FP64 engine on the group's second stream, within what
+// is left of the caller's iteration and time limits; the two engines'
+// rounds are enqueued back to back so their kernels overlap on the GPU.  A
+// moved instance's result is the FP64 solve's (or, if that solve stops at a
+// caller's limit further from optimal, the refined point: RefinedPoint),
+// with the FP32 iteration and attempt counts added and fp64_fallback = 1.
+static int run_group(pdlp_batch *b, BatchGroup &grp, pdlp_result *results, pdlp_stats_cb cb,
+                     void *user, double t0) {
   pdlp_ctx *ctx = b->ctx;
-  std::string *err = &ctx->err;
-  if (b->ran) {
-    *err = "a loaded batch is solved once";
-    return 1;
-  }
-  b->ran = true;
-  Engine &main = *b->main;
-  const int32_t count = b->count;
-  for (int32_t i = 0; i < count; ++i) {
-    HostInst &h = main.inst((size_t)i);
-    h.res = &results[i];
+  std::string *err = &grp.err;
+  Engine &main = *grp.main;
+  const int32_t count = (int32_t)grp.idx.size();
+  for (int32_t q = 0; q < count; ++q) {
+    HostInst &h = main.inst((size_t)q);
+    pdlp_result *r = &results[grp.idx[q]];
+    h.res = r;
     h.t_submit = t0;
     h.cb = cb;
     h.user = user;
-    results[i].refinement_rounds = 0;
-    results[i].fp64_fallback = 0;
+    r->refinement_rounds = 0;
+    r->fp64_fallback = 0;
   }
   int rc = main.activate_staged(err);  // rescaling and slicing on the device
   if (rc) return rc;
   const double prep = b->load_sec + (now_sec() - t0);
-  for (int32_t i = 0; i < count; ++i) results[i].preprocess_time_sec = prep;
+  for (int32_t q = 0; q < count; ++q) results[grp.idx[q]].preprocess_time_sec = prep;
   const pdlp_params *params = &b->prm;
   if (params->precision != PDLP_PRECISION_FP32_REFINE) return main.run(err);
 
   pdlp_params p64 = *params;
   p64.precision = PDLP_PRECISION_FP64;
+  hipStream_t s_fb;
+  rc = ctx_stream(ctx, grp.stream_fb, &s_fb, err);  // created before the groups start
+  if (rc) return rc;
   std::unique_ptr<Engine> fb;  // kernel profiling covers the FP32 engine only
   struct Prior {
     int64_t iters, attempts;
@@ -426,17 +490,13 @@ static int run_batch(pdlp_batch *b, pdlp_result *results, pdlp_stats_cb cb, void
       const HostInst &h = main.inst((size_t)q);
       if (!h.done || !h.stalled || moved[q]) continue;
       moved[q] = 1;
-      pdlp_result *r = &results[q];
+      pdlp_result *r = &results[grp.idx[q]];
       const int64_t left_it = (int64_t)p64.termination_criteria.iteration_limit - r->iteration_count;
       const double left_t = p64.termination_criteria.time_sec_limit - (now_sec() - t0);
       if (left_it <= 0 || !(left_t > 0)) continue;
       prior[q] = {r->iteration_count, r->total_attempts, r->refinement_rounds};
       refined[q].save(*r, h.P.n, h.P.m);
-      if (!fb) {
-        rc = second_stream(ctx, err);
-        if (rc) return rc;
-        fb = std::make_unique<Engine>(ctx->stream2, &ctx->pool, p64);
-      }
+      if (!fb) fb = std::make_unique<Engine>(s_fb, &ctx->pool, p64);
       auto g = std::make_unique<HostInst>();
       g->res = r;
       g->t_submit = t0;
@@ -444,14 +504,15 @@ static int run_batch(pdlp_batch *b, pdlp_result *results, pdlp_stats_cb cb, void
       g->user = user;
       g->iteration_limit_override = left_it;
       g->time_limit_override = left_t;
-      rc = stage_fp64(fb.get(), h, b->lps.empty() ? nullptr : b->lps[q].get(), p64, std::move(g), err);
+      rc = stage_fp64(fb.get(), h, grp.lps.empty() ? nullptr : grp.lps[q].get(), p64,
+                      std::move(g), err);
       if (rc) return rc;
       r->fp64_fallback = 2;  // being solved in FP64
     }
     if (main_done && (!fb || fb->all_done())) break;
   }
   for (int32_t q = 0; q < count; ++q) {
-    pdlp_result *r = &results[q];
+    pdlp_result *r = &results[grp.idx[q]];
     if (r->fp64_fallback != 2) continue;
     const int64_t it = (int64_t)r->iteration_count + prior[q].iters;
     r->iteration_count = (int32_t)std::min<int64_t>(it, INT32_MAX);
@@ -460,6 +521,53 @@ static int run_batch(pdlp_batch *b, pdlp_result *results, pdlp_stats_cb cb, void
     r->fp64_fallback = 1;
     refined[q].restore_if_better(r, params->termination_criteria.optimality_norm);
   }
+  return 0;
+}
+
+// Solves a loaded batch: group 0 on the calling thread, the others on host
+// threads of their own (each drives its engines on its own streams).
+static int run_batch(pdlp_batch *b, pdlp_result *results, pdlp_stats_cb cb, void *user,
+                     double t0) {
+  pdlp_ctx *ctx = b->ctx;
+  if (b->ran) {
+    ctx->err = "a loaded batch is solved once";
+    return 1;
+  }
+  b->ran = true;
+  const int G = (int)b->groups.size();
+  for (auto &grp : b->groups) grp->prof.enabled = ctx->prof.enabled;
+  if (b->prm.precision == PDLP_PRECISION_FP32_REFINE) {
+    for (auto &grp : b->groups) {
+      hipStream_t s;
+      int rc = ctx_stream(ctx, grp->stream_fb, &s, &ctx->err);
+      if (rc) return rc;
+    }
+  }
+  std::vector<int> rcs(G, 0);
+  std::vector<std::thread> threads;
+  for (int g = 1; g < G; ++g)
+    threads.emplace_back([&, g]() {
+      (void)hipSetDevice(ctx->device);
+      rcs[g] = run_group(b, *b->groups[g], results, cb, user, t0);
+    });
+  if (G > 0) rcs[0] = run_group(b, *b->groups[0], results, cb, user, t0);
+  for (auto &t : threads) t.join();
+  pdlp_kernel_profile &acc = ctx->prof.acc;
+  for (auto &grp : b->groups) {
+    const pdlp_kernel_profile &p = grp->prof.acc;
+    acc.primal_launches += p.primal_launches;
+    acc.dual_launches += p.dual_launches;
+    acc.primal_ms += p.primal_ms;
+    acc.dual_ms += p.dual_ms;
+    acc.primal_bytes += p.primal_bytes;
+    acc.dual_bytes += p.dual_bytes;
+    grp->prof.acc = pdlp_kernel_profile{};
+  }
+  for (int g = 0; g < G; ++g)
+    if (rcs[g]) {
+      ctx->err = b->groups[g]->err;
+      return rcs[g];
+    }
   return 0;
 }
 
@@ -538,13 +646,15 @@ extern "C" int pdlp_batch_run(pdlp_batch *batch, pdlp_result *results) {
     ctx->err = "invalid batch arguments";
     return 1;
   }
-  for (int32_t i = 0; i < batch->count; ++i) {
-    const Prepared &P = batch->main->inst((size_t)i).P;
-    if ((P.n > 0 && !results[i].x) || (P.m > 0 && !results[i].y)) {
-      ctx->err = "result vectors must be allocated by the caller (instance " + std::to_string(i) + ")";
-      return 1;
+  for (auto &grp : batch->groups)
+    for (size_t q = 0; q < grp->idx.size(); ++q) {
+      const Prepared &P = grp->main->inst(q).P;
+      const int32_t i = grp->idx[q];
+      if ((P.n > 0 && !results[i].x) || (P.m > 0 && !results[i].y)) {
+        ctx->err = "result vectors must be allocated by the caller (instance " + std::to_string(i) + ")";
+        return 1;
+      }
     }
-  }
   (void)hipSetDevice(ctx->device);
   return run_batch(batch, results, nullptr, nullptr, now_sec());
 }
@@ -704,9 +814,10 @@ extern "C" int pdlp_serve_poll(pdlp_server *s, int64_t *ids, int32_t max_ids, in
           d.refined.save(d.res, d.n, d.m);
           d.fb_bytes = footprint(h.P, 0);
           if (!s->fb) {
-            rc = second_stream(s->ctx, err);
+            hipStream_t s2;
+            rc = ctx_stream(s->ctx, 1, &s2, err);
             if (rc) return rc;
-            s->fb = std::make_unique<Engine>(s->ctx->stream2, &s->ctx->pool, p64);
+            s->fb = std::make_unique<Engine>(s2, &s->ctx->pool, p64);
           }
           s->in_use += (double)d.fb_bytes;
           // staging copies on the device and waits for them, so the FP32

@@ -41,6 +41,7 @@ static inline double jl_max(double a, double b) {
 DevicePool::~DevicePool() { release_all(); }
 
 void *DevicePool::get(size_t bytes) {
+  std::lock_guard<std::mutex> lk(mu_);
   bytes = (bytes + 255) & ~(size_t)255;
   if (bytes == 0) bytes = 256;
   size_t best = (size_t)-1;
@@ -73,6 +74,7 @@ void *DevicePool::get(size_t bytes) {
 
 void DevicePool::put(void *p) {
   if (!p) return;
+  std::lock_guard<std::mutex> lk(mu_);
   for (size_t i = 0; i < used_.size(); ++i)
     if (used_[i].p == p) {
       in_use_ -= used_[i].bytes;
@@ -83,6 +85,7 @@ void DevicePool::put(void *p) {
 }
 
 void DevicePool::release_all() {
+  std::lock_guard<std::mutex> lk(mu_);
   for (auto &b : free_) (void)hipFree(b.p);
   for (auto &b : used_) (void)hipFree(b.p);
   free_.clear();

@@ -14,6 +14,7 @@
 #include <stdint.h>
 
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -23,17 +24,22 @@
 
 namespace pdlp {
 
-// Caching device allocator owned by a context.
+// Caching device allocator owned by a context; the engines of a batch's
+// groups share it from their host threads.
 class DevicePool {
  public:
   ~DevicePool();
   void *get(size_t bytes);
   void put(void *p);
   void release_all();
-  size_t in_use() const { return in_use_; }
+  size_t in_use() const {
+    std::lock_guard<std::mutex> lk(mu_);
+    return in_use_;
+  }
 
  private:
   struct Blk { void *p; size_t bytes; };
+  mutable std::mutex mu_;
   std::vector<Blk> free_, used_;
   size_t in_use_ = 0;
 };

@@ -65,7 +65,8 @@ class pdlp_params(C.Structure):
         ("refine_max_scale", C.c_double),
         ("device_preconditioning", C.c_int32),
         ("refine_inner_iteration_limit", C.c_int32),
-        ("reserved", C.c_int32 * 6),
+        ("batch_groups", C.c_int32),
+        ("reserved", C.c_int32 * 5),
     ]
 
 

@@ -92,6 +92,7 @@ def to_c_params(p: PdhgParameters) -> _abi.pdlp_params:
     cp.refine_max_scale = float(p.refine_max_scale)
     cp.device_preconditioning = int(bool(p.device_preconditioning))
     cp.refine_inner_iteration_limit = int(p.refine_inner_iteration_limit)
+    cp.batch_groups = int(p.batch_groups)
     return cp
 
 

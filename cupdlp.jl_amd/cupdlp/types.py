@@ -218,6 +218,7 @@ class PdhgParameters:
     refine_max_scale: float = 1.0e6
     device_preconditioning: bool = True
     refine_inner_iteration_limit: int = 0
+    batch_groups: int = 0
 
 
 @dataclass

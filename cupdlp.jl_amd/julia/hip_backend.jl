@@ -144,7 +144,8 @@ struct CParams
     refine_max_scale::Float64
     device_preconditioning::Int32
     refine_inner_iteration_limit::Int32
-    reserved::NTuple{6,Int32}
+    batch_groups::Int32
+    reserved::NTuple{5,Int32}
 end
 
 struct CConvergence
@@ -232,7 +233,7 @@ function _c_params(p::PdhgParameters)
                    adaptive ? p.step_size_policy_params.growth_exponent : 0.6,
                    ct, cr,
                    0,        # PDLP_PRECISION_FP64: the reference's arithmetic
-                   12, 1e-4, 1e6, 1, 0, ntuple(_ -> Int32(0), 6))
+                   12, 1e-4, 1e6, 1, 0, 0, ntuple(_ -> Int32(0), 5))
 end
 
 function _stats(s::CIterationStats)

@@ -143,7 +143,10 @@ typedef struct pdlp_params {
   int32_t device_preconditioning;    /* 1: Ruiz/Pock-Chambolle on the GPU */
   int32_t refine_inner_iteration_limit; /* FP32_REFINE: iteration cap of each correction
                                            solve; 0 = max(10000, 20 x the first solve's) */
-  int32_t reserved[6];
+  int32_t batch_groups;              /* batched solves: size groups run side by side, each
+                                        with its own launches and streams; 0 = automatic
+                                        (3 from 24 instances up), 1 = one group */
+  int32_t reserved[5];
 } pdlp_params;
 
 /* ConvergenceInformation — src/solve_log.jl:51-155 (same field order). */

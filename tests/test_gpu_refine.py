@@ -167,6 +167,31 @@ def test_batch_load_then_run_matches_batch_solve(gpu_ctx):
             assert any(o.fp64_fallback for o in outs)
 
 
+def test_batch_groups_bitwise(gpu_ctx):
+    """A batch split into size groups, each on its own streams and host
+    thread, returns bitwise what one group returns, in FP64 and in FP32 +
+    refinement with correction solves capped at 64 iterations (FP64
+    fallbacks in every group)."""
+    say("case: 6 LPs, batch_groups 3 vs 1, FP64 and FP32 + refinement (inner cap 64)")
+    probs = [config_lp(0)[0], transportation_lp(40, 50, seed=6), lp_cases.tiny_two_var()[0],
+             random_lp(n=3000, m=1500, seed=3)[0], random_lp(n=800, m=400, seed=4)[0],
+             lp_cases.long_col_lp()[0]]
+    p64 = params(1e-8)
+    p64.precision = Precision.FP64
+    for p in (p64, params(1e-8, refine_inner_iteration_limit=64)):
+        p.batch_groups = 1
+        ref = gpu_ctx.batch_solve(probs, p)
+        p.batch_groups = 3
+        out = gpu_ctx.batch_solve(probs, p)
+        for o, r in zip(out, ref):
+            assert o.termination_reason == r.termination_reason
+            assert o.iteration_count == r.iteration_count
+            assert o.total_attempts == r.total_attempts
+            assert o.fp64_fallback == r.fp64_fallback
+            assert np.array_equal(o.primal_solution, r.primal_solution)
+            assert np.array_equal(o.dual_solution, r.dual_solution)
+
+
 def test_refined_small_cases(gpu_ctx):
     say("case: tiny LPs, FP32 + refinement")
     prob, opt = lp_cases.tiny_two_var()
