@@ -50,9 +50,8 @@ HBM_PEAK_GBS = 8000.0
 # and used only when this run's algorithmic bytes per launch match the
 # profiled run's (the same deterministic workload).
 MEASURED_TRAFFIC = {
-    ("batch", "fp64", "k_dual"): (109563308.65, 164.82e6),
-    ("batch", "fp64", "k_primal"): (135796999.84, 164.45e6),
-    ("batch", "fp32", "k_dual"): (454888827.88, 670.05e6),
+    ("batch", "fp64", "k_dual"): (69376581.66, 103.47e6),
+    ("batch", "fp32", "k_dual"): (180747947.88, 266.16e6),
     ("c1", "fp64", "k_dual"): (195460496.0, 285.70e6),
     ("c1", "fp64", "k_primal"): (229460496.0, 248.51e6),
 }
@@ -62,12 +61,16 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+BATCH_GROUPS = 0  # --batch-groups
+
+
 def make_params(tol, precision, iteration_limit=2**31 - 1, time_limit=float("inf")):
     from cupdlp import PdhgParameters, Precision, construct_termination_criteria
     tc = construct_termination_criteria(eps_optimal_absolute=tol, eps_optimal_relative=tol,
                                         iteration_limit=iteration_limit, time_sec_limit=time_limit)
     return PdhgParameters(termination_criteria=tc, record_iteration_stats=False,
-                          precision=Precision.FP32_REFINE if precision == "fp32" else Precision.FP64)
+                          precision=Precision.FP32_REFINE if precision == "fp32" else Precision.FP64,
+                          batch_groups=BATCH_GROUPS)
 
 
 def batch_problems(count):
@@ -411,6 +414,8 @@ def main():
                     help="batch: skip the plain FP64 batched solve reported beside the FP32 one")
     ap.add_argument("--tol", type=float, default=1e-8)
     ap.add_argument("--count", type=int, default=512)
+    ap.add_argument("--batch-groups", type=int, default=0,
+                    help="batch: size groups run side by side (0 = the library's default)")
     ap.add_argument("--iters", type=int, default=1000)
     ap.add_argument("--serial-budget", type=float, default=20.0)
     ap.add_argument("--cpu-budget", type=float, default=20.0)
@@ -418,6 +423,8 @@ def main():
     ap.add_argument("--time-limit", type=float, default=float("inf"),
                     help="per-solve time limit (a guard; instances that hit it count as not optimal)")
     args = ap.parse_args()
+    global BATCH_GROUPS
+    BATCH_GROUPS = args.batch_groups
     if args.precision is None:
         args.precision = "fp32" if args.workload == "batch" else "fp64"
     if args.gpus != 1:
