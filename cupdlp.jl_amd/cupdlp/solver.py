@@ -150,7 +150,8 @@ def run_solve(call, problem: QuadraticProgrammingProblem, params: PdhgParameters
         primal_solution=x[:n].copy(), dual_solution=y[:m].copy(), termination_reason=reason,
         termination_string=termination_reason_to_string(reason),
         iteration_count=int(res.iteration_count), iteration_stats=stats,
-        solve_time_sec=res.solve_time_sec, total_attempts=int(res.total_attempts),
+        solve_time_sec=res.solve_time_sec, preprocess_time_sec=res.preprocess_time_sec,
+        total_attempts=int(res.total_attempts),
         refinement_rounds=int(res.refinement_rounds), fp64_fallback=bool(res.fp64_fallback),
         primal_current=None if xc is None else xc[:n].copy(),
         dual_current=None if yc is None else yc[:m].copy())
@@ -273,7 +274,8 @@ def _output_from_result(r, x, y, n, m):
         primal_solution=x[:n].copy(), dual_solution=y[:m].copy(),
         termination_reason=reason, termination_string=termination_reason_to_string(reason),
         iteration_count=int(r.iteration_count), iteration_stats=[from_c_stats(r.final_stats)],
-        solve_time_sec=r.solve_time_sec, total_attempts=int(r.total_attempts),
+        solve_time_sec=r.solve_time_sec, preprocess_time_sec=r.preprocess_time_sec,
+        total_attempts=int(r.total_attempts),
         refinement_rounds=int(r.refinement_rounds), fp64_fallback=bool(r.fp64_fallback))
 
 

@@ -275,6 +275,7 @@ class SaddlePointOutput:
     iteration_stats: List[IterationStats]
     # extensions
     solve_time_sec: float = 0.0
+    preprocess_time_sec: float = 0.0
     total_attempts: int = 0
     refinement_rounds: int = 0
     fp64_fallback: bool = False
