@@ -14,3 +14,4 @@ from .types import (AdaptiveStepsizeParams, ConstantStepsizeParams, PdhgParamete
 from .solver import (Context, LoadedBatch, PdlpError, Server, default_context,  # noqa: F401
                      default_params, optimize)
 from .moi import Optimizer  # noqa: F401
+from .io import MpsError, qps_reader_to_standard_form  # noqa: F401

@@ -23,6 +23,13 @@ class pdlp_lp(C.Structure):
     ]
 
 
+class pdlp_mps_problem(C.Structure):
+    _fields_ = [
+        ("lp", pdlp_lp), ("num_file_variables", C.c_int64), ("num_slack_variables", C.c_int64),
+        ("quadratic_entries", C.c_int64), ("owner", C.c_void_p),
+    ]
+
+
 class pdlp_termination_criteria(C.Structure):
     _fields_ = [
         ("optimality_norm", C.c_int32), ("iteration_limit", C.c_int32),
@@ -140,6 +147,7 @@ EXPORTED = (
     "pdlp_serve_poll", "pdlp_serve_result", "pdlp_serve_close", "pdlp_abi_version",
     "pdlp_abi_layout", "pdlp_set_profiling", "pdlp_get_profile",
     "pdlp_batch_load", "pdlp_batch_run", "pdlp_batch_free",
+    "pdlp_read_mps", "pdlp_mps_free",
 )
 
 
@@ -166,6 +174,11 @@ def declare(lib):
     lib.pdlp_batch_run.restype = C.c_int
     lib.pdlp_batch_free.argtypes = [C.c_void_p]
     lib.pdlp_batch_free.restype = None
+    lib.pdlp_read_mps.argtypes = [C.c_char_p, C.c_int32, C.POINTER(C.POINTER(pdlp_mps_problem)),
+                                  C.c_char_p, C.c_int64]
+    lib.pdlp_read_mps.restype = C.c_int
+    lib.pdlp_mps_free.argtypes = [C.POINTER(pdlp_mps_problem)]
+    lib.pdlp_mps_free.restype = None
     lib.pdlp_serve_open.argtypes = [C.c_void_p, C.POINTER(pdlp_params), C.c_double, C.c_int32,
                                     C.POINTER(C.c_void_p)]
     lib.pdlp_serve_open.restype = C.c_int

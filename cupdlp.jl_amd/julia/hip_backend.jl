@@ -324,3 +324,38 @@ function optimize(params::PdhgParameters, original_problem::QuadraticProgramming
     reason = TerminationReason(res.termination_reason)
     return SaddlePointOutput(x, y, reason, "", res.iteration_count, stats)
 end
+
+# ---------------------------------------------------------------- MPS input
+# qps_reader_to_standard_form (src/quadratic_programming_io.jl:147-186) over
+# pdlp_read_mps: the file is parsed and put in standard form natively (the
+# QPSReader and GZip dependencies go away).  Include this after
+# quadratic_programming_io.jl to replace its definition, or in its place.
+
+struct CMpsProblem
+    lp::CLp
+    num_file_variables::Int64
+    num_slack_variables::Int64
+    quadratic_entries::Int64
+    owner::Ptr{Cvoid}
+end
+
+function qps_reader_to_standard_form(filename::String; fixed_format::Bool = false)
+    out = Ref{Ptr{CMpsProblem}}(C_NULL)
+    err = zeros(UInt8, 512)
+    rc = ccall((:pdlp_read_mps, LIBPDLP), Cint,
+               (Cstring, Int32, Ref{Ptr{CMpsProblem}}, Ptr{UInt8}, Int64),
+               filename, Int32(fixed_format), out, err, length(err))
+    rc == 0 || error(unsafe_string(pointer(err)))
+    try
+        p = unsafe_load(out[])
+        lp = p.lp
+        n, m, nnz = Int(lp.n), Int(lp.m), Int(lp.nnz)
+        p.quadratic_entries == 0 || error("quadratic objectives are not supported by the GPU path")
+        vec(ptr, len) = copy(unsafe_wrap(Vector{eltype(ptr)}, ptr, len))
+        A = SparseMatrixCSC(m, n, vec(lp.colptr, n + 1), vec(lp.rowval, nnz), vec(lp.nzval, nnz))
+        return QuadraticProgrammingProblem(vec(lp.lb, n), vec(lp.ub, n), spzeros(n, n),
+                                           vec(lp.c, n), lp.c0, A, vec(lp.b, m), Int(lp.num_eq))
+    finally
+        ccall((:pdlp_mps_free, LIBPDLP), Cvoid, (Ptr{CMpsProblem},), out[])
+    end
+end

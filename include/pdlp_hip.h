@@ -272,6 +272,27 @@ int pdlp_batch_load(pdlp_ctx *ctx, int32_t count, const pdlp_lp *lps,
 int pdlp_batch_run(pdlp_batch *batch, pdlp_result *results);
 void pdlp_batch_free(pdlp_batch *batch);
 
+/* An LP read from an MPS / QPS file in the reference's standard form --
+ * replaces qps_reader_to_standard_form (src/quadratic_programming_io.jl:147-186)
+ * with transform_to_standard_form (:33-82) and two_sided_rows_to_slacks
+ * (:91-127): equality rows first, ">=" rows after, "<=" rows negated, a slack
+ * column per two-sided row (after the file's columns).  Host-only (no
+ * device).  `lp` points into memory the library owns until pdlp_mps_free. */
+typedef struct pdlp_mps_problem {
+  pdlp_lp lp;
+  int64_t num_file_variables;   /* columns of the file (slacks follow) */
+  int64_t num_slack_variables;
+  int64_t quadratic_entries;    /* nonzero QUADOBJ / QMATRIX entries read */
+  void *owner;                  /* library-owned storage */
+} pdlp_mps_problem;
+/* Reads `path` (free MPS unless fixed_format; gzip-compressed files are read
+ * as such).  Returns 0, or > 0 with a message in err (err_len bytes):
+ * 1 bad argument, 2 unreadable / malformed file, 3 a row with no finite
+ * bound (the reference's error). */
+int pdlp_read_mps(const char *path, int32_t fixed_format, pdlp_mps_problem **out, char *err,
+                  int64_t err_len);
+void pdlp_mps_free(pdlp_mps_problem *p);
+
 /* Continuous-batching request loop (extension). */
 typedef struct pdlp_server pdlp_server;
 int pdlp_serve_open(pdlp_ctx *ctx, const pdlp_params *params,
