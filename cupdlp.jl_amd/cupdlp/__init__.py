@@ -15,3 +15,4 @@ from .solver import (Context, LoadedBatch, PdlpError, Server, default_context,  
                      default_params, optimize)
 from .moi import Optimizer  # noqa: F401
 from .io import MpsError, qps_reader_to_standard_form  # noqa: F401
+from .types import SolveLog  # noqa: F401

@@ -11,6 +11,7 @@ reference (paths relative to the cuPDLP.jl tree):
 * ``ConvergenceInformation`` / ``InfeasibilityInformation`` / ``IterationStats``
   src/solve_log.jl:51-302
 * ``SaddlePointOutput``              src/saddle_point_gpu.jl:3-34
+* ``SolveLog``                       src/solve_log.jl:336-402
 * the enums                         src/solve_log.jl:19-45, 323-334;
   src/saddle_point_gpu.jl:343-350; src/termination.jl:6
 """
@@ -263,6 +264,19 @@ class IterationStats:
     step_size: float = 0.0
     primal_weight: float = 0.0
     method_specific_stats: dict = field(default_factory=dict)
+
+
+@dataclass
+class SolveLog:
+    instance_name: str = ""
+    command_line_invocation: str = ""
+    termination_reason: TerminationReason = TerminationReason.TERMINATION_REASON_UNSPECIFIED
+    termination_string: str = ""
+    iteration_count: int = 0
+    solve_time_sec: float = 0.0
+    solution_stats: IterationStats = field(default_factory=IterationStats)
+    solution_type: PointType = PointType.POINT_TYPE_UNSPECIFIED
+    iteration_stats: List[IterationStats] = field(default_factory=list)
 
 
 @dataclass
