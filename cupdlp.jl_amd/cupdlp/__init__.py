@@ -16,3 +16,4 @@ from .solver import (Context, LoadedBatch, PdlpError, Server, default_context,  
 from .moi import Optimizer  # noqa: F401
 from .io import MpsError, qps_reader_to_standard_form  # noqa: F401
 from .types import SolveLog  # noqa: F401
+from .presolve import PresolveInfo, presolve, undo_presolve  # noqa: F401
