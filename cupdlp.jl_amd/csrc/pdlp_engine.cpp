@@ -742,10 +742,8 @@ int Engine::rebuild_tables(std::string *err) {
   d_epm_ = (double *)alloc(sizeof(double) * 2 * kEvalM * db.size());
   d_epr_ = (double *)alloc(sizeof(double) * 2 * pb.size());
   d_out_ = (double *)alloc(sizeof(double) * kEvalStride * ninst);
-  d_step_cnt_ = (int32_t *)alloc(sizeof(int32_t) * ninst);
   if (!d_pslices_ || !d_dslices_ || !d_pblocks_ || !d_dblocks_ || !d_longs_ || !d_long_cnt_ || !d_long_part_ || !d_insts_ ||
-      !d_ctl_ || !d_ppart_ || !d_dpart_ || !d_epn_ || !d_epm_ || !d_epr_ || !d_out_ ||
-      !d_step_cnt_) {
+      !d_ctl_ || !d_ppart_ || !d_dpart_ || !d_epn_ || !d_epm_ || !d_epr_ || !d_out_) {
     if (err) *err = "device allocation of block tables failed";
     return -2;
   }
@@ -761,7 +759,6 @@ int Engine::rebuild_tables(std::string *err) {
     HIPCHK(hipMemcpyAsync(d_longs_, longs.data(), sizeof(LongRow) * longs.size(), hipMemcpyHostToDevice, s_));
     HIPCHK(hipMemsetAsync(d_long_cnt_, 0, sizeof(int32_t) * longs.size(), s_));
   }
-  HIPCHK(hipMemsetAsync(d_step_cnt_, 0, sizeof(int32_t) * std::max<size_t>(ninst, 1), s_));
   std::vector<InstDev> devs(ninst);
   for (size_t q = 0; q < ninst; ++q) devs[q] = inst_[q]->dev;
   HIPCHK(hipMemcpyAsync(d_insts_, devs.data(), sizeof(InstDev) * ninst, hipMemcpyHostToDevice, s_));

@@ -183,7 +183,6 @@ class Engine {
   BlockDesc *d_pblocks_ = nullptr, *d_dblocks_ = nullptr;
   LongRow *d_longs_ = nullptr;
   int32_t *d_long_cnt_ = nullptr;
-  int32_t *d_step_cnt_ = nullptr;
   double *d_long_part_ = nullptr;
   InstDev *d_insts_ = nullptr;
   Ctl *d_ctl_ = nullptr;
