@@ -54,6 +54,8 @@ MEASURED_TRAFFIC = {
     ("batch", "fp32", "k_dual"): (180747947.88, 266.16e6),
     ("c1", "fp64", "k_dual"): (195460496.0, 285.70e6),
     ("c1", "fp64", "k_primal"): (229460496.0, 248.51e6),
+    ("c1", "fp32", "k_dual"): (123640332.0, 174.86e6),
+    ("c2", "fp64", "k_primal"): (400032004.0, 421.45e6),
 }
 
 
