@@ -51,10 +51,10 @@ HBM_PEAK_GBS = 8000.0
 # profiled run's (the same deterministic workload).
 MEASURED_TRAFFIC = {
     ("batch", "fp64", "k_dual"): (69376581.66, 103.47e6),
-    ("batch", "fp32", "k_dual"): (180747947.88, 266.16e6),
-    ("c1", "fp64", "k_dual"): (195460496.0, 285.70e6),
-    ("c1", "fp64", "k_primal"): (229460496.0, 248.51e6),
-    ("c1", "fp32", "k_dual"): (123640332.0, 174.86e6),
+    ("batch", "fp32", "k_dual"): (180747947.88, 264.46e6),
+    ("c1", "fp64", "k_dual"): (195460496.0, 283.94e6),
+    ("c1", "fp64", "k_primal"): (229460496.0, 235.38e6),
+    ("c1", "fp32", "k_dual"): (123640332.0, 171.69e6),
     ("c2", "fp64", "k_primal"): (400032004.0, 421.45e6),
 }
 

@@ -43,7 +43,7 @@ constexpr int kEvalStride = kEvalOut + 1;   // doubles per instance in the outpu
 // lane sums at most kSellWidth entries; longer rows get a whole wave, and a
 // row longer than kSellChunk entries is cut into chunk slices (nrows = 0)
 // whose partial sums the last chunk combines in chunk order.
-constexpr int kSellWidth = 32;
+constexpr int kSellWidth = 64;
 constexpr int kSellChunk = 16384;
 
 struct SliceDesc {
