@@ -47,8 +47,11 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes per launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE
 # (2 * FETCH + WRITE, factors from a plain 1 GiB copy on the same box), see
 # profiles/hbm_traffic_counters.md.  Keyed by (workload, precision, kernel)
-# and used only when this run's algorithmic bytes per launch match the
-# profiled run's (the same deterministic workload).
+# and used only when this run's algorithmic bytes per launch are within
+# 0.5 % of the profiled run's (the same workload: when a batch instance
+# leaves the launches depends on the host's round timing, so the mix moves
+# by a few hundredths of a percent between runs); the profiled ratio of
+# traffic to algorithmic bytes is then applied to this run's bytes.
 MEASURED_TRAFFIC = {
     ("batch", "fp64", "k_dual"): (69376581.66, 102.82e6),
     ("batch", "fp32", "k_dual"): (180747947.88, 264.46e6),
@@ -64,15 +67,19 @@ def log(*a):
 
 
 BATCH_GROUPS = 0  # --batch-groups
+REFINE_INNER_TOL = 0.0  # --refine-inner-tol (0: the library default)
 
 
 def make_params(tol, precision, iteration_limit=2**31 - 1, time_limit=float("inf")):
     from cupdlp import PdhgParameters, Precision, construct_termination_criteria
     tc = construct_termination_criteria(eps_optimal_absolute=tol, eps_optimal_relative=tol,
                                         iteration_limit=iteration_limit, time_sec_limit=time_limit)
-    return PdhgParameters(termination_criteria=tc, record_iteration_stats=False,
-                          precision=Precision.FP32_REFINE if precision == "fp32" else Precision.FP64,
-                          batch_groups=BATCH_GROUPS)
+    p = PdhgParameters(termination_criteria=tc, record_iteration_stats=False,
+                       precision=Precision.FP32_REFINE if precision == "fp32" else Precision.FP64,
+                       batch_groups=BATCH_GROUPS)
+    if REFINE_INNER_TOL > 0:
+        p.refine_inner_tolerance = REFINE_INNER_TOL
+    return p
 
 
 def batch_problems(count):
@@ -89,9 +96,9 @@ REFERENCE_OBJECTIVES = {2: 1219334.0106920893}  # 2,000 x 2,000 transportation L
 
 def measured_traffic(workload, precision, kernel, bytes_per_launch):
     rec = MEASURED_TRAFFIC.get((workload, precision, kernel))
-    if rec is None or abs(rec[0] - bytes_per_launch) > 1e-6 * rec[0]:
+    if rec is None or abs(rec[0] - bytes_per_launch) > 5e-3 * rec[0]:
         return None
-    return rec[1]
+    return round(rec[1] * bytes_per_launch / rec[0], -3)
 
 
 def profile_roofline(prof, kernel_names, note, workload=None, precision=None):
@@ -277,6 +284,8 @@ def serve_stream(count, nmax, seed=4):
     rng = np.random.default_rng(seed)
     probs, tols = [], []
     for k in range(count):
+        if k and k % 32 == 0:
+            log(f"  {k} of {count} requests generated")
         s = int(rng.integers(0, 2**31 - 1))
         if rng.random() < 0.9:
             n = int(np.exp(rng.uniform(np.log(1000), np.log(nmax))))
@@ -304,7 +313,11 @@ def run_serve(args, ctx, lib):
     nxt = 0
     t0 = time.time()
     ids = {}
+    last_log = t0
     while done < len(probs):
+        if time.time() - last_log > 30.0:
+            last_log = time.time()
+            log(f"  {last_log - t0:.0f} s: {nxt} submitted, {done} done")
         now = time.time() - t0
         while nxt < len(probs) and arrive[nxt] <= now:
             ids[srv.submit(probs[nxt], tols[nxt])] = nxt
@@ -418,6 +431,9 @@ def main():
     ap.add_argument("--count", type=int, default=512)
     ap.add_argument("--batch-groups", type=int, default=0,
                     help="batch: size groups run side by side (0 = the library's default)")
+    ap.add_argument("--refine-inner-tol", type=float, default=0.0,
+                    help="FP32 + refinement: tolerance of the first FP32 solve and the tightest "
+                         "of the correction solves (0 = the library's default, 1e-4)")
     ap.add_argument("--iters", type=int, default=1000)
     ap.add_argument("--serial-budget", type=float, default=20.0)
     ap.add_argument("--cpu-budget", type=float, default=20.0)
@@ -425,8 +441,9 @@ def main():
     ap.add_argument("--time-limit", type=float, default=float("inf"),
                     help="per-solve time limit (a guard; instances that hit it count as not optimal)")
     args = ap.parse_args()
-    global BATCH_GROUPS
+    global BATCH_GROUPS, REFINE_INNER_TOL
     BATCH_GROUPS = args.batch_groups
+    REFINE_INNER_TOL = args.refine_inner_tol
     if args.precision is None:
         args.precision = "fp32" if args.workload == "batch" else "fp64"
     if args.gpus != 1:

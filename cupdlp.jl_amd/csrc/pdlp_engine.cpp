@@ -1419,6 +1419,12 @@ void Engine::refine_logic(size_t q, const double *o) {
   C.ref_dd = ddn;
   if (h.cb && prm_.record_iteration_stats) h.cb(&st, h.user);
   reset_inner(q, omega0);
+  // A correction solved to relative tolerance t leaves about t times the
+  // current violations, so a point at err times the outer tolerance needs t
+  // near 1/err: ask for ten times less, never tighter than
+  // refine_inner_tolerance (nor looser than 0.1).
+  if (std::isfinite(err))
+    h.tol_cur = std::max(prm_.refine_inner_tolerance, std::min(0.1, 0.1 / err));
 }
 
 // Fresh PDHG state for the next correction LP (the device side is reset by

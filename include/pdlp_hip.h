@@ -138,7 +138,10 @@ typedef struct pdlp_params {
   /* ---- extensions ---- */
   int32_t precision;                 /* PDLP_PRECISION_* */
   int32_t refine_max_outer;          /* FP32_REFINE: max refinement rounds */
-  double refine_inner_tolerance;     /* FP32_REFINE: tolerance of each FP32 solve */
+  double refine_inner_tolerance;     /* FP32_REFINE: tolerance of the first FP32 solve and
+                                        the tightest asked of a correction solve, which
+                                        asks for max(this, min(0.1, 0.1 / e)), e = the
+                                        refined point's error over the outer tolerance */
   double refine_max_scale;           /* FP32_REFINE: cap of the per-round scale-up */
   int32_t device_preconditioning;    /* 1: Ruiz/Pock-Chambolle on the GPU */
   int32_t refine_inner_iteration_limit; /* FP32_REFINE: iteration cap of each correction

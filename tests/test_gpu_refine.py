@@ -47,7 +47,7 @@ def check_kkt(prob, out, tol):
     return ci
 
 
-@pytest.mark.parametrize("tol", [1e-4, 1e-8])
+@pytest.mark.parametrize("tol", [1e-4, 1e-6, 1e-8])
 def test_refined_c0(gpu_ctx, tol):
     say(f"case: n=2000 LP, FP32 + refinement to {tol}")
     prob, cert = config_lp(0)
