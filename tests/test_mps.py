@@ -191,6 +191,59 @@ def test_errors(tmp_path):
         qps_reader_to_standard_form(p)
 
 
+def test_damaged_files_are_read_or_refused(tmp_path):
+    """Damaged copies of the example (truncated, bytes changed, lines dropped,
+    repeated or shuffled, tokens replaced by odd ones), free and fixed
+    format: each is either read into a consistent problem or refused with
+    MpsError / ValueError, never a crash."""
+    rng = np.random.default_rng(11)
+    lines = open(EXAMPLE).read().split("\n")
+    odd = ["nan", "inf", "1e999", "+", "-", "1e", "1.2.3", "", "X" * 300, "R1", "MARKER",
+           "'INTORG'", "RANGES", "BOUNDS", "UP", "FR", "BV", "SC", "N", "E", "ENDATA", "\t"]
+    read = refused = 0
+    for k in range(150):
+        t = list(lines)
+        op = k % 5
+        if op == 0:
+            text = "\n".join(t)[: int(rng.integers(0, len("\n".join(t)) + 1))]
+        elif op == 1:
+            b = bytearray("\n".join(t).encode())
+            for _ in range(int(rng.integers(1, 6))):
+                b[int(rng.integers(len(b)))] = int(rng.integers(32, 127))
+            text = b.decode()
+        elif op == 2:
+            for _ in range(int(rng.integers(1, 4))):
+                del t[int(rng.integers(len(t)))]
+            text = "\n".join(t)
+        elif op == 3:
+            i = int(rng.integers(len(t)))
+            t.insert(i, t[i])
+            j = int(rng.integers(len(t) - 3))
+            t[j:j + 3] = t[j:j + 3][::-1]
+            text = "\n".join(t)
+        else:
+            i = int(rng.integers(len(t)))
+            w = t[i].split()
+            if w:
+                w[int(rng.integers(len(w)))] = odd[int(rng.integers(len(odd)))]
+            t[i] = " " + " ".join(w)
+            text = "\n".join(t)
+        path = os.path.join(tmp_path, f"d{k}.mps")
+        with open(path, "w") as f:
+            f.write(text)
+        for fixed in (False, True):
+            try:
+                p = qps_reader_to_standard_form(path, fixed_format=fixed)
+            except (MpsError, ValueError):
+                refused += 1
+                continue
+            read += 1
+            m, n = p.constraint_matrix.shape
+            assert p.objective_vector.size == n and p.right_hand_side.size == m
+            assert 0 <= p.num_equalities <= m
+    assert read > 0 and refused > 0
+
+
 def test_read_then_oracle_solve(tmp_path):
     """A generated LP written as MPS, read back and solved by the oracle: the
     objective is the generator's certified optimum."""
