@@ -68,6 +68,7 @@ def log(*a):
 
 BATCH_GROUPS = 0  # --batch-groups
 REFINE_INNER_TOL = 0.0  # --refine-inner-tol (0: the library default)
+REFINE_EARLY = 0  # --refine-early-fallback (0: the library default)
 
 
 def make_params(tol, precision, iteration_limit=2**31 - 1, time_limit=float("inf")):
@@ -79,6 +80,7 @@ def make_params(tol, precision, iteration_limit=2**31 - 1, time_limit=float("inf
                        batch_groups=BATCH_GROUPS)
     if REFINE_INNER_TOL > 0:
         p.refine_inner_tolerance = REFINE_INNER_TOL
+    p.refine_early_fallback = REFINE_EARLY
     return p
 
 
@@ -434,6 +436,9 @@ def main():
     ap.add_argument("--refine-inner-tol", type=float, default=0.0,
                     help="FP32 + refinement: tolerance of the first FP32 solve and the tightest "
                          "of the correction solves (0 = the library's default, 1e-4)")
+    ap.add_argument("--refine-early-fallback", type=int, default=0,
+                    help="FP32 + refinement, batches: iterations into a correction solve after "
+                         "which the FP64 fallback starts beside it (0 = never, the default)")
     ap.add_argument("--iters", type=int, default=1000)
     ap.add_argument("--serial-budget", type=float, default=20.0)
     ap.add_argument("--cpu-budget", type=float, default=20.0)
@@ -441,9 +446,10 @@ def main():
     ap.add_argument("--time-limit", type=float, default=float("inf"),
                     help="per-solve time limit (a guard; instances that hit it count as not optimal)")
     args = ap.parse_args()
-    global BATCH_GROUPS, REFINE_INNER_TOL
+    global BATCH_GROUPS, REFINE_INNER_TOL, REFINE_EARLY
     BATCH_GROUPS = args.batch_groups
     REFINE_INNER_TOL = args.refine_inner_tol
+    REFINE_EARLY = args.refine_early_fallback
     if args.precision is None:
         args.precision = "fp32" if args.workload == "batch" else "fp64"
     if args.gpus != 1:

@@ -342,6 +342,18 @@ static int stage_fp64(Engine *fb, const HostInst &h32, const LpCopy *lp, const p
   return fb->stage(std::move(g), err, &src);
 }
 
+// A plain FP64 solve of an FP32 instance started while its refinement still
+// runs (refine_early_fallback), into a result of its own.  It becomes the
+// instance's result only if that refinement then stalls and the early start
+// cannot have changed it; otherwise it is cancelled.
+struct EarlyFp64 {
+  int32_t k = -1;      // index in the group's FP64 engine (-1: none)
+  int32_t round = 0;   // the correction round it started in
+  bool committed = false;
+  pdlp_result res{};
+  std::vector<double> x, y;
+};
+
 // One group of a batch: its instances packed into one engine's launches on
 // its own stream, with its own FP64 engine for stalled refinements.
 struct BatchGroup {
@@ -470,6 +482,8 @@ static int run_group(pdlp_batch *b, BatchGroup &grp, pdlp_result *results, pdlp_
   std::vector<Prior> prior(count);
   std::vector<RefinedPoint> refined(count);
   std::vector<char> moved(count, 0);
+  std::vector<EarlyFp64> early(count);
+  const bool early_on = params->refine_early_fallback > 0 && !cb;
   bool main_done = false;
   for (;;) {
     const int r1 = main_done ? 1 : main.begin_round(err);
@@ -488,11 +502,63 @@ static int run_group(pdlp_batch *b, BatchGroup &grp, pdlp_result *results, pdlp_
     if (r1 == 1) main_done = true;
     for (int32_t q = 0; q < count; ++q) {
       const HostInst &h = main.inst((size_t)q);
+      pdlp_result *r = &results[grp.idx[q]];
+      EarlyFp64 &e = early[q];
+      const double now = now_sec();
+      // an early FP64 solve whose FP32 instance has left its round: keep it
+      // if the refinement stalled and the solve is the one a start at the
+      // stall would make, else cancel it (once it is active)
+      if (e.k >= 0 && !e.committed && (h.done || h.ref_round != e.round)) {
+        HostInst &g = fb->inst((size_t)e.k);
+        const int64_t left_it = (int64_t)p64.termination_criteria.iteration_limit - r->iteration_count;
+        const double left_t = p64.termination_criteria.time_sec_limit - (now - t0);
+        // with an iteration limit of left_it a solve evaluates at the same
+        // iterations up to left_it + 1, so it is the same solve as long as
+        // its next (or final) evaluation is not past that point
+        if (h.done && h.stalled && !moved[q] && left_it > 0 && left_t > 0 &&
+            g.eval_iter <= left_it + 1) {
+          g.iteration_limit_override = left_it;
+          g.time_limit_override = (g.staged ? 0.0 : now - g.start_time) + left_t;
+          e.committed = true;
+          moved[q] = 1;
+          prior[q] = {r->iteration_count, r->total_attempts, r->refinement_rounds};
+          refined[q].save(*r, h.P.n, h.P.m);
+          r->fp64_fallback = 2;
+        } else if (!g.staged) {
+          rc = fb->cancel((size_t)e.k, err);
+          if (rc) return rc;
+          e.k = -1;
+        }
+      }
+      if (early_on && e.k < 0 && !h.done && h.ref_round >= 1 && !r->x_current && !r->y_current) {
+        const int64_t at = params->refine_early_fallback;
+        const int64_t sofar = h.iters_total + h.iter_start - 1;
+        const int64_t left_it = (int64_t)p64.termination_criteria.iteration_limit - sofar;
+        const double left_t = p64.termination_criteria.time_sec_limit - (now - t0);
+        if (h.inner_cap > at && h.iter_start - 1 >= at && left_it > 0 && left_t > 0) {
+          if (!fb) fb = std::make_unique<Engine>(s_fb, &ctx->pool, p64);
+          e.x.assign((size_t)std::max<int64_t>(h.P.n, 1), 0.0);
+          e.y.assign((size_t)std::max<int64_t>(h.P.m, 1), 0.0);
+          e.res = pdlp_result{};
+          e.res.x = e.x.data();
+          e.res.y = e.y.data();
+          auto g = std::make_unique<HostInst>();
+          g->res = &e.res;
+          g->t_submit = t0;
+          g->iteration_limit_override = left_it;
+          g->time_limit_override = left_t;
+          const int32_t k = (int32_t)fb->count();
+          rc = stage_fp64(fb.get(), h, grp.lps.empty() ? nullptr : grp.lps[q].get(), p64,
+                          std::move(g), err);
+          if (rc) return rc;
+          e.k = k;
+          e.round = h.ref_round;
+        }
+      }
       if (!h.done || !h.stalled || moved[q]) continue;
       moved[q] = 1;
-      pdlp_result *r = &results[grp.idx[q]];
       const int64_t left_it = (int64_t)p64.termination_criteria.iteration_limit - r->iteration_count;
-      const double left_t = p64.termination_criteria.time_sec_limit - (now_sec() - t0);
+      const double left_t = p64.termination_criteria.time_sec_limit - (now - t0);
       if (left_it <= 0 || !(left_t > 0)) continue;
       prior[q] = {r->iteration_count, r->total_attempts, r->refinement_rounds};
       refined[q].save(*r, h.P.n, h.P.m);
@@ -509,10 +575,25 @@ static int run_group(pdlp_batch *b, BatchGroup &grp, pdlp_result *results, pdlp_
       if (rc) return rc;
       r->fp64_fallback = 2;  // being solved in FP64
     }
-    if (main_done && (!fb || fb->all_done())) break;
+    bool pending = false;  // early solves still to be cancelled
+    for (const EarlyFp64 &e : early) pending = pending || (e.k >= 0 && !e.committed);
+    if (main_done && !pending && (!fb || fb->all_done())) break;
   }
   for (int32_t q = 0; q < count; ++q) {
     pdlp_result *r = &results[grp.idx[q]];
+    const EarlyFp64 &e = early[q];
+    if (e.committed) {
+      // the fields a plain FP64 solve writes into its result
+      const Prepared &P = main.inst((size_t)q).P;
+      if (P.n) memcpy(r->x, e.x.data(), 8 * (size_t)P.n);
+      if (P.m) memcpy(r->y, e.y.data(), 8 * (size_t)P.m);
+      r->termination_reason = e.res.termination_reason;
+      r->solve_time_sec = e.res.solve_time_sec;
+      r->iteration_count = e.res.iteration_count;
+      r->final_stats = e.res.final_stats;
+      r->total_attempts = e.res.total_attempts;
+      r->refinement_rounds = e.res.refinement_rounds;
+    }
     if (r->fp64_fallback != 2) continue;
     const int64_t it = (int64_t)r->iteration_count + prior[q].iters;
     r->iteration_count = (int32_t)std::min<int64_t>(it, INT32_MAX);

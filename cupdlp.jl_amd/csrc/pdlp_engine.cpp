@@ -1517,6 +1517,25 @@ int Engine::run(std::string *err) {
   }
 }
 
+int Engine::cancel(size_t q, std::string *err) {
+  HostInst &h = *inst_[q];
+  if (h.done) return 0;
+  if (h.staged) {
+    if (err) *err = "cancel of a staged instance";
+    return -1;
+  }
+  h.done = true;
+  h.needs_eval = false;
+  ctl_[q].active = 0;
+  // Only this instance's block is copied: instances staged since the last
+  // table rebuild have no device block yet (the rebuild at the next round
+  // copies them all), and an active instance's block exists.
+  HIPCHK(hipStreamSynchronize(s_));
+  HIPCHK(hipMemcpyAsync(d_ctl_ + q, &ctl_[q], sizeof(Ctl), hipMemcpyHostToDevice, s_));
+  HIPCHK(hipStreamSynchronize(s_));
+  return 0;
+}
+
 void Engine::remove_done() {
   std::vector<std::unique_ptr<HostInst>> keep;
   std::vector<Ctl> kc;

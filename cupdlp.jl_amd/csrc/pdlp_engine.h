@@ -143,6 +143,9 @@ class Engine {
   size_t count() const { return inst_.size(); }
   HostInst &inst(size_t i) { return *inst_[i]; }
   void remove_done();  // drops finished instances (serving)
+  // Ends active (not staged) instance q between rounds without writing its
+  // result.
+  int cancel(size_t q, std::string *err);
 
  private:
   int transfer(HostInst &h, const DevSource *src, std::string *err);

@@ -73,7 +73,8 @@ class pdlp_params(C.Structure):
         ("device_preconditioning", C.c_int32),
         ("refine_inner_iteration_limit", C.c_int32),
         ("batch_groups", C.c_int32),
-        ("reserved", C.c_int32 * 5),
+        ("refine_early_fallback", C.c_int32),
+        ("reserved", C.c_int32 * 4),
     ]
 
 

@@ -93,6 +93,7 @@ def to_c_params(p: PdhgParameters) -> _abi.pdlp_params:
     cp.device_preconditioning = int(bool(p.device_preconditioning))
     cp.refine_inner_iteration_limit = int(p.refine_inner_iteration_limit)
     cp.batch_groups = int(p.batch_groups)
+    cp.refine_early_fallback = int(p.refine_early_fallback)
     return cp
 
 

@@ -220,6 +220,7 @@ class PdhgParameters:
     device_preconditioning: bool = True
     refine_inner_iteration_limit: int = 0
     batch_groups: int = 0
+    refine_early_fallback: int = 0
 
 
 @dataclass

@@ -145,7 +145,8 @@ struct CParams
     device_preconditioning::Int32
     refine_inner_iteration_limit::Int32
     batch_groups::Int32
-    reserved::NTuple{5,Int32}
+    refine_early_fallback::Int32
+    reserved::NTuple{4,Int32}
 end
 
 struct CConvergence
@@ -233,7 +234,7 @@ function _c_params(p::PdhgParameters)
                    adaptive ? p.step_size_policy_params.growth_exponent : 0.6,
                    ct, cr,
                    0,        # PDLP_PRECISION_FP64: the reference's arithmetic
-                   12, 1e-4, 1e6, 1, 0, 0, ntuple(_ -> Int32(0), 5))
+                   12, 1e-4, 1e6, 1, 0, 0, 0, ntuple(_ -> Int32(0), 4))
 end
 
 function _stats(s::CIterationStats)

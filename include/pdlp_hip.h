@@ -149,7 +149,14 @@ typedef struct pdlp_params {
   int32_t batch_groups;              /* batched solves: size groups run side by side, each
                                         with its own launches and streams; 0 = automatic
                                         (3 from 24 instances up), 1 = one group */
-  int32_t reserved[5];
+  int32_t refine_early_fallback;     /* FP32_REFINE, pdlp_batch_solve / pdlp_batch_run:
+                                        iterations into a correction solve after which
+                                        the instance's plain FP64 solve starts beside it,
+                                        used only if the refinement then stalls (results
+                                        are those of starting it at the stall); <= 0 =
+                                        never (the default: the extra FP64 work costs the
+                                        512-LP batch more than it saves) */
+  int32_t reserved[4];
 } pdlp_params;
 
 /* ConvergenceInformation — src/solve_log.jl:51-155 (same field order). */

@@ -192,6 +192,38 @@ def test_batch_groups_bitwise(gpu_ctx):
             assert np.array_equal(o.dual_solution, r.dual_solution)
 
 
+def test_early_fallback_bitwise(gpu_ctx):
+    """FP64 solves started early (refine_early_fallback: 5 iterations into a
+    correction solve capped at 256) return bitwise what starting them at the
+    stall returns, and are dropped for instances that refine after all; also
+    under an iteration limit of 400 with a cap of 64, where an early solve
+    that has run past what is left of the limit is replaced by a fresh one."""
+    say("case: 6 LPs, FP32 + refinement, early FP64 fallback vs none (caps 256 and 64)")
+    probs = [config_lp(0)[0], transportation_lp(40, 50, seed=6), lp_cases.tiny_two_var()[0],
+             random_lp(n=3000, m=1500, seed=3)[0], random_lp(n=800, m=400, seed=4)[0],
+             lp_cases.long_col_lp()[0]]
+    cases = [(params(1e-8, refine_inner_iteration_limit=256), None),
+             (params(1e-8, refine_inner_iteration_limit=64, refine_max_outer=6), 400)]
+    for p, limit in cases:
+        if limit:
+            p.termination_criteria.iteration_limit = limit
+        p.refine_early_fallback = 0
+        ref = gpu_ctx.batch_solve(probs, p)
+        p.refine_early_fallback = 5
+        out = gpu_ctx.batch_solve(probs, p)
+        say(f"  cap {p.refine_inner_iteration_limit}: fallbacks {[o.fp64_fallback for o in out]}, "
+            f"rounds {[o.refinement_rounds for o in out]}")
+        for o, r in zip(out, ref):
+            assert o.termination_reason == r.termination_reason
+            assert o.iteration_count == r.iteration_count
+            assert o.total_attempts == r.total_attempts
+            assert o.refinement_rounds == r.refinement_rounds
+            assert o.fp64_fallback == r.fp64_fallback
+            assert np.array_equal(o.primal_solution, r.primal_solution)
+            assert np.array_equal(o.dual_solution, r.dual_solution)
+        assert any(o.fp64_fallback for o in out)
+
+
 def test_refined_small_cases(gpu_ctx):
     say("case: tiny LPs, FP32 + refinement")
     prob, opt = lp_cases.tiny_two_var()
