@@ -350,6 +350,7 @@ struct EarlyFp64 {
   int32_t k = -1;      // index in the group's FP64 engine (-1: none)
   int32_t round = 0;   // the correction round it started in
   bool committed = false;
+  double commit_sec = 0.0;  // time from the call's start to the stall
   pdlp_result res{};
   std::vector<double> x, y;
 };
@@ -520,6 +521,7 @@ static int run_group(pdlp_batch *b, BatchGroup &grp, pdlp_result *results, pdlp_
           g.iteration_limit_override = left_it;
           g.time_limit_override = (g.staged ? 0.0 : now - g.start_time) + left_t;
           e.committed = true;
+          e.commit_sec = now - t0;
           moved[q] = 1;
           prior[q] = {r->iteration_count, r->total_attempts, r->refinement_rounds};
           refined[q].save(*r, h.P.n, h.P.m);
@@ -588,7 +590,8 @@ static int run_group(pdlp_batch *b, BatchGroup &grp, pdlp_result *results, pdlp_
       if (P.n) memcpy(r->x, e.x.data(), 8 * (size_t)P.n);
       if (P.m) memcpy(r->y, e.y.data(), 8 * (size_t)P.m);
       r->termination_reason = e.res.termination_reason;
-      r->solve_time_sec = e.res.solve_time_sec;
+      // the result exists once the refinement has stalled
+      r->solve_time_sec = std::max(e.res.solve_time_sec, e.commit_sec);
       r->iteration_count = e.res.iteration_count;
       r->final_stats = e.res.final_stats;
       r->total_attempts = e.res.total_attempts;
