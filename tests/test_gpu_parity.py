@@ -6,8 +6,8 @@ once, i.e. the FP64 value nearest the exact sum, so summation order does not
 matter; every elementwise formula is evaluated in the same written order
 without contraction.  Step sizes, primal weights, restart decisions, every
 recorded KKT statistic and the returned iterates must therefore be equal.
-(The north-star tolerance is 1e-9 relative over the first 1,000 iterations;
-bitwise equality is stronger.)  The one documented exception would be an
+(Bitwise equality is stronger than a 1e-9 relative bar over the first 1,000
+iterations.)  The one documented exception would be an
 exact sum within ~1e-30 relative of an FP64 rounding midpoint.
 """
 import math
