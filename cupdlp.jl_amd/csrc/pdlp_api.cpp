@@ -4,6 +4,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <atomic>
 #include <deque>
 #include <map>
 #include <memory>
@@ -421,11 +422,32 @@ static int load_batch(pdlp_ctx *ctx, int32_t count, const pdlp_lp *lps, const pd
     std::sort(grp->idx.begin(), grp->idx.end());
     b->groups.push_back(std::move(grp));
   }
+  // Host preparation (validation, numbering, slice tables) of the instances
+  // on up to 16 host threads; then each is copied to the device in group
+  // order on its group's stream.
+  std::vector<std::unique_ptr<HostInst>> prepared(count);
+  std::vector<std::string> perr(count);
+  std::vector<int> prc(count, 0);
+  std::atomic<int32_t> next{0};
+  auto work = [&]() {
+    for (int32_t i = next.fetch_add(1); i < count; i = next.fetch_add(1)) {
+      auto h = std::make_unique<HostInst>();
+      prc[i] = prepare_instance(&lps[i], params, h.get(), &perr[i]);
+      prepared[i] = std::move(h);
+    }
+  };
+  const int nth = std::max(1, std::min({(int)count, 16, (int)std::thread::hardware_concurrency()}));
+  {
+    std::vector<std::thread> threads;
+    for (int t = 1; t < nth; ++t) threads.emplace_back(work);
+    work();
+    for (auto &t : threads) t.join();
+  }
   for (auto &grp : b->groups) {
     for (int32_t i : grp->idx) {
-      auto h = std::make_unique<HostInst>();
-      int rc = prepare_instance(&lps[i], params, h.get(), err);
-      if (rc == 0) rc = grp->main->stage(std::move(h), err);
+      int rc = prc[i];
+      if (rc) *err = perr[i];
+      else rc = grp->main->stage(std::move(prepared[i]), err);
       if (rc) {
         if (count > 1) *err = "instance " + std::to_string(i) + ": " + *err;
         return rc;
