@@ -50,6 +50,10 @@ def lib():
                                C.POINTER(C.c_double), C.POINTER(C.c_double),
                                C.POINTER(C.c_double)]
         L.orc_spmv.restype = C.c_int
+        L.orc_restart_choice.argtypes = [C.POINTER(_abi.pdlp_restart_params), C.c_int64,
+                                         C.c_int64, C.POINTER(C.c_double),
+                                         C.POINTER(C.c_double), C.POINTER(C.c_int32)]
+        L.orc_restart_choice.restype = C.c_int32
         L.orc_num_threads.restype = C.c_int
         L.orc_set_exact.argtypes = [C.c_int]
         L.orc_set_exact.restype = None
@@ -113,6 +117,26 @@ def step_rule(eta, interaction, movement, attempt, red=0.3, grow=0.6):
     acc, ne = C.c_int(), C.c_int()
     v = L.orc_step_rule(eta, interaction, movement, attempt, red, grow, C.byref(acc), C.byref(ne))
     return v, bool(acc.value), bool(ne.value)
+
+
+def restart_choice(rp, restart_length, iteration, cur_kkt, avg_kkt, last_kkt, last_trial):
+    """The oracle's restart decision on given weighted KKT errors: returns
+    (RestartChoice, updated last_trial, set of the errors consulted, of
+    "cur", "avg", "last")."""
+    L = lib()
+    crp = to_c_params_restart(rp)
+    kkt = (C.c_double * 3)(cur_kkt, avg_kkt, last_kkt)
+    lt = C.c_double(last_trial)
+    asked = C.c_int32()
+    r = L.orc_restart_choice(C.byref(crp), restart_length, iteration, kkt, C.byref(lt),
+                             C.byref(asked))
+    names = {n for i, n in enumerate(("cur", "avg", "last")) if asked.value >> i & 1}
+    return int(r), lt.value, names
+
+
+def to_c_params_restart(rp):
+    from cupdlp import PdhgParameters
+    return to_c_params(PdhgParameters(restart_params=rp)).restart_params
 
 
 def spmv(problem, x=None, y=None):

@@ -649,6 +649,87 @@ static double weighted_kkt(const oprob *S, const double *x, const double *y, con
   return sqrt(omega * (l2p * l2p) + 1.0 / omega * (l2d * l2d) + gap * gap);
 }
 
+/* The decision of run_restart_scheme — src/saddle_point_gpu.jl:479-622:
+ * the artificial restart (:510-512), the candidate chosen by
+ * restart_to_current_metric (should_reset_to_average :419-440), and the
+ * adaptive KKT test (should_do_adaptive_restart_kkt :442-474) or the fixed
+ * frequency.  kkt(ctx, 0 | 1 | 2) is the weighted KKT error of the current,
+ * average or last-restart point, asked for only where the reference computes
+ * it.  Returns the RestartChoice; *last_trial is updated on the adaptive
+ * branch, *reset_to_average says which point a restart starts from. */
+static int32_t restart_choice(const pdlp_restart_params *rp, int64_t restart_length,
+                              int64_t iteration, double (*kkt)(void *, int), void *ctx,
+                              double *last_trial, int *reset_to_average_out) {
+  *reset_to_average_out = 0;
+  if (!(restart_length > 0)) return PDLP_RESTART_CHOICE_NO_RESTART;
+  int do_restart = 0;
+  if ((double)restart_length >= rp->artificial_restart_threshold * (double)(iteration - 1))
+    do_restart = 1;
+  int reset_to_average = 0;
+  double cand = NAN;
+  if (rp->restart_scheme != PDLP_NO_RESTARTS) {
+    double cur_kkt = kkt(ctx, 0);
+    double avg_kkt = kkt(ctx, 1);
+    reset_to_average = rp->restart_to_current_metric == PDLP_KKT_GREEDY ? (cur_kkt >= avg_kkt) : 1;
+    cand = reset_to_average ? avg_kkt : cur_kkt;
+  }
+  if (!do_restart) {
+    if (rp->restart_scheme == PDLP_ADAPTIVE_KKT) {
+      double ratio = cand / kkt(ctx, 2);
+      if (ratio < rp->necessary_reduction_for_restart) {
+        if (ratio < rp->sufficient_reduction_for_restart)
+          do_restart = 1;
+        else if (ratio > *last_trial)
+          do_restart = 1;
+      }
+      *last_trial = ratio;
+    } else if (rp->restart_scheme == PDLP_FIXED_FREQUENCY &&
+               rp->restart_frequency_if_fixed <= restart_length) {
+      do_restart = 1;
+    }
+  }
+  *reset_to_average_out = reset_to_average;
+  if (!do_restart) return PDLP_RESTART_CHOICE_NO_RESTART;
+  return reset_to_average ? PDLP_RESTART_CHOICE_RESTART_TO_AVERAGE
+                          : PDLP_RESTART_CHOICE_WEIGHTED_AVERAGE_RESET;
+}
+
+/* The three weighted KKT errors of the solve's points, for restart_choice. */
+typedef struct {
+  const oprob *S;
+  kkt_buf *K;
+  double omega;
+  const double *x[3], *y[3], *Ax[3], *g[3];
+} kkt_points;
+
+static double kkt_point(void *p, int which) {
+  kkt_points *k = (kkt_points *)p;
+  return weighted_kkt(k->S, k->x[which], k->y[which], k->Ax[which], k->g[which], k->K, k->omega);
+}
+
+/* restart_choice on given errors (tests): kkt3 = current, average,
+ * last-restart; *asked gets bit i set when error i was consulted. */
+typedef struct {
+  const double *v;
+  int32_t asked;
+} kkt_table;
+
+static double kkt_lookup(void *p, int which) {
+  kkt_table *t = (kkt_table *)p;
+  t->asked |= 1 << which;
+  return t->v[which];
+}
+
+int32_t orc_restart_choice(const pdlp_restart_params *rp, int64_t restart_length,
+                           int64_t iteration, const double *kkt3, double *last_trial,
+                           int32_t *asked) {
+  kkt_table t = {kkt3, 0};
+  int reset = 0;
+  int32_t r = restart_choice(rp, restart_length, iteration, kkt_lookup, &t, last_trial, &reset);
+  if (asked) *asked = t.asked;
+  return r;
+}
+
 /* optimality_criteria_met etc. — src/termination.jl:142-246. */
 static int check_termination(const pdlp_termination_criteria *tc, const qp_cache_t *qc,
                              const pdlp_iteration_stats *st) {
@@ -1013,44 +1094,13 @@ int orc_solve(const pdlp_lp *lp, const pdlp_params *prm, pdlp_result *res, pdlp_
       for (int64_t j = 0; j < n; ++j) gbuf[j] = S.c[j] - st.ATy[j];
 
       /* run_restart_scheme — src/saddle_point_gpu.jl:479-622 */
-      int32_t restart_used;
-      if (!(st.count > 0)) {
-        restart_used = PDLP_RESTART_CHOICE_NO_RESTART;
-      } else {
-        int64_t restart_length = st.count;
-        int do_restart = 0;
-        if ((double)restart_length >= rp->artificial_restart_threshold * (double)(iteration - 1))
-          do_restart = 1;
-        int reset_to_average = 0;
-        double cand = NAN;
-        if (rp->restart_scheme == PDLP_NO_RESTARTS) {
-          reset_to_average = 0;
-        } else {
-          double cur_kkt = weighted_kkt(&S, st.x, st.y, st.Ax, gbuf, &K, st.primal_weight);
-          double avg_kkt = weighted_kkt(&S, avg_x, avg_y, avg_Ax, avg_g, &K, st.primal_weight);
-          reset_to_average =
-              rp->restart_to_current_metric == PDLP_KKT_GREEDY ? (cur_kkt >= avg_kkt) : 1;
-          cand = reset_to_average ? avg_kkt : cur_kkt;
-        }
-        if (!do_restart) {
-          if (rp->restart_scheme == PDLP_ADAPTIVE_KKT) {
-            double last_kkt = weighted_kkt(&S, lx, ly, lAx, lg, &K, st.primal_weight);
-            double ratio = cand / last_kkt;
-            if (ratio < rp->necessary_reduction_for_restart) {
-              if (ratio < rp->sufficient_reduction_for_restart)
-                do_restart = 1;
-              else if (ratio > last_trial)
-                do_restart = 1;
-            }
-            last_trial = ratio;
-          } else if (rp->restart_scheme == PDLP_FIXED_FREQUENCY &&
-                     rp->restart_frequency_if_fixed <= restart_length) {
-            do_restart = 1;
-          }
-        }
-        if (!do_restart) {
-          restart_used = PDLP_RESTART_CHOICE_NO_RESTART;
-        } else {
+      kkt_points kp = {&S, &K, st.primal_weight,
+                       {st.x, avg_x, lx}, {st.y, avg_y, ly}, {st.Ax, avg_Ax, lAx}, {gbuf, avg_g, lg}};
+      int reset_to_average = 0;
+      const int32_t restart_used = restart_choice(rp, st.count, iteration, kkt_point, &kp,
+                                                  &last_trial, &reset_to_average);
+      {
+        if (restart_used != PDLP_RESTART_CHOICE_NO_RESTART) {
           if (reset_to_average) {
             memcpy(st.x, avg_x, sizeof(double) * n);
             memcpy(st.y, avg_y, sizeof(double) * m);
@@ -1076,8 +1126,6 @@ int orc_solve(const pdlp_lp *lp, const pdlp_params *prm, pdlp_result *res, pdlp_
           memcpy(ly, st.y, sizeof(double) * m);
           memcpy(lAx, st.Ax, sizeof(double) * m);
           memcpy(lg, gbuf, sizeof(double) * n);
-          restart_used = reset_to_average ? PDLP_RESTART_CHOICE_RESTART_TO_AVERAGE
-                                          : PDLP_RESTART_CHOICE_WEIGHTED_AVERAGE_RESET;
         }
       }
       stats.restart_used = restart_used;

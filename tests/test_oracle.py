@@ -120,6 +120,46 @@ def test_restart_schedule_early_artificial_restarts():
     assert used[3] == RestartChoice.RESTART_CHOICE_NO_RESTART   # 1 < 0.36 * 3
 
 
+def test_restart_decision_branches():
+    """run_restart_scheme's decision (src/saddle_point_gpu.jl:419-622) on
+    scripted weighted-KKT errors, one case per branch, expected values read
+    off the reference: the artificial restart (restart length at least 0.36
+    times the iterations completed, `iteration` - 1), KKT_GREEDY (restart
+    to the average iff current >= average) against NO_RESTART_TO_CURRENT
+    (always the average), the adaptive test (ratio below 0.2; below 0.8
+    and above the last trial; the last trial always updated), the fixed
+    frequency and NO_RESTARTS; also which errors the reference computes."""
+    NO, RESET, AVG = (RestartChoice.RESTART_CHOICE_NO_RESTART,
+                      RestartChoice.RESTART_CHOICE_WEIGHTED_AVERAGE_RESET,
+                      RestartChoice.RESTART_CHOICE_RESTART_TO_AVERAGE)
+
+    def rp(scheme=2, metric=1):
+        return construct_restart_parameters(scheme, metric, 1000, 0.36, 0.2, 0.8, 0.5)
+
+    # (params, restart length, iteration, cur, avg, last, last trial) ->
+    # (choice, last trial after, errors consulted)
+    cases = [
+        (rp(), 0, 100, 1.0, 0.5, 1.0, 0.7, NO, 0.7, set()),
+        (rp(), 50, 100, 1.0, 0.5, 1.0, 0.7, AVG, 0.7, {"cur", "avg"}),    # 50 >= 35.64
+        (rp(), 50, 100, 0.4, 0.5, 1.0, 0.7, RESET, 0.7, {"cur", "avg"}),
+        (rp(), 36, 101, 0.4, 0.5, 1.0, 0.7, RESET, 0.7, {"cur", "avg"}),  # 36 >= 36.0
+        (rp(), 35, 99, 1.0, 0.5, 1.0, 0.7, NO, 0.5, {"cur", "avg", "last"}),  # 35 < 35.28; 0.5 < 0.7
+        (rp(), 10, 100, 1.0, 0.1, 1.0, 0.7, AVG, 0.1, {"cur", "avg", "last"}),   # 0.1 < 0.2
+        (rp(), 10, 100, 1.0, 0.5, 1.0, 0.4, AVG, 0.5, {"cur", "avg", "last"}),   # 0.5 > 0.4
+        (rp(), 10, 100, 1.0, 0.5, 1.0, 0.9, NO, 0.5, {"cur", "avg", "last"}),    # 0.5 < 0.9
+        (rp(), 10, 100, 1.0, 0.9, 1.0, 0.1, NO, 0.9, {"cur", "avg", "last"}),    # 0.9 >= 0.8
+        (rp(), 10, 100, 0.3, 0.5, 2.0, 0.1, RESET, 0.15, {"cur", "avg", "last"}),  # current: 0.15
+        (rp(metric=0), 10, 100, 0.1, 0.5, 1.0, 0.4, AVG, 0.5, {"cur", "avg", "last"}),
+        (rp(scheme=1), 1000, 10000, 1.0, 2.0, 1.0, 0.7, RESET, 0.7, {"cur", "avg"}),
+        (rp(scheme=1), 999, 10000, 1.0, 0.5, 1.0, 0.7, NO, 0.7, {"cur", "avg"}),
+        (rp(scheme=0), 10, 100, 1.0, 0.5, 1.0, 0.7, NO, 0.7, set()),
+        (rp(scheme=0), 50, 100, 1.0, 0.5, 1.0, 0.7, RESET, 0.7, set()),
+    ]
+    for k, (p, length, it, cur, avg, last, trial, want, trial_after, asked) in enumerate(cases):
+        got, t, a = O.restart_choice(p, length, it, cur, avg, last, trial)
+        assert (got, t, a) == (want, trial_after, asked), (k, got, t, a)
+
+
 def test_fixed_frequency_and_no_restarts_run():
     prob, _ = config_lp(0, scale=0.25)
     rp = construct_restart_parameters(1, 1, 64, 1.0, 0.2, 0.8, 0.5)
