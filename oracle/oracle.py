@@ -54,6 +54,8 @@ def lib():
                                          C.c_int64, C.POINTER(C.c_double),
                                          C.POINTER(C.c_double), C.POINTER(C.c_int32)]
         L.orc_restart_choice.restype = C.c_int32
+        L.orc_primal_weight.argtypes = [C.c_double] * 4
+        L.orc_primal_weight.restype = C.c_double
         L.orc_num_threads.restype = C.c_int
         L.orc_set_exact.argtypes = [C.c_int]
         L.orc_set_exact.restype = None
@@ -132,6 +134,11 @@ def restart_choice(rp, restart_length, iteration, cur_kkt, avg_kkt, last_kkt, la
                              C.byref(asked))
     names = {n for i, n in enumerate(("cur", "avg", "last")) if asked.value >> i & 1}
     return int(r), lt.value, names
+
+
+def primal_weight(primal_distance, dual_distance, omega, smoothing):
+    """The oracle's primal-weight update after a restart."""
+    return lib().orc_primal_weight(primal_distance, dual_distance, omega, smoothing)
 
 
 def to_c_params_restart(rp):

@@ -694,6 +694,19 @@ static int32_t restart_choice(const pdlp_restart_params *rp, int64_t restart_len
                           : PDLP_RESTART_CHOICE_WEIGHTED_AVERAGE_RESET;
 }
 
+/* compute_new_primal_weight — src/saddle_point_gpu.jl:627-654: the
+ * log-space moving average of the primal weight toward the ratio of the dual
+ * and primal distances moved in the last restart period; unchanged when
+ * either distance is at most eps(). */
+double orc_primal_weight(double primal_distance, double dual_distance, double omega,
+                         double smoothing) {
+  const double eps = 2.220446049250313e-16;
+  if (!(primal_distance > eps && dual_distance > eps)) return omega;
+  const double est = dual_distance / primal_distance;
+  const double lw = smoothing * log(est) + (1.0 - smoothing) * log(omega);
+  return exp(lw);
+}
+
 /* The three weighted KKT errors of the solve's points, for restart_choice. */
 typedef struct {
   const oprob *S;
@@ -1130,13 +1143,7 @@ int orc_solve(const pdlp_lp *lp, const pdlp_params *prm, pdlp_result *res, pdlp_
       }
       stats.restart_used = restart_used;
       if (restart_used != PDLP_RESTART_CHOICE_NO_RESTART) {
-        /* compute_new_primal_weight — src/saddle_point_gpu.jl:627-654 */
-        const double eps = 2.220446049250313e-16;
-        if (last_pdist > eps && last_ddist > eps) {
-          double est = last_ddist / last_pdist;
-          double lw = smoothing * log(est) + (1.0 - smoothing) * log(st.primal_weight);
-          st.primal_weight = exp(lw);
-        }
+        st.primal_weight = orc_primal_weight(last_pdist, last_ddist, st.primal_weight, smoothing);
       }
       if (cb && prm->record_iteration_stats) cb(&stats, user);
     }

@@ -160,6 +160,20 @@ def test_restart_decision_branches():
         assert (got, t, a) == (want, trial_after, asked), (k, got, t, a)
 
 
+def test_primal_weight_update():
+    """compute_new_primal_weight (src/saddle_point_gpu.jl:627-654): a
+    log-space average of the old weight and dual / primal distance with
+    weight `smoothing`; no change unless both distances exceed eps()."""
+    eps = 2.220446049250313e-16
+    assert math.isclose(O.primal_weight(1.0, 4.0, 1.0, 0.5), 2.0, rel_tol=1e-15)
+    assert math.isclose(O.primal_weight(2.0, 6.0, 5.0, 1.0), 3.0, rel_tol=1e-15)
+    assert math.isclose(O.primal_weight(2.0, 6.0, 5.0, 0.0), 5.0, rel_tol=1e-15)
+    assert math.isclose(O.primal_weight(1.0, 100.0, 0.01, 0.5), 1.0, rel_tol=1e-15)
+    assert math.isclose(O.primal_weight(2.3e-16, 2.3e-16, 4.0, 0.5), 2.0, rel_tol=1e-15)
+    for p, d in ((1e-17, 1.0), (1.0, 0.0), (eps, 1.0), (1.0, eps)):
+        assert O.primal_weight(p, d, 3.7, 0.5) == 3.7
+
+
 def test_fixed_frequency_and_no_restarts_run():
     prob, _ = config_lp(0, scale=0.25)
     rp = construct_restart_parameters(1, 1, 64, 1.0, 0.2, 0.8, 0.5)
