@@ -883,10 +883,19 @@ extern "C" int pdlp_serve_poll(pdlp_server *s, int64_t *ids, int32_t max_ids, in
     d->tol = r.tol;
     d->t_submit = r.inst->t_submit;
     r.inst->res = &d->res;
+    // counted as running only once the engine holds it; a request the
+    // engine cannot take is finished with reason OTHER (its result says so)
+    // and the error is returned
+    int rc = s->eng->add(std::move(r.inst), err);
+    if (rc) {
+      d->res.termination_reason = PDLP_TERMINATION_REASON_OTHER;
+      d->res.solve_time_sec = now_sec() - d->t_submit;
+      s->done[r.id] = std::move(d);
+      s->finished.push_back(r.id);
+      return rc;
+    }
     s->in_use += (double)r.bytes;
     s->running[r.id] = std::move(d);
-    int rc = s->eng->add(std::move(r.inst), err);
-    if (rc) return rc;
   }
   const bool refine = s->prm.precision == PDLP_PRECISION_FP32_REFINE;
   if (s->eng->count() > 0) {

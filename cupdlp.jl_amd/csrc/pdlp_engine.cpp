@@ -543,7 +543,16 @@ int Engine::stage(std::unique_ptr<HostInst> h, std::string *err, const DevSource
 int Engine::add(std::unique_ptr<HostInst> h, std::string *err) {
   int rc = stage(std::move(h), err);
   if (rc) return rc;
-  return activate(inst_.size() - 1, err);
+  rc = activate(inst_.size() - 1, err);
+  if (rc) {  // the engine does not keep an instance it could not start
+    (void)hipStreamSynchronize(s_);
+    HostInst &b = *inst_.back();
+    for (void *p : b.stage.bufs) pool_->put(p);
+    if (b.arena) pool_->put(b.arena);
+    inst_.pop_back();
+    ctl_.pop_back();
+  }
+  return rc;
 }
 
 int Engine::activate_staged(std::string *err) {
@@ -853,7 +862,8 @@ int Engine::launch_attempts(int64_t count, std::string *err) {
     if (!e)
       e = launch_step(s_, (int)inst_.size(), d_insts_, d_ctl_, d_ppart_, d_dpart_, d_red_, d_grow_,
                       tab_len_, prm_.reduction_exponent, prm_.growth_exponent,
-                      prm_.step_size_policy == PDLP_STEP_CONSTANT ? 1 : 0);
+                      prm_.step_size_policy == PDLP_STEP_CONSTANT ? 1 : 0,
+                      prm_.debug_numerical_error_attempt);
     if (e) {
       if (err) *err = std::string("kernel launch failed: ") + hipGetErrorString((hipError_t)e);
       return -e;
@@ -964,6 +974,12 @@ void Engine::host_eval_logic(size_t q, const double *o) {
   if (h.iteration_limit_override >= 0)
     tc.iteration_limit = (int32_t)std::min<int64_t>(tc.iteration_limit, h.iteration_limit_override);
   if (h.time_limit_override >= 0) tc.time_sec_limit = std::min(tc.time_sec_limit, h.time_limit_override);
+  // FP32 + refinement: a correction solve counts its iterations from 1, so
+  // its limit is what is left of the caller's after the earlier rounds (the
+  // next evaluation point and the limit test both use it)
+  if (prec_)
+    tc.iteration_limit =
+        (int32_t)std::max<int64_t>((int64_t)tc.iteration_limit - h.iters_total, 0);
   const QpCache &qc = h.qc_cur;
   const double c0 = P.c0;
   C.kkt_passes += 2.0;
@@ -1313,6 +1329,9 @@ void Engine::refine_logic(size_t q, const double *o) {
     tc.eps_optimal_absolute = h.tolerance_override;
     tc.eps_optimal_relative = h.tolerance_override;
   }
+  if (h.iteration_limit_override >= 0)
+    tc.iteration_limit = (int32_t)std::min<int64_t>(tc.iteration_limit, h.iteration_limit_override);
+  if (h.time_limit_override >= 0) tc.time_sec_limit = std::min(tc.time_sec_limit, h.time_limit_override);
   const double eps_ratio = tc.eps_optimal_absolute / tc.eps_optimal_relative;
   const double c0 = P.c0;
   pdlp_iteration_stats st;

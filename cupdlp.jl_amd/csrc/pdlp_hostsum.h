@@ -1,6 +1,8 @@
 // pdlp_hostsum.h — compensated sums on the host.  Same arithmetic as the
-// device's: error-free products (TwoProd with fma) and TwoSum, rounded once,
-// so host and device sums of the same terms agree whatever the order.
+// device's: error-free products (TwoProd with fma) and TwoSum, errors added in
+// FP64, rounded once (Dot2: error <= eps |sum| + gamma_n^2 sum |terms|), so
+// host and device sums of the same terms agree in any order unless the sum
+// cancels to ~n^2 eps of its terms' magnitude.
 #pragma once
 #include <math.h>
 #include <stddef.h>

@@ -36,7 +36,8 @@ int launch_slice_fill(hipStream_t s, int prec, int nslices, const SliceDesc *sli
                       void *sval);
 int launch_step(hipStream_t s, int ninst, const InstDev *insts, Ctl *ctl, const double *ppart,
                 const double *dpart, const double *red_tab, const double *grow_tab,
-                int64_t tab_len, double red_exp, double grow_exp, int constant);
+                int64_t tab_len, double red_exp, double grow_exp, int constant,
+                int64_t debug_attempt = 0);
 int launch_eval(hipStream_t s, int prec, int np, const BlockDesc *pblocks, int nd,
                 const BlockDesc *dblocks, const InstDev *insts, const Ctl *ctl, double *epn,
                 double *epm, double *epr, int ninst, double *out);

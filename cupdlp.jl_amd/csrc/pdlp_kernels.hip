@@ -15,13 +15,17 @@
 // costs nothing to undo and the host needs no synchronisation per attempt.
 //
 // Every sum (SpMV rows and all reductions) in the FP64 path is a compensated
-// sum of error-free products (TwoProd with FMA, TwoSum), so the rounded
-// result is the FP64 value nearest the exact sum whatever the summation order.
-// It is also deterministic: per-thread sums in a fixed stride order, wave
-// butterflies read from lane 0, waves and blocks combined in order.  Results
-// therefore match a CPU evaluation of the same formulas bit for bit (except
-// for exact sums lying within ~1e-30 relative of a rounding midpoint) and do
-// not depend on which other instances share the launch.
+// sum of error-free products (TwoProd with FMA, TwoSum) whose rounding errors
+// are themselves added in plain FP64 (the Dot2 scheme of Ogita, Rump and
+// Oishi): |result - exact| <= eps |exact| + gamma_n^2 sum |terms|.  Unless a
+// sum cancels to ~n^2 eps of its terms' magnitude, that is the FP64 value
+// nearest the exact sum whatever the summation order; under heavier
+// cancellation two orders can differ in the last bits.  Bitwise agreement
+// with the CPU oracle (which sums in another order) is therefore what the
+// tests observe, not a theorem.  Sums are deterministic: per-thread sums in a
+// fixed stride order, wave butterflies, waves and blocks combined in order,
+// and an instance's sums do not depend on which other instances share the
+// launch.
 //
 // Floating-point contraction is disabled for this file (-ffp-contract=off) so
 // each elementwise formula rounds exactly as the reference's written form.
@@ -481,10 +485,13 @@ __global__ __launch_bounds__(kThreads) void k_primal_long(const SliceDesc *__res
 
 // The adaptive step rule of take_step! (src/primal_dual_hybrid_gradient_gpu.jl:324-412)
 // applied to the reduced sums of one attempt; runs on one thread.
+// debug_attempt > 0: the attempt with that number sees movement = 0 (fault
+// injection of the numerical-error branch for tests).
 __device__ __forceinline__ void step_rule(Ctl *C, double ssx, double ssy, double sxy,
                                           const double *__restrict__ red_tab,
                                           const double *__restrict__ grow_tab, int64_t tab_len,
-                                          double red_exp, double grow_exp) {
+                                          double red_exp, double grow_exp,
+                                          int64_t debug_attempt) {
   const double omega = C->omega;
   double eta = C->eta;
   const int64_t k = C->attempts + 1;
@@ -496,7 +503,8 @@ __device__ __forceinline__ void step_rule(Ctl *C, double ssx, double ssy, double
   C->aty_valid = 1;
   const double interaction = fabs(sxy);
   const double nx = sqrt(ssx), ny = sqrt(ssy);
-  const double movement = 0.5 * omega * (nx * nx) + (0.5 / omega) * (ny * ny);
+  double movement = 0.5 * omega * (nx * nx) + (0.5 / omega) * (ny * ny);
+  if (k == debug_attempt) movement = 0.0;
   C->last_interaction = interaction;
   C->last_movement = movement;
   double limit;
@@ -746,7 +754,8 @@ __device__ __forceinline__ void step_instance(int q, const InstDev *__restrict__
                                               const double *__restrict__ red_tab,
                                               const double *__restrict__ grow_tab,
                                               int64_t tab_len, double red_exp, double grow_exp,
-                                              int constant, double (*sc)[2 * kStepThreads / 64]) {
+                                              int constant, int64_t debug_attempt,
+                                              double (*sc)[2 * kStepThreads / 64]) {
   Ctl *C = &ctl[q];
   if (!C->active) return;
   if (constant) {
@@ -784,7 +793,7 @@ __device__ __forceinline__ void step_instance(int q, const InstDev *__restrict__
     for (int k = 1; k < kStepThreads / 64; ++k) dd_merge(r[v], DD{sc[v][2 * k], sc[v][2 * k + 1]});
   }
   step_rule(C, dd_val(r[0]), dd_val(r[1]), dd_val(r[2]), red_tab, grow_tab, tab_len, red_exp,
-            grow_exp);
+            grow_exp, debug_attempt);
 }
 
 __global__ __launch_bounds__(kStepThreads) void k_step(const InstDev *__restrict__ insts,
@@ -794,10 +803,11 @@ __global__ __launch_bounds__(kStepThreads) void k_step(const InstDev *__restrict
                                                        const double *__restrict__ red_tab,
                                                        const double *__restrict__ grow_tab,
                                                        int64_t tab_len, double red_exp,
-                                                       double grow_exp, int constant) {
+                                                       double grow_exp, int constant,
+                                                       int64_t debug_attempt) {
   __shared__ double sc[3][2 * kStepThreads / 64];
   step_instance(blockIdx.x, insts, ctl, ppart, dpart, red_tab, grow_tab, tab_len, red_exp,
-                grow_exp, constant, sc);
+                grow_exp, constant, debug_attempt, sc);
 }
 
 // ---------------------------------------------------------------------------
@@ -1509,10 +1519,12 @@ int launch_slice_fill(hipStream_t s, int prec, int nslices, const SliceDesc *sli
 
 int launch_step(hipStream_t s, int ninst, const InstDev *insts, Ctl *ctl, const double *ppart,
                 const double *dpart, const double *red_tab, const double *grow_tab,
-                int64_t tab_len, double red_exp, double grow_exp, int constant) {
+                int64_t tab_len, double red_exp, double grow_exp, int constant,
+                int64_t debug_attempt) {
   if (ninst <= 0) return 0;
   hipLaunchKernelGGL(k_step, dim3(grid_of(ninst)), dim3(kStepThreads), 0, s, insts, ctl, ppart,
-                     dpart, red_tab, grow_tab, tab_len, red_exp, grow_exp, constant);
+                     dpart, red_tab, grow_tab, tab_len, red_exp, grow_exp, constant,
+                     debug_attempt);
   PDLP_CHECK_LAUNCH();
 }
 

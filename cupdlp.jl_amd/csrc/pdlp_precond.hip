@@ -264,7 +264,7 @@ __global__ __launch_bounds__(kPT) void k_norms(int64_t nnz, const double *__rest
 }
 
 // Power method of the constant step policy: out = M x over the rows of a
-// slice table (whole rows, exactly rounded sums; a chunked row by the wave of
+// slice table (whole rows, compensated sums; a chunked row by the wave of
 // its first chunk).
 __global__ __launch_bounds__(kPT) void k_row_spmv(const SliceDesc *__restrict__ sl, int nsl,
                                                   const int32_t *__restrict__ ptr,

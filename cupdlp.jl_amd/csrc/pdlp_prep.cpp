@@ -43,12 +43,17 @@ int validate_lp(const pdlp_lp *lp, std::string *err) {
     *err = "colptr inconsistent with nnz";
     return 1;
   }
+  // colptr in full first (monotone, within 1..nnz+1), so that the row scan
+  // below never reads outside rowval
   for (int64_t j = 0; j < lp->n; ++j) {
-    if (lp->colptr[j + 1] < lp->colptr[j]) {
-      snprintf(buf, sizeof buf, "colptr not monotone at column %lld", (long long)j + 1);
+    if (lp->colptr[j + 1] < lp->colptr[j] || lp->colptr[j + 1] > lp->nnz + 1) {
+      snprintf(buf, sizeof buf, "colptr not monotone or out of range at column %lld",
+               (long long)j + 1);
       *err = buf;
       return 1;
     }
+  }
+  for (int64_t j = 0; j < lp->n; ++j) {
     for (int64_t k = lp->colptr[j] - 1; k < lp->colptr[j + 1] - 1; ++k) {
       if (lp->rowval[k] < 1 || lp->rowval[k] > lp->m ||
           (k > lp->colptr[j] - 1 && lp->rowval[k] <= lp->rowval[k - 1])) {
@@ -102,8 +107,9 @@ int load_problem(const pdlp_lp *lp, Prepared *P, std::string *err, bool build_cs
   // count (descending) within the equality and the ">=" group, ties in the
   // original order.  The vectors the SpMVs gather from then keep their most
   // used entries together in few cache lines.  Every sum the engine forms is
-  // exactly rounded, so the numbering changes no computed value; results are
-  // mapped back to the caller's numbering.
+  // compensated (Dot2: error <= eps |sum| + gamma_n^2 sum |terms|), so the
+  // numbering changes no computed value unless a sum cancels to ~n^2 eps of
+  // its terms; results are mapped back to the caller's numbering.
   std::vector<int64_t> rcount(m, 0);
   for (int64_t k = 0; k < nnz; ++k) rcount[lp->rowval[k] - 1]++;
   // stable counting sorts by descending count
@@ -263,7 +269,10 @@ void rescale_host(const pdlp_params *prm, Prepared *P) {
   }
   // l2_norm_rescaling — src/preprocess.jl:89-103, 344-358
   if (prm->l2_norm_rescaling) {
-    std::vector<double> rsf(m, 0.0), csf(n, 0.0), rs(m, 0.0), cs(n, 0.0);
+    // sums of squares as compensated sums (no cancellation, so the value
+    // does not depend on the renumbered order)
+    std::vector<double> rsf(m, 0.0), csf(n, 0.0);
+    std::vector<HostDD> rs(m), cs(n);
     for (int64_t j = 0; j < n; ++j)
       for (int32_t k = P->t_ptr[j]; k < P->t_ptr[j + 1]; ++k) {
         const double a = fabs(P->t_val[k]);
@@ -277,16 +286,16 @@ void rescale_host(const pdlp_params *prm, Prepared *P) {
         const int32_t i = P->t_col[k];
         const double tr = (1.0 / rsf[i]) * P->t_val[k];
         const double tc = P->t_val[k] * (1.0 / csf[j]);
-        rs[i] += tr * tr;
-        cs[j] += tc * tc;
+        rs[i].add_prod(tr, tr);
+        cs[j].add_prod(tc, tc);
       }
     for (int64_t i = 0; i < m; ++i) {
-      double nr = rsf[i] * sqrt(rs[i]);
+      double nr = rsf[i] * sqrt(rs[i].value());
       if (nr == 0.0) nr = 1.0;
       r[i] = sqrt(nr);
     }
     for (int64_t j = 0; j < n; ++j) {
-      double nc = csf[j] * sqrt(cs[j]);
+      double nc = csf[j] * sqrt(cs[j].value());
       if (nc == 0.0) nc = 1.0;
       v[j] = sqrt(nc);
     }

@@ -32,10 +32,11 @@ class pdlp_mps_problem(C.Structure):
 
 class pdlp_termination_criteria(C.Structure):
     _fields_ = [
-        ("optimality_norm", C.c_int32), ("iteration_limit", C.c_int32),
+        ("optimality_norm", C.c_int32), ("pad0_", C.c_int32),
         ("eps_optimal_absolute", C.c_double), ("eps_optimal_relative", C.c_double),
         ("eps_primal_infeasible", C.c_double), ("eps_dual_infeasible", C.c_double),
-        ("time_sec_limit", C.c_double), ("kkt_matrix_pass_limit", C.c_double),
+        ("time_sec_limit", C.c_double), ("iteration_limit", C.c_int32), ("pad1_", C.c_int32),
+        ("kkt_matrix_pass_limit", C.c_double),
     ]
 
 
@@ -58,14 +59,16 @@ class pdlp_params(C.Structure):
         ("pock_chambolle_alpha", C.c_double),
         ("primal_importance", C.c_double),
         ("scale_invariant_initial_primal_weight", C.c_int32),
-        ("record_iteration_stats", C.c_int32),
+        ("pad0_", C.c_int32),
         ("verbosity", C.c_int64),
+        ("record_iteration_stats", C.c_int32),
         ("termination_evaluation_frequency", C.c_int32),
-        ("step_size_policy", C.c_int32),
-        ("reduction_exponent", C.c_double),
-        ("growth_exponent", C.c_double),
         ("termination_criteria", pdlp_termination_criteria),
         ("restart_params", pdlp_restart_params),
+        ("step_size_policy", C.c_int32),
+        ("pad1_", C.c_int32),
+        ("reduction_exponent", C.c_double),
+        ("growth_exponent", C.c_double),
         ("precision", C.c_int32),
         ("refine_max_outer", C.c_int32),
         ("refine_inner_tolerance", C.c_double),
@@ -74,7 +77,8 @@ class pdlp_params(C.Structure):
         ("refine_inner_iteration_limit", C.c_int32),
         ("batch_groups", C.c_int32),
         ("refine_early_fallback", C.c_int32),
-        ("reserved", C.c_int32 * 4),
+        ("debug_numerical_error_attempt", C.c_int32),
+        ("reserved", C.c_int32 * 3),
     ]
 
 

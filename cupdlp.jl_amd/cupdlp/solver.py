@@ -94,6 +94,7 @@ def to_c_params(p: PdhgParameters) -> _abi.pdlp_params:
     cp.refine_inner_iteration_limit = int(p.refine_inner_iteration_limit)
     cp.batch_groups = int(p.batch_groups)
     cp.refine_early_fallback = int(p.refine_early_fallback)
+    cp.debug_numerical_error_attempt = int(p.debug_numerical_error_attempt)
     return cp
 
 

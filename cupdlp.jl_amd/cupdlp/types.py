@@ -221,6 +221,7 @@ class PdhgParameters:
     refine_inner_iteration_limit: int = 0
     batch_groups: int = 0
     refine_early_fallback: int = 0
+    debug_numerical_error_attempt: int = 0  # test-only fault injection (movement = 0)
 
 
 @dataclass

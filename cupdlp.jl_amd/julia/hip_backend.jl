@@ -61,6 +61,9 @@ function construct_restart_parameters(
                              necessary_reduction_for_restart, primal_weight_update_smoothing)
 end
 
+# The reference's eleven fields in their order, then this backend's
+# extensions (include/pdlp_hip.h), which MOI's RawOptimizerAttribute can set
+# by name like any other field (src/MOI_wrapper.jl:104-121).
 mutable struct PdhgParameters
     l_inf_ruiz_iterations::Int
     l2_norm_rescaling::Bool
@@ -73,7 +76,22 @@ mutable struct PdhgParameters
     termination_criteria::TerminationCriteria
     restart_params::RestartParameters
     step_size_policy_params::Union{AdaptiveStepsizeParams,ConstantStepsizeParams}
+    # ---- extensions ----
+    precision::Symbol                  # :fp64 (the reference's arithmetic) or :fp32_refine
+    refine_max_outer::Int32
+    refine_inner_tolerance::Float64
+    refine_max_scale::Float64
+    device_preconditioning::Bool
+    refine_inner_iteration_limit::Int32
+    batch_groups::Int32
+    refine_early_fallback::Int32
 end
+
+# The reference's positional constructor (src/MOI_wrapper.jl:58-78 calls it
+# with eleven arguments): extensions take the library's defaults.
+PdhgParameters(a1, a2, a3, a4, a5, a6, a7, a8, a9, a10, a11) =
+    PdhgParameters(a1, a2, a3, a4, a5, a6, a7, a8, a9, a10, a11,
+                   :fp64, Int32(12), 1e-4, 1e6, true, Int32(0), Int32(0), Int32(0))
 
 struct SaddlePointOutput
     primal_solution::Vector{Float64}
@@ -104,12 +122,14 @@ end
 
 struct CTermination
     optimality_norm::Int32
-    iteration_limit::Int32
+    pad0::Int32
     eps_optimal_absolute::Float64
     eps_optimal_relative::Float64
     eps_primal_infeasible::Float64
     eps_dual_infeasible::Float64
     time_sec_limit::Float64
+    iteration_limit::Int32
+    pad1::Int32
     kkt_matrix_pass_limit::Float64
 end
 
@@ -130,14 +150,16 @@ struct CParams
     pock_chambolle_alpha::Float64
     primal_importance::Float64
     scale_invariant_initial_primal_weight::Int32
-    record_iteration_stats::Int32
+    pad0::Int32
     verbosity::Int64
+    record_iteration_stats::Int32
     termination_evaluation_frequency::Int32
-    step_size_policy::Int32
-    reduction_exponent::Float64
-    growth_exponent::Float64
     termination_criteria::CTermination
     restart_params::CRestart
+    step_size_policy::Int32
+    pad1::Int32
+    reduction_exponent::Float64
+    growth_exponent::Float64
     precision::Int32
     refine_max_outer::Int32
     refine_inner_tolerance::Float64
@@ -146,7 +168,8 @@ struct CParams
     refine_inner_iteration_limit::Int32
     batch_groups::Int32
     refine_early_fallback::Int32
-    reserved::NTuple{4,Int32}
+    debug_numerical_error_attempt::Int32
+    reserved::NTuple{3,Int32}
 end
 
 struct CConvergence
@@ -216,25 +239,29 @@ function _c_params(p::PdhgParameters)
     tc = p.termination_criteria
     rp = p.restart_params
     adaptive = p.step_size_policy_params isa AdaptiveStepsizeParams
-    ct = CTermination(tc.optimality_norm == L_INF ? 0 : 1, Int32(tc.iteration_limit),
+    ct = CTermination(tc.optimality_norm == L_INF ? 0 : 1, 0,
                       tc.eps_optimal_absolute, tc.eps_optimal_relative,
                       tc.eps_primal_infeasible, tc.eps_dual_infeasible,
-                      tc.time_sec_limit, tc.kkt_matrix_pass_limit)
+                      tc.time_sec_limit, Int32(tc.iteration_limit), 0,
+                      tc.kkt_matrix_pass_limit)
     cr = CRestart(Int32(rp.restart_scheme), Int32(rp.restart_to_current_metric),
                   rp.restart_frequency_if_fixed, rp.artificial_restart_threshold,
                   rp.sufficient_reduction_for_restart, rp.necessary_reduction_for_restart,
                   rp.primal_weight_update_smoothing)
+    p.precision in (:fp64, :fp32_refine) || error("precision must be :fp64 or :fp32_refine")
     return CParams(p.l_inf_ruiz_iterations, p.l2_norm_rescaling,
                    p.pock_chambolle_alpha === nothing ? 0 : 1,
                    p.pock_chambolle_alpha === nothing ? 0.0 : p.pock_chambolle_alpha,
-                   p.primal_importance, p.scale_invariant_initial_primal_weight,
-                   p.record_iteration_stats, p.verbosity, p.termination_evaluation_frequency,
-                   adaptive ? 0 : 1,
+                   p.primal_importance, p.scale_invariant_initial_primal_weight, 0,
+                   p.verbosity, p.record_iteration_stats, p.termination_evaluation_frequency,
+                   ct, cr,
+                   adaptive ? 0 : 1, 0,
                    adaptive ? p.step_size_policy_params.reduction_exponent : 0.3,
                    adaptive ? p.step_size_policy_params.growth_exponent : 0.6,
-                   ct, cr,
-                   0,        # PDLP_PRECISION_FP64: the reference's arithmetic
-                   12, 1e-4, 1e6, 1, 0, 0, 0, ntuple(_ -> Int32(0), 4))
+                   p.precision === :fp64 ? 0 : 1,   # PDLP_PRECISION_FP64 / _FP32_REFINE
+                   p.refine_max_outer, p.refine_inner_tolerance, p.refine_max_scale,
+                   Int32(p.device_preconditioning), p.refine_inner_iteration_limit,
+                   p.batch_groups, p.refine_early_fallback, 0, ntuple(_ -> Int32(0), 3))
 end
 
 function _stats(s::CIterationStats)
@@ -323,7 +350,173 @@ function optimize(params::PdhgParameters, original_problem::QuadraticProgramming
         stats[end] = final
     end
     reason = TerminationReason(res.termination_reason)
-    return SaddlePointOutput(x, y, reason, "", res.iteration_count, stats)
+    # termination_reason_to_string: src/termination.jl:248-251 (the string
+    # MOI's RawStatusString prints, src/MOI_wrapper.jl:216-221)
+    return SaddlePointOutput(x, y, reason, termination_reason_to_string(reason),
+                             res.iteration_count, stats)
+end
+
+function _c_lp(p::QuadraticProgrammingProblem, keep::Vector{Any})
+    A = p.constraint_matrix
+    colptr = Vector{Int64}(A.colptr)
+    rowval = Vector{Int64}(A.rowval)
+    nzval = Vector{Float64}(A.nzval)
+    c, lb, ub, b = p.objective_vector, p.variable_lower_bound, p.variable_upper_bound,
+                   p.right_hand_side
+    append!(keep, (colptr, rowval, nzval, c, lb, ub, b))
+    return CLp(size(A, 2), size(A, 1), p.num_equalities, length(nzval), pointer(colptr),
+               pointer(rowval), pointer(nzval), pointer(c), pointer(lb), pointer(ub),
+               pointer(b), p.objective_constant)
+end
+
+function _output(x, y, res::CResult)
+    reason = TerminationReason(res.termination_reason)
+    return SaddlePointOutput(x, y, reason, termination_reason_to_string(reason),
+                             res.iteration_count, [_stats(res.final_stats)])
+end
+
+"""
+    batch_optimize(params::PdhgParameters, problems::Vector{QuadraticProgrammingProblem})
+
+Solves independent LPs together on the device (pdlp_batch_solve): packed
+block-diagonally into the same kernel launches, each with its own step
+sizes, restarts and termination, each result equal to its own `optimize`
+(without the iteration history: each output holds the final statistics).
+`params.precision = :fp32_refine` runs the FP32 + FP64 refinement form.
+"""
+function batch_optimize(params::PdhgParameters, problems::Vector{QuadraticProgrammingProblem})
+    k = length(problems)
+    keep = Any[]
+    xs = [zeros(Float64, size(p.constraint_matrix, 2)) for p in problems]
+    ys = [zeros(Float64, size(p.constraint_matrix, 1)) for p in problems]
+    lps = Vector{CLp}(undef, k)
+    results = [CResult() for _ in 1:k]
+    GC.@preserve keep xs ys begin
+        for i in 1:k
+            lps[i] = _c_lp(problems[i], keep)
+            results[i].x = pointer(xs[i])
+            results[i].y = pointer(ys[i])
+            results[i].x_current = C_NULL
+            results[i].y_current = C_NULL
+        end
+        # pdlp_result is read and written in place: pass a contiguous array of
+        # the isbits images and copy back
+        flat = [_result_image(r) for r in results]
+        ctx = _context()
+        rc = ccall((:pdlp_batch_solve, LIBPDLP), Cint,
+                   (Ptr{Cvoid}, Int32, Ptr{CLp}, Ref{CParams}, Ptr{CResultImage}),
+                   ctx, Int32(k), lps, _c_params(params), flat)
+        if rc != 0
+            msg = unsafe_string(ccall((:pdlp_last_error, LIBPDLP), Cstring, (Ptr{Cvoid},), ctx))
+            rc > 0 ? error(msg) : throw(ErrorException("HIP failure: " * msg))
+        end
+        return [_output(xs[i], ys[i], _from_image(flat[i])) for i in 1:k]
+    end
+end
+
+# isbits image of pdlp_result, for arrays of results
+struct CResultImage
+    x::Ptr{Float64}
+    y::Ptr{Float64}
+    x_current::Ptr{Float64}
+    y_current::Ptr{Float64}
+    termination_reason::Int32
+    iteration_count::Int32
+    final_stats::CIterationStats
+    solve_time_sec::Float64
+    preprocess_time_sec::Float64
+    total_attempts::Int64
+    refinement_rounds::Int32
+    fp64_fallback::Int32
+end
+
+_result_image(r::CResult) = CResultImage(r.x, r.y, r.x_current, r.y_current, 0, 0,
+    _zero_stats(), 0.0, 0.0, 0, 0, 0)
+_zero_stats() = reinterpret(CIterationStats, zeros(UInt8, sizeof(CIterationStats)))[1]
+
+function _from_image(im::CResultImage)
+    r = CResult()
+    r.x, r.y, r.x_current, r.y_current = im.x, im.y, im.x_current, im.y_current
+    r.termination_reason = im.termination_reason
+    r.iteration_count = im.iteration_count
+    r.final_stats = im.final_stats
+    r.solve_time_sec = im.solve_time_sec
+    r.preprocess_time_sec = im.preprocess_time_sec
+    r.total_attempts = im.total_attempts
+    r.refinement_rounds = im.refinement_rounds
+    r.fp64_fallback = im.fp64_fallback
+    return r
+end
+
+# ------------------------------------------------------------ request loop
+# Continuous batching (pdlp_serve_*): requests are admitted by HBM footprint
+# as slots free up, run in shared launches, and retire as they converge.
+
+mutable struct HipServer
+    handle::Ptr{Cvoid}
+    sizes::Dict{Int64,Tuple{Int,Int}}
+end
+
+function HipServer(params::PdhgParameters; hbm_budget_bytes::Float64 = 200e9,
+                   max_slots::Integer = 0)
+    out = Ref{Ptr{Cvoid}}(C_NULL)
+    ctx = _context()
+    rc = ccall((:pdlp_serve_open, LIBPDLP), Cint,
+               (Ptr{Cvoid}, Ref{CParams}, Float64, Int32, Ref{Ptr{Cvoid}}),
+               ctx, _c_params(params), hbm_budget_bytes, Int32(max_slots), out)
+    rc == 0 || error(unsafe_string(ccall((:pdlp_last_error, LIBPDLP), Cstring,
+                                         (Ptr{Cvoid},), ctx)))
+    return HipServer(out[], Dict{Int64,Tuple{Int,Int}}())
+end
+
+"Queues an LP with its own tolerance; returns the request id."
+function submit!(s::HipServer, p::QuadraticProgrammingProblem, tolerance::Float64)
+    keep = Any[]
+    id = Ref{Int64}(0)
+    GC.@preserve keep begin
+        lp = _c_lp(p, keep)
+        rc = ccall((:pdlp_serve_submit, LIBPDLP), Cint,
+                   (Ptr{Cvoid}, Ref{CLp}, Float64, Ref{Int64}), s.handle, lp, tolerance, id)
+        rc == 0 || error(unsafe_string(ccall((:pdlp_last_error, LIBPDLP), Cstring,
+                                             (Ptr{Cvoid},), _context())))
+    end
+    s.sizes[id[]] = (size(p.constraint_matrix, 2), size(p.constraint_matrix, 1))
+    return id[]
+end
+
+"Advances the device work; returns the ids finished since the last poll."
+function poll!(s::HipServer)
+    ids = zeros(Int64, 256)
+    got = Ref{Int32}(0)
+    rc = ccall((:pdlp_serve_poll, LIBPDLP), Cint,
+               (Ptr{Cvoid}, Ptr{Int64}, Int32, Ref{Int32}), s.handle, ids, Int32(256), got)
+    rc == 0 || error(unsafe_string(ccall((:pdlp_last_error, LIBPDLP), Cstring,
+                                         (Ptr{Cvoid},), _context())))
+    return ids[1:got[]]
+end
+
+"The SaddlePointOutput of a finished request."
+function result!(s::HipServer, id::Int64)
+    n, m = s.sizes[id]
+    x, y = zeros(Float64, n), zeros(Float64, m)
+    res = CResult()
+    GC.@preserve x y begin
+        res.x, res.y = pointer(x), pointer(y)
+        res.x_current, res.y_current = C_NULL, C_NULL
+        rc = ccall((:pdlp_serve_result, LIBPDLP), Cint, (Ptr{Cvoid}, Int64, Ref{CResult}),
+                   s.handle, id, res)
+        rc == 0 || error(unsafe_string(ccall((:pdlp_last_error, LIBPDLP), Cstring,
+                                             (Ptr{Cvoid},), _context())))
+    end
+    delete!(s.sizes, id)
+    return _output(x, y, res)
+end
+
+function Base.close(s::HipServer)
+    s.handle == C_NULL && return
+    ccall((:pdlp_serve_close, LIBPDLP), Cint, (Ptr{Cvoid},), s.handle)
+    s.handle = C_NULL
+    return
 end
 
 # ---------------------------------------------------------------- MPS input

@@ -27,7 +27,7 @@
 extern "C" {
 #endif
 
-#define PDLP_ABI_VERSION 1
+#define PDLP_ABI_VERSION 2
 
 /* TerminationReason — src/solve_log.jl:323-334 */
 enum {
@@ -95,15 +95,18 @@ typedef struct pdlp_lp {
   double c0;              /* objective_constant */
 } pdlp_lp;
 
-/* TerminationCriteria — src/termination.jl:9-77 (same field order). */
+/* TerminationCriteria — src/termination.jl:9-77 (same field order; pad_
+ * fields are C alignment only). */
 typedef struct pdlp_termination_criteria {
   int32_t optimality_norm;          /* PDLP_L_INF / PDLP_L2 */
-  int32_t iteration_limit;          /* Int32 in the reference */
+  int32_t pad0_;
   double eps_optimal_absolute;
   double eps_optimal_relative;
   double eps_primal_infeasible;
   double eps_dual_infeasible;
   double time_sec_limit;
+  int32_t iteration_limit;          /* Int32 in the reference */
+  int32_t pad1_;
   double kkt_matrix_pass_limit;
 } pdlp_termination_criteria;
 
@@ -119,7 +122,15 @@ typedef struct pdlp_restart_params {
 } pdlp_restart_params;
 
 /* PdhgParameters — src/primal_dual_hybrid_gradient_gpu.jl:9-24, field for
- * field, followed by extensions that have no reference counterpart. */
+ * field in the same order (pad_ fields are C alignment only):
+ *   pock_chambolle_alpha::Union{Float64,Nothing}  -> has_pock_chambolle_alpha
+ *                                                    + pock_chambolle_alpha
+ *   step_size_policy_params::Union{AdaptiveStepsizeParams,
+ *                                  ConstantStepsizeParams}
+ *                                                 -> step_size_policy (the
+ *                                                    variant) + the fields of
+ *                                                    AdaptiveStepsizeParams (:1-4)
+ * followed by extensions that have no reference counterpart. */
 typedef struct pdlp_params {
   int64_t l_inf_ruiz_iterations;
   int32_t l2_norm_rescaling;
@@ -127,14 +138,16 @@ typedef struct pdlp_params {
   double pock_chambolle_alpha;
   double primal_importance;
   int32_t scale_invariant_initial_primal_weight;
-  int32_t record_iteration_stats;
+  int32_t pad0_;
   int64_t verbosity;
+  int32_t record_iteration_stats;
   int32_t termination_evaluation_frequency;
-  int32_t step_size_policy;          /* PDLP_STEP_ADAPTIVE / PDLP_STEP_CONSTANT */
-  double reduction_exponent;         /* AdaptiveStepsizeParams */
-  double growth_exponent;
   pdlp_termination_criteria termination_criteria;
   pdlp_restart_params restart_params;
+  int32_t step_size_policy;          /* PDLP_STEP_ADAPTIVE / PDLP_STEP_CONSTANT */
+  int32_t pad1_;
+  double reduction_exponent;         /* AdaptiveStepsizeParams */
+  double growth_exponent;
   /* ---- extensions ---- */
   int32_t precision;                 /* PDLP_PRECISION_* */
   int32_t refine_max_outer;          /* FP32_REFINE: max refinement rounds */
@@ -156,7 +169,12 @@ typedef struct pdlp_params {
                                         are those of starting it at the stall); <= 0 =
                                         never (the default: the extra FP64 work costs the
                                         512-LP batch more than it saves) */
-  int32_t reserved[4];
+  int32_t debug_numerical_error_attempt; /* fault injection for tests: > 0 makes the
+                                            step attempt with this total_number_iterations
+                                            see movement = 0, the reference's numerical
+                                            error branch (src/primal_dual_hybrid_gradient_gpu.jl:
+                                            384-390) when its interaction is > 0; 0 = off */
+  int32_t reserved[3];
 } pdlp_params;
 
 /* ConvergenceInformation — src/solve_log.jl:51-155 (same field order). */

@@ -26,9 +26,10 @@
  *
  * Choices where the reference leaves the result open (documented):
  *  - Every sum (SpMV rows, dot products, 2-norms, the Pock-Chambolle row and
- *    column sums) is a compensated sum of error-free products rounded once,
- *    i.e. the FP64 value nearest the exact sum, independent of the order;
- *    cuSPARSE's and cuBLAS's orders are unspecified.  2-norms are
+ *    column sums) is a compensated sum of error-free products rounded once
+ *    (Dot2: error <= eps |sum| + gamma_n^2 sum |terms|), i.e. the FP64 value
+ *    nearest the exact sum in any order unless the sum cancels to ~n^2 eps of
+ *    its terms; cuSPARSE's and cuBLAS's orders are unspecified.  2-norms are
  *    sqrt(sum x^2) without cuBLAS's overflow guard.
  *  - The triple product Diag(1/r) * A * Diag(1/v) of scale_problem is taken
  *    as ((1/r_i) * a_ij) * (1/v_j).
@@ -86,8 +87,10 @@ static inline double jl_min(double a, double b) {
 
 /* ---------------- sums ----------------
  * Exact mode (default): every sum is a compensated sum of error-free
- * products (TwoProd with fma, TwoSum), rounded once at the end, i.e. the
- * FP64 value nearest the exact sum whatever the summation order.  Plain mode
+ * products (TwoProd with fma, TwoSum), the errors added in FP64 and the
+ * total rounded once (Dot2 of Ogita, Rump and Oishi): error <= eps |sum| +
+ * gamma_n^2 sum |terms|, so the order changes the result only for sums that
+ * cancel to ~n^2 eps of their terms' magnitude.  Plain mode
  * (orc_set_exact(0)) uses ordinary FP64 accumulation and exists only to time
  * an ordinary CPU implementation. */
 
@@ -287,11 +290,19 @@ static int validate_lp(const pdlp_lp *lp) {
     snprintf(g_err, sizeof g_err, "colptr inconsistent with nnz");
     return 1;
   }
+  if (lp->nnz > 0 && (!lp->rowval || !lp->nzval)) {
+    snprintf(g_err, sizeof g_err, "null array in problem");
+    return 1;
+  }
+  /* colptr in full before any rowval read */
   for (int64_t j = 0; j < lp->n; ++j) {
-    if (lp->colptr[j + 1] < lp->colptr[j]) {
-      snprintf(g_err, sizeof g_err, "colptr not monotone at column %lld", (long long)j + 1);
+    if (lp->colptr[j + 1] < lp->colptr[j] || lp->colptr[j + 1] > lp->nnz + 1) {
+      snprintf(g_err, sizeof g_err, "colptr not monotone or out of range at column %lld",
+               (long long)j + 1);
       return 1;
     }
+  }
+  for (int64_t j = 0; j < lp->n; ++j) {
     for (int64_t k = lp->colptr[j] - 1; k < lp->colptr[j + 1] - 1; ++k) {
       if (lp->rowval[k] < 1 || lp->rowval[k] > lp->m ||
           (k > lp->colptr[j] - 1 && lp->rowval[k] <= lp->rowval[k - 1])) {
@@ -391,7 +402,11 @@ static void ruiz_inf(oprob *P, int64_t iters, double *cum_r, double *cum_v) {
 /* l2_norm_rescaling — src/preprocess.jl:89-103, 344-358. */
 static void l2_rescale(oprob *P, double *r_out, double *v_out) {
   int64_t m = P->m, n = P->n;
-  double *rsf = zeros(m), *csf = zeros(n), *rs = zeros(m), *cs = zeros(n);
+  double *rsf = zeros(m), *csf = zeros(n);
+  /* sums of squares as compensated sums (no cancellation: any order gives
+     the same value) */
+  dd_t *rs = (dd_t *)calloc((size_t)(m > 0 ? m : 1), sizeof(dd_t));
+  dd_t *cs = (dd_t *)calloc((size_t)(n > 0 ? n : 1), sizeof(dd_t));
   for (int64_t j = 0; j < n; ++j)
     for (int64_t k = P->colptr[j]; k < P->colptr[j + 1]; ++k) {
       int64_t i = P->rowidx[k];
@@ -407,16 +422,16 @@ static void l2_rescale(oprob *P, double *r_out, double *v_out) {
       int64_t i = P->rowidx[k];
       double tr = (1.0 / rsf[i]) * P->cval[k];
       double tc = P->cval[k] * (1.0 / csf[j]);
-      rs[i] += tr * tr;
-      cs[j] += tc * tc;
+      dd_add_prod(&rs[i], tr, tr);
+      dd_add_prod(&cs[j], tc, tc);
     }
   for (int64_t i = 0; i < m; ++i) {
-    double nr = rsf[i] * sqrt(rs[i]);
+    double nr = rsf[i] * sqrt(dd_val(rs[i]));
     if (nr == 0.0) nr = 1.0;
     r_out[i] = sqrt(nr);
   }
   for (int64_t j = 0; j < n; ++j) {
-    double nc = csf[j] * sqrt(cs[j]);
+    double nc = csf[j] * sqrt(dd_val(cs[j]));
     if (nc == 0.0) nc = 1.0;
     v_out[j] = sqrt(nc);
   }
@@ -834,8 +849,11 @@ typedef struct {
   double *dx, *dy, *Adx;
 } ostate;
 
-/* take_step! (adaptive) — src/primal_dual_hybrid_gradient_gpu.jl:343-413. */
-static void take_step_adaptive(const oprob *S, ostate *st, double red_exp, double grow_exp) {
+/* take_step! (adaptive) — src/primal_dual_hybrid_gradient_gpu.jl:343-413.
+ * debug_attempt > 0: the attempt with that total_number_iterations sees
+ * movement = 0 (fault injection of the numerical-error branch, :384-390). */
+static void take_step_adaptive(const oprob *S, ostate *st, double red_exp, double grow_exp,
+                               int64_t debug_attempt) {
   int64_t n = S->n, m = S->m, neq = S->neq;
   double eta = st->step_size;
   int done = 0;
@@ -860,6 +878,7 @@ static void take_step_adaptive(const oprob *S, ostate *st, double red_exp, doubl
     double interaction = fabs(vdot(st->Adx, st->dy, m));
     double nx = vnorm2(st->dx, n), ny = vnorm2(st->dy, m);
     double movement = 0.5 * omega * (nx * nx) + (0.5 / omega) * (ny * ny);
+    if (st->total_iters == debug_attempt) movement = 0.0;
     st->kkt_passes += 1.0;
     double limit;
     if (interaction > 0.0) {
@@ -1149,7 +1168,8 @@ int orc_solve(const pdlp_lp *lp, const pdlp_params *prm, pdlp_result *res, pdlp_
     }
     double t0 = now_sec();
     if (prm->step_size_policy == PDLP_STEP_ADAPTIVE)
-      take_step_adaptive(&S, &st, prm->reduction_exponent, prm->growth_exponent);
+      take_step_adaptive(&S, &st, prm->reduction_exponent, prm->growth_exponent,
+                         prm->debug_numerical_error_attempt);
     else
       take_step_constant(&S, &st);
     basic_time += now_sec() - t0;

@@ -36,7 +36,32 @@ def test_library_exports_every_declared_function():
 def test_abi_layout_matches_ctypes_mirror():
     lib = _abi.load()
     _abi.check_layout(lib)
-    assert lib.pdlp_abi_version() == 1
+    assert lib.pdlp_abi_version() == 2
+
+
+def test_struct_field_order_is_the_references():
+    """pdlp_termination_criteria follows src/termination.jl:9-77 and
+    pdlp_params src/primal_dual_hybrid_gradient_gpu.jl:9-24 field for field
+    (pad_ fields are C alignment; Union{Float64,Nothing} is has_ + value; the
+    step-size policy variant is step_size_policy + AdaptiveStepsizeParams'
+    two fields), then the extensions."""
+    def names(t):
+        return [f for f, _ in t._fields_ if not f.startswith("pad")]
+    assert names(_abi.pdlp_termination_criteria) == [
+        "optimality_norm", "eps_optimal_absolute", "eps_optimal_relative",
+        "eps_primal_infeasible", "eps_dual_infeasible", "time_sec_limit", "iteration_limit",
+        "kkt_matrix_pass_limit"]
+    head = names(_abi.pdlp_params)[:15]
+    assert head == [
+        "l_inf_ruiz_iterations", "l2_norm_rescaling", "has_pock_chambolle_alpha",
+        "pock_chambolle_alpha", "primal_importance", "scale_invariant_initial_primal_weight",
+        "verbosity", "record_iteration_stats", "termination_evaluation_frequency",
+        "termination_criteria", "restart_params", "step_size_policy", "reduction_exponent",
+        "growth_exponent", "precision"]
+    assert names(_abi.pdlp_restart_params) == [
+        "restart_scheme", "restart_to_current_metric", "restart_frequency_if_fixed",
+        "artificial_restart_threshold", "sufficient_reduction_for_restart",
+        "necessary_reduction_for_restart", "primal_weight_update_smoothing"]
 
 
 def test_default_params_match_reference_defaults():
@@ -45,7 +70,7 @@ def test_default_params_match_reference_defaults():
     lib.pdlp_default_params(C.byref(p))
     mirror = to_c_params(PdhgParameters())
     for f, _ in _abi.pdlp_params._fields_:
-        if f in ("termination_criteria", "restart_params", "reserved"):
+        if f in ("termination_criteria", "restart_params", "reserved") or f.startswith("pad"):
             continue
         assert getattr(p, f) == getattr(mirror, f), f
     for sub in ("termination_criteria", "restart_params"):

@@ -2,13 +2,15 @@
 
 Bar (FP64 path): bit-for-bit.  Both sides compute every sum (SpMV rows,
 dot products, norms) as a compensated sum of error-free products rounded
-once, i.e. the FP64 value nearest the exact sum, so summation order does not
-matter; every elementwise formula is evaluated in the same written order
-without contraction.  Step sizes, primal weights, restart decisions, every
-recorded KKT statistic and the returned iterates must therefore be equal.
-(Bitwise equality is stronger than a 1e-9 relative bar over the first 1,000
-iterations.)  The one documented exception would be an
-exact sum within ~1e-30 relative of an FP64 rounding midpoint.
+once (Dot2: error <= eps |sum| + gamma_n^2 sum |terms|), so the two
+summation orders give the same FP64 value unless a sum cancels to ~n^2 eps
+of its terms; every elementwise formula is evaluated in the same written
+order without contraction.  Step sizes, primal weights, restart decisions, every
+recorded KKT statistic and the returned iterates are then equal, and the
+tests ask for that.  (Bitwise equality is stronger than the required
+1e-9 relative bar over the first 1,000 iterations; a heavily cancelling sum
+rounding differently in the two orders would show up here as a last-bit
+difference, to be judged against that 1e-9 bar.)
 """
 import math
 import sys
