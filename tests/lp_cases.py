@@ -74,3 +74,22 @@ def long_col_lp(m=6000, seed=9):
     p = QuadraticProgrammingProblem(prob.variable_lower_bound, prob.variable_upper_bound, None,
                                     c, 0.0, A, b, prob.num_equalities)
     return p, float(c @ cert.x)
+
+
+def raw_lp(colptr, rowval, nzval, m, num_eq=0):
+    """A pdlp_lp view over hand-written (possibly malformed) 1-based CSC
+    arrays; returns (lp, arrays kept alive)."""
+    import ctypes as C
+    from cupdlp import _abi
+    n = len(colptr) - 1
+    cp = np.asarray(colptr, dtype=np.int64)
+    rv = np.asarray(rowval, dtype=np.int64)
+    nz = np.asarray(nzval, dtype=np.float64)
+    c, lb, ub, b = np.ones(n), np.zeros(n), np.full(n, 10.0), np.ones(max(m, 1))
+
+    def P(a, t):
+        return a.ctypes.data_as(C.POINTER(t))
+    lp = _abi.pdlp_lp(n=n, m=m, num_eq=num_eq, nnz=len(rv), colptr=P(cp, C.c_int64),
+                      rowval=P(rv, C.c_int64), nzval=P(nz, C.c_double), c=P(c, C.c_double),
+                      lb=P(lb, C.c_double), ub=P(ub, C.c_double), b=P(b, C.c_double), c0=0.0)
+    return lp, (cp, rv, nz, c, lb, ub, b)

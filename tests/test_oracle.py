@@ -198,6 +198,25 @@ def test_invalid_inputs_raise():
         O.solve(prob, params(primal_importance=-1.0))
 
 
+def test_malformed_colptr_is_refused_before_row_reads():
+    """colptr = [1, 10, 3] with nnz = 2: the endpoints agree with nnz, the
+    middle entry points past rowval; validation rejects it before reading
+    any row index."""
+    import ctypes as C
+    from cupdlp import _abi
+    from cupdlp.solver import to_c_params
+    for colptr in ([1, 10, 3], [1, 3, 2, 3]):
+        lp, keep = lp_cases.raw_lp(colptr, [1, 2], [1.0, 1.0], m=2)
+        res = _abi.pdlp_result()
+        x, y = np.zeros(len(colptr) - 1), np.zeros(2)
+        res.x = x.ctypes.data_as(C.POINTER(C.c_double))
+        res.y = y.ctypes.data_as(C.POINTER(C.c_double))
+        cp = to_c_params(params())
+        rc = O.lib().orc_solve(C.byref(lp), C.byref(cp), C.byref(res), _abi.STATS_CB(), None)
+        assert rc > 0
+        assert b"colptr" in O.lib().orc_last_error()
+
+
 # ---------------------------------------------------------------- rescaling
 
 def _dense_scaled(prob, vals):
