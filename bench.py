@@ -16,12 +16,24 @@ stalled refinements and the result download.  The rate with the load
 included is reported beside it.
 
 Reported beside it:
-  * plain FP64 one-at-a-time solves/s on the same GPU (a bounded prefix of the
-    batch), and the speed-up over it;
+  * plain FP64 one-at-a-time solves/s on the same GPU (the whole batch, or a
+    prefix if the budget runs out), the speed-up over it, and its p50/p99
+    latency (completion times with the batch submitted at once and solved
+    in order) and p99 per-solve time;
   * the same batch in plain FP64, batched (one timed solve);
-  * p50/p99 latency of the instances within a batch step;
+  * p50/p99 latency of the instances within a batch step (time from the
+    start of the run to each result);
+  * per-result error of every returned point: the reference's relative KKT
+    measure (max, median) and the objective error against the generator's
+    certified optimum;
   * `roofline`: the dominant iteration kernel's algorithmic bytes per launch
     over its HIP-event duration (events on the library's own stream);
+    `fallback_kernels`: the same for the FP64 engines that finish stalled
+    refinements; `aggregate_step_kernels`: the algorithmic bytes of every
+    step-kernel launch of the timed runs over their wall time;
+  * `served`: a bounded served stream in the same precision (request loop,
+    Poisson arrivals), solves/s and p50/p99 latency against plain FP64 one
+    at a time on the same requests;
   * `cpu_baseline`: the CPU oracle (same restarted PDHG, OpenMP) on a bounded
     sample of the same LPs on this host's cores.
 
@@ -86,9 +98,52 @@ def make_params(tol, precision, iteration_limit=2**31 - 1, time_limit=float("inf
 
 
 def batch_problems(count):
+    """The batch LPs and their certified optimal objectives."""
     from cupdlp.generators import batch_sizes, random_lp
     sizes = batch_sizes(count)
-    return [random_lp(n=int(n), m=int(n) // 2, seed=i)[0] for i, n in enumerate(sizes)]
+    gens = [random_lp(n=int(n), m=int(n) // 2, seed=i) for i, n in enumerate(sizes)]
+    return [g[0] for g in gens], [g[1].objective for g in gens]
+
+
+def accuracy(problems, outs, objectives):
+    """Per-result error of the returned points: the reference's relative KKT
+    measure (cupdlp.kkt; optimal at eps iff < eps) and the objective error
+    against the generator's certified optimum (an FP64 optimal solution by
+    construction), relative to 1 + |optimum|."""
+    from cupdlp.kkt import kkt_parts
+    kkt, err = [], []
+    for prob, o, ref in zip(problems, outs, objectives):
+        rp, rd, rg, p, _ = kkt_parts(prob, o.primal_solution, o.dual_solution)
+        kkt.append(max(rp, rd, rg))
+        err.append(abs(p - ref) / (1.0 + abs(ref)))
+    return {"max_rel_kkt": float(np.max(kkt)), "median_rel_kkt": float(np.median(kkt)),
+            "max_rel_obj_error_vs_certificate": float(np.max(err)),
+            "median_rel_obj_error_vs_certificate": float(np.median(err))}
+
+
+def fallback_kernels(prof):
+    """The FP64 engines that finish stalled FP32 + refinement instances:
+    launches, average durations and achieved GB/s of their step kernels."""
+    if prof.fallback_primal_launches == 0:
+        return None
+    pm, dm = prof.fallback_primal_ms, prof.fallback_dual_ms
+    return {"primal_launches": prof.fallback_primal_launches,
+            "primal_avg_us": round(pm / prof.fallback_primal_launches * 1e3, 2),
+            "primal_GBs": round(prof.fallback_primal_bytes / max(pm, 1e-12) / 1e6, 1),
+            "dual_launches": prof.fallback_dual_launches,
+            "dual_avg_us": round(dm / max(prof.fallback_dual_launches, 1) * 1e3, 2),
+            "dual_GBs": round(prof.fallback_dual_bytes / max(dm, 1e-12) / 1e6, 1)}
+
+
+def aggregate_roofline(prof, seconds):
+    """All step kernels of the timed runs (FP32 engines and FP64 fallback
+    engines, both half-steps): algorithmic bytes over the runs' wall time."""
+    total = (prof.primal_bytes + prof.dual_bytes + prof.fallback_primal_bytes
+             + prof.fallback_dual_bytes)
+    gbs = total / seconds / 1e9
+    return {"achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(gbs / HBM_PEAK_GBS, 4), "algorithmic_bytes": total,
+            "seconds": round(seconds, 3)}
 
 
 # Optimal objectives of generated LPs without a certificate, from HiGHS
@@ -212,7 +267,7 @@ def run_batch(args, ctx, lib):
     from cupdlp import _abi
     log(f"generating {args.count} LPs ...")
     t = time.time()
-    problems = batch_problems(args.count)
+    problems, objectives = batch_problems(args.count)
     nnz = sum(p.constraint_matrix.nnz for p in problems)
     log(f"  {time.time() - t:.1f} s, total nnz {nnz}, total n {sum(p.num_variables for p in problems)}")
     params = make_params(args.tol, args.precision, time_limit=args.time_limit)
@@ -239,24 +294,40 @@ def run_batch(args, ctx, lib):
     value = args.steps * len(problems) / dt
     extra = {"load_inclusive_solves_per_s": round(args.steps * len(problems) / (dt + load_s), 4),
              "load_s_per_step": round(load_s / args.steps, 3)}
+    if args.dump_instances:
+        with open(args.dump_instances, "w") as f:
+            json.dump([{"i": i, "n": p.num_variables, "nnz": int(p.constraint_matrix.nnz),
+                        "iterations": o.iteration_count, "attempts": o.total_attempts,
+                        "rounds": o.refinement_rounds, "fallback": int(o.fp64_fallback),
+                        "latency_s": round(o.solve_time_sec, 4)}
+                       for i, (p, o) in enumerate(zip(problems, outs))], f)
+    # error of every returned point of the last timed step
+    extra.update(accuracy(problems, outs, objectives))
+    extra["aggregate_step_kernels"] = aggregate_roofline(prof, dt)
+    extra["fallback_kernels"] = fallback_kernels(prof)
     if args.precision == "fp32" and not args.no_fp64_batch:
         # the same batch in plain FP64, batched (one timed solve)
         o64, d64, _ = solve_resident(ctx, problems,
                                      make_params(args.tol, "fp64", time_limit=args.time_limit))
         extra["fp64_batched_solves_per_s"] = round(len(problems) / d64, 4)
         extra["fp64_batched_p99_latency_s"] = round(float(np.percentile([o.solve_time_sec for o in o64], 99)), 4)
+        a64 = accuracy(problems, o64, objectives)
+        extra["fp64_batched_max_rel_kkt"] = a64["max_rel_kkt"]
+        extra["fp64_batched_max_rel_obj_error_vs_certificate"] = a64["max_rel_obj_error_vs_certificate"]
         log(f"fp64 batched: {d64:.2f} s")
-    # plain FP64 one at a time on a prefix of the batch (each LP loaded, then
-    # its solve timed, as above)
+    # plain FP64 one at a time (each LP loaded, then its solve timed, as
+    # above), the whole batch unless the budget runs out first; its latency
+    # is the completion time with the batch submitted at once and solved in
+    # order, its per-solve time beside it
     p64 = make_params(args.tol, "fp64")
     t1 = time.time()
-    nser, ser_s = 0, 0.0
+    ser_times = []
     for prob in problems:
         _, d_run, _ = solve_resident(ctx, [prob], p64)
-        ser_s += d_run
-        nser += 1
+        ser_times.append(d_run)
         if time.time() - t1 > args.serial_budget:
             break
+    nser, ser_s = len(ser_times), float(np.sum(ser_times))
     ser = nser / ser_s
     extra.update({
         "p50_latency_s": round(float(np.percentile(lat, 50)), 4),
@@ -266,8 +337,13 @@ def run_batch(args, ctx, lib):
         "median_iterations": int(np.median(iters)),
         "fp64_serial_solves_per_s": round(ser, 3),
         "fp64_serial_sample": nser,
+        "fp64_serial_p99_latency_s": round(float(np.percentile(np.cumsum(ser_times), 99)), 4),
+        "fp64_serial_p50_latency_s": round(float(np.percentile(np.cumsum(ser_times), 50)), 4),
+        "fp64_serial_p99_solve_s": round(float(np.percentile(ser_times, 99)), 4),
         "speedup_vs_fp64_serial": round(value / ser, 3),
     })
+    if args.serve_count > 0:
+        extra["served"] = served_leg(args, ctx)
     roof = profile_roofline(prof, ("k_primal", "k_dual"),
                             f"{args.count}-LP batch, {args.precision}, tol {args.tol:g}",
                             "batch", args.precision)
@@ -300,22 +376,17 @@ def serve_stream(count, nmax, seed=4):
     return probs, tols
 
 
-def run_serve(args, ctx, lib):
+def serve_once(ctx, probs, tols, rate, precision, hbm_budget_gb):
+    """Streams the requests through the request loop with Poisson arrivals
+    at `rate`: returns (seconds, latencies, results not OPTIMAL)."""
     from cupdlp.solver import Server
-    log(f"generating a stream of {args.count} requests ...")
-    probs, tols = serve_stream(args.count, args.serve_nmax)
     rng = np.random.default_rng(5)
-    gaps = rng.exponential(1.0 / args.rate, size=len(probs)) if args.rate > 0 else np.zeros(len(probs))
+    gaps = rng.exponential(1.0 / rate, size=len(probs)) if rate > 0 else np.zeros(len(probs))
     arrive = np.cumsum(gaps) - gaps[0]
-    params = make_params(1e-4, args.precision)
-    for w in range(args.warmup):  # warm-up: the first few requests alone
-        for p, t in list(zip(probs, tols))[:4]:
-            ctx.solve(p, make_params(t, args.precision))
-    srv = Server(ctx, params, hbm_budget_bytes=args.hbm_budget * 1e9)
+    srv = Server(ctx, make_params(1e-4, precision), hbm_budget_bytes=hbm_budget_gb * 1e9)
     lat, bad, done = [], 0, 0
     nxt = 0
     t0 = time.time()
-    ids = {}
     last_log = t0
     while done < len(probs):
         if time.time() - last_log > 30.0:
@@ -323,7 +394,7 @@ def run_serve(args, ctx, lib):
             log(f"  {last_log - t0:.0f} s: {nxt} submitted, {done} done")
         now = time.time() - t0
         while nxt < len(probs) and arrive[nxt] <= now:
-            ids[srv.submit(probs[nxt], tols[nxt])] = nxt
+            srv.submit(probs[nxt], tols[nxt])
             nxt += 1
         if done < nxt:
             for f in srv.poll():
@@ -335,15 +406,54 @@ def run_serve(args, ctx, lib):
             time.sleep(max(0.0, min(arrive[nxt] - now, 0.05)))
     dt = time.time() - t0
     srv.close()
-    value = len(probs) / dt
+    return dt, lat, bad
+
+
+def serial_stream(ctx, probs, tols, budget):
+    """Plain FP64 one request at a time, in arrival order, until the budget
+    is spent: (solves/s, requests solved)."""
     t1 = time.time()
     nser = 0
     for p, t in zip(probs, tols):
         ctx.solve(p, make_params(t, "fp64"))
         nser += 1
-        if time.time() - t1 > args.serial_budget:
+        if time.time() - t1 > budget:
             break
-    ser = nser / (time.time() - t1)
+    return nser / (time.time() - t1), nser
+
+
+def served_leg(args, ctx):
+    """A bounded served stream beside the batch line: args.serve_count
+    requests of the served mix (random LPs n <= args.serve_nmax, 10 %
+    transportation), Poisson arrivals at args.rate, in the batch line's
+    precision, against plain FP64 one at a time on the same requests."""
+    log(f"served leg: generating {args.serve_count} requests ...")
+    probs, tols = serve_stream(args.serve_count, args.serve_nmax)
+    dt, lat, bad = serve_once(ctx, probs, tols, args.rate, args.precision, args.hbm_budget)
+    ser, nser = serial_stream(ctx, probs, tols, args.serial_budget / 2)
+    value = len(probs) / dt
+    log(f"served leg: {value:.2f} solves/s, FP64 one at a time {ser:.2f}")
+    return {"requests": len(probs), "completed": len(lat), "not_optimal": bad,
+            "solves_per_s": round(value, 3), "seconds": round(dt, 2),
+            "p50_latency_s": round(float(np.percentile(lat, 50)), 4),
+            "p99_latency_s": round(float(np.percentile(lat, 99)), 4),
+            "fp64_serial_solves_per_s": round(ser, 3), "fp64_serial_sample": nser,
+            "speedup_vs_fp64_serial": round(value / ser, 3),
+            "workload": f"{len(probs)} requests: 90% random LPs n log-uniform 1000..{args.serve_nmax}, "
+                        f"10% transportation 200..1000 per side; tolerance 1e-4 (70%) / 1e-8 "
+                        f"(30%); Poisson arrivals at {args.rate:g}/s; {args.precision}; HBM budget "
+                        f"{args.hbm_budget:g} GB"}
+
+
+def run_serve(args, ctx, lib):
+    log(f"generating a stream of {args.count} requests ...")
+    probs, tols = serve_stream(args.count, args.serve_nmax)
+    for w in range(args.warmup):  # warm-up: the first few requests alone
+        for p, t in list(zip(probs, tols))[:4]:
+            ctx.solve(p, make_params(t, args.precision))
+    dt, lat, bad = serve_once(ctx, probs, tols, args.rate, args.precision, args.hbm_budget)
+    value = len(probs) / dt
+    ser, nser = serial_stream(ctx, probs, tols, args.serial_budget)
     extra = {"p50_latency_s": round(float(np.percentile(lat, 50)), 4),
              "p99_latency_s": round(float(np.percentile(lat, 99)), 4),
              "not_optimal": bad, "fp64_serial_solves_per_s": round(ser, 3),
@@ -441,7 +551,14 @@ def main():
                     help="FP32 + refinement, batches: iterations into a correction solve after "
                          "which the FP64 fallback starts beside it (0 = never, the default)")
     ap.add_argument("--iters", type=int, default=1000)
-    ap.add_argument("--serial-budget", type=float, default=20.0)
+    ap.add_argument("--serial-budget", type=float, default=60.0,
+                    help="seconds of plain FP64 one-at-a-time solves (the batch: stops early "
+                         "only if its LPs take longer; the served leg: half of it)")
+    ap.add_argument("--dump-instances", default="",
+                    help="batch: write each instance's size, iterations and latency (last "
+                         "timed step) to this JSON file")
+    ap.add_argument("--serve-count", type=int, default=160,
+                    help="batch: requests of the served leg reported beside it (0 = none)")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--time-limit", type=float, default=float("inf"),

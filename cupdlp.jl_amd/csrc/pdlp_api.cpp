@@ -363,7 +363,8 @@ struct BatchGroup {
   std::vector<int32_t> idx;  // the caller's index of each engine instance
   std::vector<std::unique_ptr<LpCopy>> lps;  // FP32 + refinement with host rescaling only
   int stream_main = 0, stream_fb = 1;        // ctx_stream numbers
-  Profiler prof;
+  Profiler prof;                             // the FP32 (or plain FP64) engine
+  Profiler fbprof;                           // the FP64 engine of stalled instances
   std::string err;
 };
 
@@ -384,6 +385,7 @@ struct pdlp_batch {
     for (auto &g : groups) {
       g->main.reset();
       for (hipEvent_t ev : g->prof.events) (void)hipEventDestroy(ev);
+      for (hipEvent_t ev : g->fbprof.events) (void)hipEventDestroy(ev);
     }
   }
 };
@@ -497,7 +499,7 @@ static int run_group(pdlp_batch *b, BatchGroup &grp, pdlp_result *results, pdlp_
   hipStream_t s_fb;
   rc = ctx_stream(ctx, grp.stream_fb, &s_fb, err);  // created before the groups start
   if (rc) return rc;
-  std::unique_ptr<Engine> fb;  // kernel profiling covers the FP32 engine only
+  std::unique_ptr<Engine> fb;  // profiled into grp.fbprof
   struct Prior {
     int64_t iters, attempts;
     int32_t rounds;
@@ -560,7 +562,7 @@ static int run_group(pdlp_batch *b, BatchGroup &grp, pdlp_result *results, pdlp_
         const int64_t left_it = (int64_t)p64.termination_criteria.iteration_limit - sofar;
         const double left_t = p64.termination_criteria.time_sec_limit - (now - t0);
         if (h.inner_cap > at && h.iter_start - 1 >= at && left_it > 0 && left_t > 0) {
-          if (!fb) fb = std::make_unique<Engine>(s_fb, &ctx->pool, p64);
+          if (!fb) fb = std::make_unique<Engine>(s_fb, &ctx->pool, p64, &grp.fbprof);
           e.x.assign((size_t)std::max<int64_t>(h.P.n, 1), 0.0);
           e.y.assign((size_t)std::max<int64_t>(h.P.m, 1), 0.0);
           e.res = pdlp_result{};
@@ -586,7 +588,7 @@ static int run_group(pdlp_batch *b, BatchGroup &grp, pdlp_result *results, pdlp_
       if (left_it <= 0 || !(left_t > 0)) continue;
       prior[q] = {r->iteration_count, r->total_attempts, r->refinement_rounds};
       refined[q].save(*r, h.P.n, h.P.m);
-      if (!fb) fb = std::make_unique<Engine>(s_fb, &ctx->pool, p64);
+      if (!fb) fb = std::make_unique<Engine>(s_fb, &ctx->pool, p64, &grp.fbprof);
       auto g = std::make_unique<HostInst>();
       g->res = r;
       g->t_submit = t0;
@@ -667,7 +669,15 @@ static int run_batch(pdlp_batch *b, pdlp_result *results, pdlp_stats_cb cb, void
     acc.dual_ms += p.dual_ms;
     acc.primal_bytes += p.primal_bytes;
     acc.dual_bytes += p.dual_bytes;
+    const pdlp_kernel_profile &f = grp->fbprof.acc;
+    acc.fallback_primal_launches += f.primal_launches;
+    acc.fallback_dual_launches += f.dual_launches;
+    acc.fallback_primal_ms += f.primal_ms;
+    acc.fallback_dual_ms += f.dual_ms;
+    acc.fallback_primal_bytes += f.primal_bytes;
+    acc.fallback_dual_bytes += f.dual_bytes;
     grp->prof.acc = pdlp_kernel_profile{};
+    grp->fbprof.acc = pdlp_kernel_profile{};
   }
   for (int g = 0; g < G; ++g)
     if (rcs[g]) {
@@ -778,6 +788,8 @@ struct ServeReq {
   double tol;
   size_t bytes;
   std::unique_ptr<LpCopy> lp;
+  double time_limit = INFINITY;  // the request's own, from submission
+  int64_t iter_limit = -1;       // the request's own (< 0: none)
 };
 
 struct ServeDone {
@@ -786,6 +798,8 @@ struct ServeDone {
   pdlp_result res;
   std::unique_ptr<LpCopy> lp;  // FP32 + refinement servers with host rescaling
   double tol = 0.0, t_submit = 0.0;
+  double time_limit = INFINITY;  // the request's own limits (see ServeReq)
+  int64_t iter_limit = -1;
   int64_t prior_iters = 0, prior_attempts = 0;
   int32_t prior_rounds = 0;
   size_t fb_bytes = 0;         // footprint of the FP64 solve, while it runs
@@ -834,7 +848,8 @@ extern "C" int pdlp_serve_open(pdlp_ctx *ctx, const pdlp_params *params, double 
   return 0;
 }
 
-extern "C" int pdlp_serve_submit(pdlp_server *s, const pdlp_lp *lp, double tolerance, int64_t *id) {
+static int serve_submit(pdlp_server *s, const pdlp_lp *lp, double tolerance, double time_limit,
+                        int64_t iter_limit, int64_t *id) {
   if (!s || !lp || !id) return 1;
   auto h = std::make_unique<HostInst>();
   int rc = pdlp::prepare_instance(lp, &s->prm, h.get(), &s->ctx->err);
@@ -844,6 +859,8 @@ extern "C" int pdlp_serve_submit(pdlp_server *s, const pdlp_lp *lp, double toler
   ServeReq r;
   r.id = s->next_id++;
   r.tol = tolerance;
+  r.time_limit = time_limit > 0.0 ? time_limit : INFINITY;
+  r.iter_limit = iter_limit;
   r.bytes = footprint(h->P, s->prm.precision == PDLP_PRECISION_FP32_REFINE);
   if ((double)r.bytes > s->budget) {
     s->ctx->err = "request exceeds the HBM budget";
@@ -862,27 +879,74 @@ extern "C" int pdlp_serve_submit(pdlp_server *s, const pdlp_lp *lp, double toler
   return 0;
 }
 
+extern "C" int pdlp_serve_submit(pdlp_server *s, const pdlp_lp *lp, double tolerance, int64_t *id) {
+  return serve_submit(s, lp, tolerance, INFINITY, -1, id);
+}
+
+extern "C" int pdlp_serve_submit_limits(pdlp_server *s, const pdlp_lp *lp, double tolerance,
+                                        double time_sec_limit, int32_t iteration_limit,
+                                        int64_t *id) {
+  return serve_submit(s, lp, tolerance, time_sec_limit, iteration_limit, id);
+}
+
+// Result storage of a request leaving the queue (admitted, expired or cancelled).
+static std::unique_ptr<ServeDone> serve_record(ServeReq &r) {
+  auto d = std::make_unique<ServeDone>();
+  d->x.assign(std::max<int64_t>(r.inst->P.n, 1), 0.0);
+  d->y.assign(std::max<int64_t>(r.inst->P.m, 1), 0.0);
+  d->n = r.inst->P.n;
+  d->m = r.inst->P.m;
+  memset(&d->res, 0, sizeof d->res);
+  d->res.x = d->x.data();
+  d->res.y = d->y.data();
+  d->lp = std::move(r.lp);
+  d->tol = r.tol;
+  d->t_submit = r.inst->t_submit;
+  d->time_limit = r.time_limit;
+  d->iter_limit = r.iter_limit;
+  return d;
+}
+
+// A request finished without a solution (its deadline passed in the queue,
+// or it was cancelled): reason, zero x and y.
+static void serve_finish_empty(pdlp_server *s, int64_t id, std::unique_ptr<ServeDone> d,
+                               int32_t reason) {
+  d->res.termination_reason = reason;
+  d->res.solve_time_sec = now_sec() - d->t_submit;
+  std::fill(d->x.begin(), d->x.end(), 0.0);
+  std::fill(d->y.begin(), d->y.end(), 0.0);
+  s->done[id] = std::move(d);
+  s->finished.push_back(id);
+}
+
 extern "C" int pdlp_serve_poll(pdlp_server *s, int64_t *ids, int32_t max_ids, int32_t *finished) {
   if (!s || !finished) return 1;
   std::string *err = &s->ctx->err;
   (void)hipSetDevice(s->ctx->device);
+  // requests whose own time limit ran out while queued finish with TIME_LIMIT
+  const double t_now = now_sec();
+  for (auto it = s->queue.begin(); it != s->queue.end();) {
+    if (t_now - it->inst->t_submit >= it->time_limit) {
+      const int64_t id = it->id;
+      serve_finish_empty(s, id, serve_record(*it), PDLP_TERMINATION_REASON_TIME_LIMIT);
+      it = s->queue.erase(it);
+    } else {
+      ++it;
+    }
+  }
   // admit by footprint and slot count
   while (!s->queue.empty() && (int32_t)s->eng->count() < s->max_slots &&
          s->in_use + (double)s->queue.front().bytes <= s->budget) {
     ServeReq r = std::move(s->queue.front());
     s->queue.pop_front();
-    auto d = std::make_unique<ServeDone>();
-    d->x.assign(std::max<int64_t>(r.inst->P.n, 1), 0.0);
-    d->y.assign(std::max<int64_t>(r.inst->P.m, 1), 0.0);
-    d->n = r.inst->P.n;
-    d->m = r.inst->P.m;
-    memset(&d->res, 0, sizeof d->res);
-    d->res.x = d->x.data();
-    d->res.y = d->y.data();
-    d->lp = std::move(r.lp);
-    d->tol = r.tol;
-    d->t_submit = r.inst->t_submit;
+    auto d = serve_record(r);
     r.inst->res = &d->res;
+    // the request's own limits on top of the server's; its time counts from
+    // submission (the solve's clock starts at admission)
+    if (r.iter_limit >= 0) r.inst->iteration_limit_override = r.iter_limit;
+    if (std::isfinite(r.time_limit))
+      r.inst->time_limit_override =
+          std::max(0.0, r.time_limit - (now_sec() - r.inst->t_submit));
     // counted as running only once the engine holds it; a request the
     // engine cannot take is finished with reason OTHER (its result says so)
     // and the error is returned
@@ -913,8 +977,11 @@ extern "C" int pdlp_serve_poll(pdlp_server *s, int64_t *ids, int32_t max_ids, in
         // within what is left of its iteration and time limits
         pdlp_params p64 = s->prm;
         p64.precision = PDLP_PRECISION_FP64;
-        const int64_t left_it = (int64_t)p64.termination_criteria.iteration_limit - d.res.iteration_count;
-        const double left_t = p64.termination_criteria.time_sec_limit - (now_sec() - d.t_submit);
+        int64_t lim_it = (int64_t)p64.termination_criteria.iteration_limit;
+        if (d.iter_limit >= 0) lim_it = std::min(lim_it, d.iter_limit);
+        const double lim_t = std::min(p64.termination_criteria.time_sec_limit, d.time_limit);
+        const int64_t left_it = lim_it - d.res.iteration_count;
+        const double left_t = lim_t - (now_sec() - d.t_submit);
         if (left_it > 0 && left_t > 0) {
           auto g = std::make_unique<HostInst>();
           g->tolerance_override = d.tol;
@@ -1000,6 +1067,52 @@ extern "C" int pdlp_serve_result(pdlp_server *s, int64_t id, pdlp_result *result
   if (y && d.m > 0) memcpy(y, d.y.data(), sizeof(double) * (size_t)d.m);
   s->done.erase(it);
   return 0;
+}
+
+extern "C" int pdlp_serve_cancel(pdlp_server *s, int64_t id) {
+  if (!s) return 1;
+  std::string *err = &s->ctx->err;
+  (void)hipSetDevice(s->ctx->device);
+  for (auto it = s->queue.begin(); it != s->queue.end(); ++it) {
+    if (it->id != id) continue;
+    serve_finish_empty(s, id, serve_record(*it), PDLP_TERMINATION_REASON_OTHER);
+    s->queue.erase(it);
+    return 0;
+  }
+  auto rit = s->running.find(id);
+  if (rit == s->running.end()) {
+    *err = "request not running or queued";
+    return 6;
+  }
+  ServeDone &d = *rit->second;
+  const bool refine = s->prm.precision == PDLP_PRECISION_FP32_REFINE;
+  // the request is on the FP32 engine, or (stalled) on the FP64 one
+  for (int which = 0; which < 2; ++which) {
+    Engine *e = which == 0 ? s->eng.get() : s->fb.get();
+    if (!e) continue;
+    for (size_t q = 0; q < e->count(); ++q) {
+      HostInst &h = e->inst(q);
+      if (h.tag != id || h.done) continue;
+      const int64_t it_now = which == 0 && refine ? h.iters_total + h.iter_start - 1
+                                                  : h.iter_start - 1;
+      const int64_t att = (which == 0 && refine ? h.attempts_total : 0) + e->attempts(q);
+      int rc = e->cancel(q, err);
+      if (rc) return rc;
+      s->in_use -= which == 0 ? (double)footprint(h.P, refine) : (double)d.fb_bytes;
+      d.res.iteration_count = (int32_t)std::min<int64_t>(
+          (which == 1 ? d.prior_iters : 0) + it_now, INT32_MAX);
+      d.res.total_attempts = (which == 1 ? d.prior_attempts : 0) + att;
+      d.res.refinement_rounds = which == 1 ? d.prior_rounds : h.ref_round;
+      d.res.fp64_fallback = which == 1 ? 1 : 0;
+      std::unique_ptr<ServeDone> rec = std::move(rit->second);
+      s->running.erase(rit);
+      serve_finish_empty(s, id, std::move(rec), PDLP_TERMINATION_REASON_OTHER);
+      return 0;
+    }
+  }
+  // admitted and finished in the engine, awaiting its FP64 staging: not expected
+  *err = "request not running or queued";
+  return 6;
 }
 
 extern "C" int pdlp_serve_close(pdlp_server *s) {

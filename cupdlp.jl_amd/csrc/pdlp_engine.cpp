@@ -557,7 +557,7 @@ int Engine::add(std::unique_ptr<HostInst> h, std::string *err) {
 
 int Engine::activate_staged(std::string *err) {
   for (size_t q = 0; q < inst_.size(); ++q) {
-    if (!inst_[q]->staged) continue;
+    if (!inst_[q]->staged || inst_[q]->done) continue;
     int rc = activate(q, err);
     if (rc) return rc;
   }
@@ -1539,9 +1539,15 @@ int Engine::run(std::string *err) {
 int Engine::cancel(size_t q, std::string *err) {
   HostInst &h = *inst_[q];
   if (h.done) return 0;
-  if (h.staged) {
-    if (err) *err = "cancel of a staged instance";
-    return -1;
+  if (h.staged) {  // not prepared yet: no device block; release the staging buffers
+    HIPCHK(hipStreamSynchronize(s_));
+    for (void *p : h.stage.bufs) pool_->put(p);
+    h.stage = HostInst::Stage();
+    h.staged = false;
+    h.done = true;
+    h.needs_eval = false;
+    tables_dirty_ = true;
+    return 0;
   }
   h.done = true;
   h.needs_eval = false;

@@ -143,9 +143,12 @@ class Engine {
   size_t count() const { return inst_.size(); }
   HostInst &inst(size_t i) { return *inst_[i]; }
   void remove_done();  // drops finished instances (serving)
-  // Ends active (not staged) instance q between rounds without writing its
-  // result.
+  // Ends instance q between rounds without writing its result (a staged
+  // instance is dropped before its preparation).
   int cancel(size_t q, std::string *err);
+  // Step attempts instance q has made in its current solve (as of the last
+  // completed round).
+  int64_t attempts(size_t q) const { return ctl_[q].attempts; }
 
  private:
   int transfer(HostInst &h, const DevSource *src, std::string *err);

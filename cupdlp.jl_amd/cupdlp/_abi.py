@@ -135,7 +135,10 @@ class pdlp_result(C.Structure):
 class pdlp_kernel_profile(C.Structure):
     _fields_ = [("primal_launches", C.c_int64), ("dual_launches", C.c_int64),
                 ("primal_ms", C.c_double), ("dual_ms", C.c_double),
-                ("primal_bytes", C.c_double), ("dual_bytes", C.c_double)]
+                ("primal_bytes", C.c_double), ("dual_bytes", C.c_double),
+                ("fallback_primal_launches", C.c_int64), ("fallback_dual_launches", C.c_int64),
+                ("fallback_primal_ms", C.c_double), ("fallback_dual_ms", C.c_double),
+                ("fallback_primal_bytes", C.c_double), ("fallback_dual_bytes", C.c_double)]
 
 
 STATS_CB = C.CFUNCTYPE(None, C.POINTER(pdlp_iteration_stats), C.c_void_p)
@@ -152,7 +155,7 @@ EXPORTED = (
     "pdlp_serve_poll", "pdlp_serve_result", "pdlp_serve_close", "pdlp_abi_version",
     "pdlp_abi_layout", "pdlp_set_profiling", "pdlp_get_profile",
     "pdlp_batch_load", "pdlp_batch_run", "pdlp_batch_free",
-    "pdlp_read_mps", "pdlp_mps_free",
+    "pdlp_read_mps", "pdlp_mps_free", "pdlp_serve_submit_limits", "pdlp_serve_cancel",
 )
 
 
@@ -190,6 +193,11 @@ def declare(lib):
     lib.pdlp_serve_submit.argtypes = [C.c_void_p, C.POINTER(pdlp_lp), C.c_double,
                                       C.POINTER(C.c_int64)]
     lib.pdlp_serve_submit.restype = C.c_int
+    lib.pdlp_serve_submit_limits.argtypes = [C.c_void_p, C.POINTER(pdlp_lp), C.c_double,
+                                             C.c_double, C.c_int32, C.POINTER(C.c_int64)]
+    lib.pdlp_serve_submit_limits.restype = C.c_int
+    lib.pdlp_serve_cancel.argtypes = [C.c_void_p, C.c_int64]
+    lib.pdlp_serve_cancel.restype = C.c_int
     lib.pdlp_serve_poll.argtypes = [C.c_void_p, C.POINTER(C.c_int64), C.c_int32,
                                     C.POINTER(C.c_int32)]
     lib.pdlp_serve_poll.restype = C.c_int

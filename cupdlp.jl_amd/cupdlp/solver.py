@@ -302,10 +302,21 @@ class Server:
         self.handle = h
         self._shape = {}
 
-    def submit(self, problem: QuadraticProgrammingProblem, tolerance: float) -> int:
+    def submit(self, problem: QuadraticProgrammingProblem, tolerance: float,
+               time_sec_limit=None, iteration_limit=None) -> int:
+        """Queues a request; optional limits of its own apply on top of the
+        server's (time counted from submission, queueing included)."""
         lp, keep = to_c_lp(problem)
         rid = C.c_int64()
-        rc = self.lib.pdlp_serve_submit(self.handle, C.byref(lp), float(tolerance), C.byref(rid))
+        if time_sec_limit is None and iteration_limit is None:
+            rc = self.lib.pdlp_serve_submit(self.handle, C.byref(lp), float(tolerance),
+                                            C.byref(rid))
+        else:
+            rc = self.lib.pdlp_serve_submit_limits(
+                self.handle, C.byref(lp), float(tolerance),
+                float("inf") if time_sec_limit is None else float(time_sec_limit),
+                -1 if iteration_limit is None else int(min(iteration_limit, 2**31 - 1)),
+                C.byref(rid))
         del keep
         if rc != 0:
             raise PdlpError(rc, self.ctx.error())
@@ -320,6 +331,13 @@ class Server:
         if rc != 0:
             raise PdlpError(rc, self.ctx.error())
         return [ids[i] for i in range(k.value)]
+
+    def cancel(self, rid: int):
+        """Cancels a queued or running request: it finishes with reason
+        OTHER and no solution, reported by the next ``poll``."""
+        rc = self.lib.pdlp_serve_cancel(self.handle, int(rid))
+        if rc != 0:
+            raise PdlpError(rc, self.ctx.error())
 
     def result(self, rid: int) -> SaddlePointOutput:
         n, m = self._shape.pop(rid)

@@ -469,14 +469,19 @@ function HipServer(params::PdhgParameters; hbm_budget_bytes::Float64 = 200e9,
     return HipServer(out[], Dict{Int64,Tuple{Int,Int}}())
 end
 
-"Queues an LP with its own tolerance; returns the request id."
-function submit!(s::HipServer, p::QuadraticProgrammingProblem, tolerance::Float64)
+"""
+Queues an LP with its own tolerance (and optionally its own time limit,
+counted from submission, and iteration limit); returns the request id.
+"""
+function submit!(s::HipServer, p::QuadraticProgrammingProblem, tolerance::Float64;
+                 time_sec_limit::Float64 = Inf, iteration_limit::Integer = -1)
     keep = Any[]
     id = Ref{Int64}(0)
     GC.@preserve keep begin
         lp = _c_lp(p, keep)
-        rc = ccall((:pdlp_serve_submit, LIBPDLP), Cint,
-                   (Ptr{Cvoid}, Ref{CLp}, Float64, Ref{Int64}), s.handle, lp, tolerance, id)
+        rc = ccall((:pdlp_serve_submit_limits, LIBPDLP), Cint,
+                   (Ptr{Cvoid}, Ref{CLp}, Float64, Float64, Int32, Ref{Int64}),
+                   s.handle, lp, tolerance, time_sec_limit, Int32(iteration_limit), id)
         rc == 0 || error(unsafe_string(ccall((:pdlp_last_error, LIBPDLP), Cstring,
                                              (Ptr{Cvoid},), _context())))
     end
@@ -493,6 +498,14 @@ function poll!(s::HipServer)
     rc == 0 || error(unsafe_string(ccall((:pdlp_last_error, LIBPDLP), Cstring,
                                          (Ptr{Cvoid},), _context())))
     return ids[1:got[]]
+end
+
+"Cancels a queued or running request (it finishes with TERMINATION_REASON_OTHER)."
+function cancel!(s::HipServer, id::Int64)
+    rc = ccall((:pdlp_serve_cancel, LIBPDLP), Cint, (Ptr{Cvoid}, Int64), s.handle, id)
+    rc == 0 || error(unsafe_string(ccall((:pdlp_last_error, LIBPDLP), Cstring,
+                                         (Ptr{Cvoid},), _context())))
+    return
 end
 
 "The SaddlePointOutput of a finished request."

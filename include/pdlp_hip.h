@@ -329,6 +329,19 @@ int pdlp_serve_open(pdlp_ctx *ctx, const pdlp_params *params,
 /* Copies the LP to the device queue; returns a request id >= 0 in *id. */
 int pdlp_serve_submit(pdlp_server *srv, const pdlp_lp *lp, double tolerance,
                       int64_t *id);
+/* pdlp_serve_submit with limits of this request's own, applied as the
+ * reference applies TerminationCriteria's (src/termination.jl:236-245) and
+ * on top of the server's: time_sec_limit counts from the request's
+ * submission, so time spent queued counts against it (<= 0 or +Inf: none);
+ * iteration_limit < 0: none.  A request that reaches one finishes with
+ * TIME_LIMIT / ITERATION_LIMIT and its last evaluated point, like a solve. */
+int pdlp_serve_submit_limits(pdlp_server *srv, const pdlp_lp *lp, double tolerance,
+                             double time_sec_limit, int32_t iteration_limit, int64_t *id);
+/* Cancels a queued or running request.  It finishes at once (its id is
+ * reported by the next poll) with termination reason OTHER, zero x and y
+ * and the counts of the work done so far; its device memory is released.
+ * Returns 6 for an id that is unknown or already finished. */
+int pdlp_serve_cancel(pdlp_server *srv, int64_t id);
 /* Advances the device work; returns the number of requests finished since
  * the last poll in *finished and copies their ids to ids[0..*finished). */
 int pdlp_serve_poll(pdlp_server *srv, int64_t *ids, int32_t max_ids,
@@ -350,6 +363,14 @@ typedef struct pdlp_kernel_profile {
   double dual_ms;
   double primal_bytes;  /* summed algorithmic bytes */
   double dual_bytes;
+  /* the same for the FP64 engines that finish stalled FP32 + refinement
+     instances (pdlp_batch_solve / pdlp_batch_run; s_A = s_v = 8) */
+  int64_t fallback_primal_launches;
+  int64_t fallback_dual_launches;
+  double fallback_primal_ms;
+  double fallback_dual_ms;
+  double fallback_primal_bytes;
+  double fallback_dual_bytes;
 } pdlp_kernel_profile;
 int pdlp_set_profiling(pdlp_ctx *ctx, int32_t enable);
 /* Copies the accumulated profile and resets it. */

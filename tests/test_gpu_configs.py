@@ -24,6 +24,7 @@ import oracle as O
 from cupdlp import (PdhgParameters, Precision, Server, TerminationReason as TR,
                     construct_termination_criteria)
 from cupdlp.generators import batch_sizes, config_lp, random_lp, transportation_lp
+from cupdlp.kkt import relative_kkt
 from test_gpu_parity import compare_runs
 
 pytestmark = pytest.mark.gpu
@@ -41,27 +42,8 @@ def params(tol, iteration_limit=2**31 - 1, precision=Precision.FP64, **kw):
 
 
 def rel_kkt(prob, x, y):
-    """max of the relative L2 primal residual, dual residual and gap of
-    (x, y) on the original LP -- the quantities the reference's optimality
-    test compares with eps (src/termination.jl:142-170,
-    src/iteration_stats_utils_gpu.jl:26-309), computed with numpy."""
-    A = prob.constraint_matrix
-    neq = prob.num_equalities
-    b, c = prob.right_hand_side, prob.objective_vector
-    lo, hi = prob.variable_lower_bound, prob.variable_upper_bound
-    t = b - A @ x
-    pv = np.concatenate([t[:neq], np.maximum(t[neq:], 0.0), np.maximum(lo - x, 0.0),
-                         np.maximum(x - hi, 0.0)])
-    g = c - A.T @ y
-    lam = np.maximum(g, 0.0) * np.isfinite(lo) + np.minimum(g, 0.0) * np.isfinite(hi)
-    dv = np.concatenate([np.maximum(-y[neq:], 0.0), g - lam])
-    p = prob.objective_constant + c @ x
-    contrib = np.where(lam > 0, np.where(np.isfinite(lo), lo, 0.0) * lam,
-                       np.where(lam < 0, np.where(np.isfinite(hi), hi, 0.0) * lam, 0.0))
-    d = prob.objective_constant + b @ y + contrib.sum()
-    return max(np.linalg.norm(pv) / (1 + np.linalg.norm(b)),
-               np.linalg.norm(dv) / (1 + np.linalg.norm(c)),
-               abs(p - d) / (1 + abs(p) + abs(d)))
+    """The reference's relative KKT measure of (x, y) (cupdlp.kkt, numpy)."""
+    return relative_kkt(prob, x, y)
 
 
 def test_c1_full_size_first_96_iterations(gpu_ctx):

@@ -147,3 +147,76 @@ def test_serve_fp32_stalled_falls_back_to_fp64(gpu_ctx):
         assert o.iteration_count > plain[k].iteration_count
         assert np.array_equal(o.primal_solution, plain[k].primal_solution)
         assert np.array_equal(o.dual_solution, plain[k].dual_solution)
+
+
+def test_serve_request_limits(gpu_ctx):
+    """Per-request limits (pdlp_serve_submit_limits): an iteration limit gives
+    the lone solve with that limit, bitwise; a deadline that passes while
+    queued or running ends the request with TIME_LIMIT; the others are
+    unaffected."""
+    print("case: per-request iteration and time limits", flush=True)
+    from cupdlp import TerminationReason as TR
+    probs, tols = stream()
+    tc = construct_termination_criteria(eps_optimal_absolute=1e-8, eps_optimal_relative=1e-8,
+                                        iteration_limit=50)
+    lim50 = gpu_ctx.solve(probs[3], PdhgParameters(termination_criteria=tc))
+    alone0 = gpu_ctx.solve(probs[0], params(tols[0]))
+    big, _ = random_lp(n=200_000, m=100_000, seed=9)
+    srv = Server(gpu_ctx, params(1e-4))
+    try:
+        out = {}
+        a = srv.submit(probs[0], tols[0])
+        b = srv.submit(probs[3], 1e-8, iteration_limit=50)
+        c = srv.submit(probs[1], 1e-8, time_sec_limit=1e-9)  # expires before admission
+        d = srv.submit(big, 1e-12, time_sec_limit=0.3)       # expires while running
+        drain(srv, [a, b, c, d], out)
+    finally:
+        srv.close()
+    same_result(out[a], alone0)
+    assert out[b].termination_reason == TR.TERMINATION_REASON_ITERATION_LIMIT
+    assert out[b].iteration_count == 50
+    same_result(out[b], lim50)
+    assert out[c].termination_reason == TR.TERMINATION_REASON_TIME_LIMIT
+    assert out[c].iteration_count == 0 and not out[c].primal_solution.any()
+    assert out[d].termination_reason == TR.TERMINATION_REASON_TIME_LIMIT
+    assert out[d].solve_time_sec >= 0.3 and out[d].iteration_count > 0
+
+
+@pytest.mark.parametrize("precision", [Precision.FP64, Precision.FP32_REFINE])
+def test_serve_cancel(gpu_ctx, precision):
+    """pdlp_serve_cancel of a running and of a queued request: both finish
+    with reason OTHER and no solution; a request running beside them and
+    the requests after them keep bitwise the results of lone solves."""
+    print(f"case: cancel a running and a queued request, {precision.name}", flush=True)
+    from cupdlp import TerminationReason as TR
+    probs, tols = stream()
+    keep = {3: 1e-8, 0: tols[0], 5: tols[5]}  # request 3 (n = 12,000) at 1e-8 runs long
+    alone = {k: gpu_ctx.solve(probs[k], params(t, precision)) for k, t in keep.items()}
+    big, _ = random_lp(n=150_000, m=75_000, seed=10)
+    srv = Server(gpu_ctx, params(1e-4, precision), max_slots=2)
+    try:
+        out = {}
+        run = srv.submit(big, 1e-12)
+        ids = {3: srv.submit(probs[3], 1e-8)}
+        queued = srv.submit(big, 1e-12)  # both slots taken: it waits
+        for _ in range(2):
+            for f in srv.poll():
+                out[f] = srv.result(f)
+        assert not out  # nothing finished yet
+        srv.cancel(run)
+        srv.cancel(queued)
+        with pytest.raises(PdlpError):
+            srv.cancel(10_000)
+        ids[0] = srv.submit(probs[0], keep[0])
+        ids[5] = srv.submit(probs[5], keep[5])
+        drain(srv, list(ids.values()) + [run, queued], out)
+        with pytest.raises(PdlpError):
+            srv.cancel(run)  # already finished
+    finally:
+        srv.close()
+    for k, rid in ids.items():
+        same_result(out[rid], alone[k])
+    for rid in (run, queued):
+        assert out[rid].termination_reason == TR.TERMINATION_REASON_OTHER
+        assert not out[rid].primal_solution.any()
+    assert out[run].iteration_count > 0 and out[queued].iteration_count == 0

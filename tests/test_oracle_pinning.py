@@ -314,3 +314,19 @@ def test_numerical_error_stops_after_the_failing_attempt(attempt):
     assert last.step_size > 0
     if attempt == 1:
         assert K == 1 and last.step_size == out.iteration_stats[0].step_size
+
+
+def test_host_relative_kkt_matches_the_oracle_statistics():
+    """cupdlp.kkt (numpy, reporting only) against the oracle's statistics of
+    the same points: certified optima (0) and returned points."""
+    from cupdlp.kkt import kkt_parts
+    for prob, cert in (config_lp(0), random_lp(n=500, m=250, seed=3)):
+        rp, rd, rg, p, d = kkt_parts(prob, cert.x, cert.y)
+        assert max(rp, rd, rg) < 1e-12
+        out = O.solve(prob, params(1e-5))
+        ci, _ = O.point_stats(prob, out.primal_solution, out.dual_solution, 1e-5, 1e-5)
+        rp, rd, rg, p, d = kkt_parts(prob, out.primal_solution, out.dual_solution)
+        np.testing.assert_allclose([rp, rd, rg, p, d],
+                                   [ci.relative_l2_primal_residual, ci.relative_l2_dual_residual,
+                                    ci.relative_optimality_gap, ci.primal_objective,
+                                    ci.dual_objective], rtol=1e-8)
