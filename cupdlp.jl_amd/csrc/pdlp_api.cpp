@@ -15,6 +15,7 @@
 
 #include "pdlp_engine.h"
 #include "pdlp_hip.h"
+#include "pdlp_noalloc.h"
 #include "pdlp_prep.h"
 
 using namespace pdlp;
@@ -27,6 +28,8 @@ struct pdlp_ctx {
   DevicePool pool;
   Profiler prof;
   std::string err;
+  int scratch_rc = 0;             // set-up of the no-allocation region (pdlp_ctx_create)
+  std::string scratch_err;
 };
 
 static std::mutex g_err_mu;
@@ -120,6 +123,9 @@ int pdlp_ctx_create(int device, pdlp_ctx **out) {
     delete c;
     return -(int)e;
   }
+  // the device address of this device's no-allocation region (once per
+  // device, the device current); a failure is reported by pdlp_solve_noalloc
+  c->scratch_rc = scratch_setup(device, &c->scratch_err);
   *out = c;
   return 0;
 }
@@ -378,6 +384,8 @@ struct pdlp_batch {
   pdlp_ctx *ctx = nullptr;
   pdlp_params prm{};
   int32_t count = 0;
+  std::unique_ptr<DevicePool> own_pool;  // no-allocation solves: the scratch region
+                                         // (declared first: outlives the engines)
   std::vector<std::unique_ptr<BatchGroup>> groups;
   double load_sec = 0.0;
   bool ran = false;
@@ -395,14 +403,33 @@ static int batch_group_count(int32_t count, const pdlp_params *params) {
   return std::max(1, std::min(g, std::min<int>(count, 8)));
 }
 
+// Device bytes an FP64 instance needs beyond its arena while it is prepared
+// and solved (rescaling scratch, slice and block tables, partial sums, the
+// step-size tables), an upper bound.
+static size_t solve_overhead_bytes(const Prepared &P) {
+  const size_t n = (size_t)P.n, m = (size_t)P.m, nnz = (size_t)P.nnz;
+  const size_t slices = P.ps.sl.size() + P.ds.sl.size();
+  const size_t blocks = 2 * (nnz / kBlockNnz + (n + m) / kBlockRows + 4);
+  return 20 * m + 8 * n + 2 * slices * sizeof(SliceDesc) + 48 * slices +
+         blocks * (sizeof(BlockDesc) + 2 * 8 * (kEvalN + 1)) + 2 * 8 * 65536 + (4 << 20);
+}
+
+static size_t footprint(const Prepared &P, int prec);
+
+// pool: the device pool of the batch's engines (null: the context's);
+// pageable: host staging without pinned memory; cap > 0: the LPs must fit
+// in a pool of that many bytes (checked on the host before any device work).
 static int load_batch(pdlp_ctx *ctx, int32_t count, const pdlp_lp *lps, const pdlp_params *params,
-                      std::unique_ptr<pdlp_batch> *out) {
+                      std::unique_ptr<pdlp_batch> *out, std::unique_ptr<DevicePool> pool = nullptr,
+                      bool pageable = false, size_t cap = 0) {
   std::string *err = &ctx->err;
   const double t0 = now_sec();
   auto b = std::make_unique<pdlp_batch>();
   b->ctx = ctx;
   b->prm = *params;
   b->count = count;
+  b->own_pool = std::move(pool);
+  DevicePool *dpool = b->own_pool ? b->own_pool.get() : &ctx->pool;
   const bool refine = params->precision == PDLP_PRECISION_FP32_REFINE;
   const bool keep = refine && !(params->device_preconditioning && !params->l2_norm_rescaling);
   // groups of equal instance counts, in order of nonzeros
@@ -418,7 +445,8 @@ static int load_batch(pdlp_ctx *ctx, int32_t count, const pdlp_lp *lps, const pd
     hipStream_t s;
     int rc = ctx_stream(ctx, grp->stream_main, &s, err);
     if (rc) return rc;
-    grp->main = std::make_unique<Engine>(s, &ctx->pool, *params, &grp->prof);
+    grp->main = std::make_unique<Engine>(s, dpool, *params, &grp->prof);
+    grp->main->set_pageable_host(pageable);
     const int32_t lo = (int32_t)((int64_t)count * g / G), hi = (int32_t)((int64_t)count * (g + 1) / G);
     for (int32_t k = lo; k < hi; ++k) grp->idx.push_back(order[k]);
     std::sort(grp->idx.begin(), grp->idx.end());
@@ -444,6 +472,16 @@ static int load_batch(pdlp_ctx *ctx, int32_t count, const pdlp_lp *lps, const pd
     for (int t = 1; t < nth; ++t) threads.emplace_back(work);
     work();
     for (auto &t : threads) t.join();
+  }
+  if (cap > 0) {
+    size_t need = 0;
+    for (int32_t i = 0; i < count; ++i)
+      if (!prc[i]) need += footprint(prepared[i]->P, 0) + solve_overhead_bytes(prepared[i]->P);
+    if (need > cap) {
+      *err = "the problem needs about " + std::to_string(need) + " bytes of device memory; " +
+             "the no-allocation region holds " + std::to_string(cap);
+      return 7;
+    }
   }
   for (auto &grp : b->groups) {
     for (int32_t i : grp->idx) {
@@ -643,7 +681,9 @@ static int run_batch(pdlp_batch *b, pdlp_result *results, pdlp_stats_cb cb, void
   }
   b->ran = true;
   const int G = (int)b->groups.size();
-  for (auto &grp : b->groups) grp->prof.enabled = ctx->prof.enabled;
+  // kernel timing creates HIP events: not in a no-allocation solve
+  const bool prof_on = ctx->prof.enabled && !b->own_pool;
+  for (auto &grp : b->groups) grp->prof.enabled = grp->fbprof.enabled = prof_on;
   if (b->prm.precision == PDLP_PRECISION_FP32_REFINE) {
     for (auto &grp : b->groups) {
       hipStream_t s;
@@ -706,6 +746,50 @@ extern "C" int pdlp_solve(pdlp_ctx *ctx, const pdlp_lp *lp, const pdlp_params *p
   std::unique_ptr<pdlp_batch> b;
   rc = load_batch(ctx, 1, lp, params, &b);
   if (rc == 0) rc = run_batch(b.get(), result, cb, user, t0);
+  if (rc) return rc;
+  result->solve_time_sec = now_sec() - t0;
+  return 0;
+}
+
+extern "C" int pdlp_solve_noalloc(pdlp_ctx *ctx, const pdlp_lp *lp, const pdlp_params *params,
+                                  pdlp_result *result, pdlp_stats_cb cb, void *user) {
+  if (!ctx) return 1;
+  ctx->err.clear();
+  if (!lp || !params || !result) {
+    ctx->err = "null argument";
+    return 1;
+  }
+  if ((lp->n > 0 && !result->x) || (lp->m > 0 && !result->y)) {
+    ctx->err = "result vectors must be allocated by the caller";
+    return 1;
+  }
+  const double t0 = now_sec();
+  int rc = check_params(params, &ctx->err);
+  if (rc) return rc;
+  if (params->precision != PDLP_PRECISION_FP64) {
+    ctx->err = "pdlp_solve_noalloc solves in PDLP_PRECISION_FP64 only";
+    return 2;
+  }
+  if (ctx->scratch_rc) {
+    ctx->err = ctx->scratch_err;
+    return ctx->scratch_rc;
+  }
+  (void)hipSetDevice(ctx->device);
+  void *base = scratch_region(ctx->device);
+  if (!base) {
+    ctx->err = "no-allocation region not set up on this device";
+    return -1;
+  }
+  std::unique_ptr<pdlp_batch> b;
+  rc = load_batch(ctx, 1, lp, params, &b, std::make_unique<DevicePool>(base, kScratchBytes),
+                  true, kScratchBytes);
+  if (rc == 0) rc = run_batch(b.get(), result, cb, user, t0);
+  if (rc < 0 && b && b->own_pool && ctx->err.find("allocation") != std::string::npos) {
+    // the region filled up beyond the host-side estimate
+    ctx->err = "the problem does not fit the no-allocation region of " +
+               std::to_string(kScratchBytes) + " bytes (" + ctx->err + ")";
+    rc = 7;
+  }
   if (rc) return rc;
   result->solve_time_sec = now_sec() - t0;
   return 0;

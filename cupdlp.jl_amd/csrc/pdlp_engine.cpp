@@ -38,12 +38,31 @@ static inline double jl_max(double a, double b) {
   } while (0)
 
 // ---------------------------------------------------------------- pool
+DevicePool::DevicePool(void *base, size_t bytes)
+    : region_((char *)base), region_bytes_(bytes & ~(size_t)255) {
+  free_.push_back({region_, region_bytes_});
+}
+
 DevicePool::~DevicePool() { release_all(); }
 
 void *DevicePool::get(size_t bytes) {
   std::lock_guard<std::mutex> lk(mu_);
   bytes = (bytes + 255) & ~(size_t)255;
   if (bytes == 0) bytes = 256;
+  if (region_) {  // first fit in the region's free list (sorted by address)
+    for (size_t i = 0; i < free_.size(); ++i) {
+      if (free_[i].bytes < bytes) continue;
+      void *p = free_[i].p;
+      free_[i].p = (char *)free_[i].p + bytes;
+      free_[i].bytes -= bytes;
+      if (free_[i].bytes == 0) free_.erase(free_.begin() + i);
+      used_.push_back({p, bytes});
+      in_use_ += bytes;
+      peak_ = std::max(peak_, in_use_);
+      return p;
+    }
+    return nullptr;
+  }
   size_t best = (size_t)-1;
   for (size_t i = 0; i < free_.size(); ++i)
     if (free_[i].bytes >= bytes && free_[i].bytes <= 2 * bytes + (1 << 20) &&
@@ -69,6 +88,7 @@ void *DevicePool::get(size_t bytes) {
   }
   used_.push_back({p, bytes});
   in_use_ += bytes;
+  peak_ = std::max(peak_, in_use_);
   return p;
 }
 
@@ -78,14 +98,36 @@ void DevicePool::put(void *p) {
   for (size_t i = 0; i < used_.size(); ++i)
     if (used_[i].p == p) {
       in_use_ -= used_[i].bytes;
-      free_.push_back(used_[i]);
+      const Blk b = used_[i];
       used_.erase(used_.begin() + i);
+      if (!region_) {
+        free_.push_back(b);
+        return;
+      }
+      // region: insert in address order and merge with the neighbours
+      size_t k = 0;
+      while (k < free_.size() && free_[k].p < b.p) ++k;
+      free_.insert(free_.begin() + k, b);
+      if (k + 1 < free_.size() && (char *)free_[k].p + free_[k].bytes == free_[k + 1].p) {
+        free_[k].bytes += free_[k + 1].bytes;
+        free_.erase(free_.begin() + k + 1);
+      }
+      if (k > 0 && (char *)free_[k - 1].p + free_[k - 1].bytes == free_[k].p) {
+        free_[k - 1].bytes += free_[k].bytes;
+        free_.erase(free_.begin() + k);
+      }
       return;
     }
 }
 
 void DevicePool::release_all() {
   std::lock_guard<std::mutex> lk(mu_);
+  if (region_) {  // the region is not ours to free
+    free_.assign(1, Blk{region_, region_bytes_});
+    used_.clear();
+    in_use_ = 0;
+    return;
+  }
   for (auto &b : free_) (void)hipFree(b.p);
   for (auto &b : used_) (void)hipFree(b.p);
   free_.clear();
@@ -108,11 +150,27 @@ Engine::~Engine() {
   for (void *p : table_allocs_) pool_->put(p);
   pool_->put(d_red_);
   pool_->put(d_grow_);
-  if (ctl_pinned_) (void)hipHostFree(ctl_pinned_);
-  if (out_pinned_) (void)hipHostFree(out_pinned_);
-  if (norm_pinned_) (void)hipHostFree(norm_pinned_);
-  if (rout_pinned_) (void)hipHostFree(rout_pinned_);
+  host_free(ctl_pinned_);
+  host_free(out_pinned_);
+  host_free(norm_pinned_);
+  host_free(rout_pinned_);
   if (d_rout_) pool_->put(d_rout_);
+}
+
+// Host staging buffers of the device copies: pinned (hipHostMalloc), or
+// ordinary heap memory when the engine must not allocate through HIP.
+hipError_t Engine::host_alloc(void **p, size_t bytes) {
+  if (!pageable_host_) return hipHostMalloc(p, bytes, 0);
+  *p = malloc(bytes);
+  return *p ? hipSuccess : hipErrorOutOfMemory;
+}
+
+void Engine::host_free(void *p) {
+  if (!p) return;
+  if (pageable_host_)
+    free(p);
+  else
+    (void)hipHostFree(p);
 }
 
 namespace {
@@ -332,7 +390,7 @@ int Engine::prepare(HostInst &h, std::string *err) {
   const size_t ts = prec_ ? sizeof(float) : sizeof(double);
   InstDev &d = h.dev;
   HostInst::Stage &sg = h.stage;
-  if (!norm_pinned_) HIPCHK(hipHostMalloc((void **)&norm_pinned_, 8 * 5 * 256, 0));
+  if (!norm_pinned_) HIPCHK(host_alloc((void **)&norm_pinned_, 8 * 5 * 256));
   std::vector<void *> tmp;
   auto scratch = [&](size_t bytes) -> void * {
     void *p = pool_->get(std::max<size_t>(bytes, 256));
@@ -491,7 +549,7 @@ int Engine::constant_step(HostInst &h, double *eta, double *passes, std::string 
     if (err) *err = "device allocation for the power method failed";
     return -2;
   }
-  if (!norm_pinned_) e = hipHostMalloc((void **)&norm_pinned_, 8 * 5 * 256, 0);
+  if (!norm_pinned_) e = host_alloc((void **)&norm_pinned_, 8 * 5 * 256);
   if (e == hipSuccess && nps)
     e = hipMemcpyAsync(tab, P.ps.sl.data(), sizeof(SliceDesc) * nps, hipMemcpyHostToDevice, s_);
   if (e == hipSuccess && nds)
@@ -771,13 +829,13 @@ int Engine::rebuild_tables(std::string *err) {
   std::vector<InstDev> devs(ninst);
   for (size_t q = 0; q < ninst; ++q) devs[q] = inst_[q]->dev;
   HIPCHK(hipMemcpyAsync(d_insts_, devs.data(), sizeof(InstDev) * ninst, hipMemcpyHostToDevice, s_));
-  if (ctl_pinned_) (void)hipHostFree(ctl_pinned_);
+  host_free(ctl_pinned_);
   ctl_pinned_ = nullptr;
-  HIPCHK(hipHostMalloc((void **)&ctl_pinned_, sizeof(Ctl) * std::max<size_t>(ninst, 1), 0));
+  HIPCHK(host_alloc((void **)&ctl_pinned_, sizeof(Ctl) * std::max<size_t>(ninst, 1)));
   if (out_cap_ < ninst) {
-    if (out_pinned_) (void)hipHostFree(out_pinned_);
+    host_free(out_pinned_);
     out_pinned_ = nullptr;
-    HIPCHK(hipHostMalloc((void **)&out_pinned_, sizeof(double) * kEvalStride * std::max<size_t>(ninst, 1), 0));
+    HIPCHK(host_alloc((void **)&out_pinned_, sizeof(double) * kEvalStride * std::max<size_t>(ninst, 1)));
     out_cap_ = ninst;
   }
   HIPCHK(push_ctl());
@@ -1276,7 +1334,7 @@ int Engine::refine(std::string *err) {
   if (rout_cap_ < ninst) {
     HIPCHK(hipStreamSynchronize(s_));
     if (d_rout_) pool_->put(d_rout_);
-    if (rout_pinned_) (void)hipHostFree(rout_pinned_);
+    host_free(rout_pinned_);
     d_rout_ = nullptr;
     rout_pinned_ = nullptr;
     rout_cap_ = 0;
@@ -1285,7 +1343,7 @@ int Engine::refine(std::string *err) {
       if (err) *err = "device allocation of refinement statistics failed";
       return -2;
     }
-    HIPCHK(hipHostMalloc((void **)&rout_pinned_, sizeof(double) * kRefOutStride * ninst, 0));
+    HIPCHK(host_alloc((void **)&rout_pinned_, sizeof(double) * kRefOutStride * ninst));
     rout_cap_ = ninst;
   }
   HIPCHK(push_ctl());

@@ -28,6 +28,12 @@ namespace pdlp {
 // groups share it from their host threads.
 class DevicePool {
  public:
+  DevicePool() = default;
+  // A pool confined to a fixed device region [base, base + bytes): get()
+  // carves 256-byte aligned blocks out of it (first fit) and returns null
+  // when it is full; put() gives them back (adjacent free blocks merge).
+  // It never calls hipMalloc or hipFree.
+  DevicePool(void *base, size_t bytes);
   ~DevicePool();
   void *get(size_t bytes);
   void put(void *p);
@@ -36,12 +42,18 @@ class DevicePool {
     std::lock_guard<std::mutex> lk(mu_);
     return in_use_;
   }
+  size_t peak() const {
+    std::lock_guard<std::mutex> lk(mu_);
+    return peak_;
+  }
 
  private:
   struct Blk { void *p; size_t bytes; };
   mutable std::mutex mu_;
   std::vector<Blk> free_, used_;
-  size_t in_use_ = 0;
+  size_t in_use_ = 0, peak_ = 0;
+  char *region_ = nullptr;  // fixed-region mode
+  size_t region_bytes_ = 0;
 };
 
 struct HipError {
@@ -149,6 +161,9 @@ class Engine {
   // Step attempts instance q has made in its current solve (as of the last
   // completed round).
   int64_t attempts(size_t q) const { return ctl_[q].attempts; }
+  // Host staging buffers from the heap instead of pinned memory (an engine
+  // that must not allocate through HIP); set before the first instance.
+  void set_pageable_host(bool on) { pageable_host_ = on; }
 
  private:
   int transfer(HostInst &h, const DevSource *src, std::string *err);
@@ -168,9 +183,13 @@ class Engine {
   void refine_logic(size_t q, const double *out);
   void reset_inner(size_t q, double omega0);
 
+  hipError_t host_alloc(void **p, size_t bytes);
+  void host_free(void *p);
+
   hipStream_t s_;
   DevicePool *pool_;
   Profiler *prof_ = nullptr;
+  bool pageable_host_ = false;
   pdlp_params prm_;
   int prec_ = 0;
   std::vector<std::unique_ptr<HostInst>> inst_;

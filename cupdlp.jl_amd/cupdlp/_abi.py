@@ -156,6 +156,7 @@ EXPORTED = (
     "pdlp_abi_layout", "pdlp_set_profiling", "pdlp_get_profile",
     "pdlp_batch_load", "pdlp_batch_run", "pdlp_batch_free",
     "pdlp_read_mps", "pdlp_mps_free", "pdlp_serve_submit_limits", "pdlp_serve_cancel",
+    "pdlp_solve_noalloc",
 )
 
 
@@ -172,6 +173,8 @@ def declare(lib):
     lib.pdlp_solve.argtypes = [C.c_void_p, C.POINTER(pdlp_lp), C.POINTER(pdlp_params),
                                C.POINTER(pdlp_result), STATS_CB, C.c_void_p]
     lib.pdlp_solve.restype = C.c_int
+    lib.pdlp_solve_noalloc.argtypes = lib.pdlp_solve.argtypes
+    lib.pdlp_solve_noalloc.restype = C.c_int
     lib.pdlp_batch_solve.argtypes = [C.c_void_p, C.c_int32, C.POINTER(pdlp_lp),
                                      C.POINTER(pdlp_params), C.POINTER(pdlp_result)]
     lib.pdlp_batch_solve.restype = C.c_int

@@ -193,6 +193,17 @@ class Context:
             raise PdlpError(rc, self.error())
         return out
 
+    def solve_noalloc(self, problem, params, want_current=False):
+        """``solve`` through pdlp_solve_noalloc: no device allocation (a
+        fixed 128 MiB region per device), FP64 only; calls on one device
+        must not overlap."""
+        def call(lp, cp, res, cb, user):
+            return self.lib.pdlp_solve_noalloc(self.handle, lp, cp, res, cb, user)
+        rc, out = run_solve(call, problem, params, want_current)
+        if rc != 0:
+            raise PdlpError(rc, self.error())
+        return out
+
     def batch_solve(self, problems, params):
         k = len(problems)
         lps = (_abi.pdlp_lp * k)()

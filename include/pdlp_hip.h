@@ -277,6 +277,22 @@ void pdlp_default_params(pdlp_params *p);
 int pdlp_solve(pdlp_ctx *ctx, const pdlp_lp *lp, const pdlp_params *params,
                pdlp_result *result, pdlp_stats_cb cb, void *user);
 
+/* pdlp_solve that allocates nothing (no counterpart in the reference): for
+ * callers that cannot call hipMalloc -- code under HIP graph capture, tight
+ * serving loops, harnesses that forbid allocation.  Same arguments, result
+ * and error behaviour as pdlp_solve (bitwise the same result), except:
+ *  - every device buffer comes from one fixed region per device, a
+ *    file-scope array of 128 MiB (room for an FP64 LP of about 1.5 million
+ *    nonzeros) whose address pdlp_ctx_create takes; no hipMalloc, hipFree,
+ *    hipHostMalloc or event creation, host staging in ordinary memory;
+ *  - an LP that does not fit the region is refused with code 7;
+ *  - PDLP_PRECISION_FP64 only (code 2 otherwise);
+ *  - the region is shared by every call on that device and keeps the last
+ *    call's contents: calls that overlap on one device, from any context or
+ *    thread, race and must not be made. */
+int pdlp_solve_noalloc(pdlp_ctx *ctx, const pdlp_lp *lp, const pdlp_params *params,
+                       pdlp_result *result, pdlp_stats_cb cb, void *user);
+
 /* Solves `count` independent LPs together, packed block-diagonally into the
  * same kernel launches with per-instance step sizes, restarts and
  * termination (extension; each instance follows optimize() exactly). */
