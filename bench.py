@@ -127,6 +127,11 @@ def fallback_kernels(prof):
     if prof.fallback_primal_launches == 0:
         return None
     pm, dm = prof.fallback_primal_ms, prof.fallback_dual_ms
+    if pm == 0.0:  # counted, not timed (--profile-fallback times them)
+        return {"primal_launches": prof.fallback_primal_launches,
+                "dual_launches": prof.fallback_dual_launches,
+                "primal_bytes": prof.fallback_primal_bytes, "dual_bytes": prof.fallback_dual_bytes,
+                "note": "launch durations in profiles/ (rocprofv3); not timed in this run"}
     return {"primal_launches": prof.fallback_primal_launches,
             "primal_avg_us": round(pm / prof.fallback_primal_launches * 1e3, 2),
             "primal_GBs": round(prof.fallback_primal_bytes / max(pm, 1e-12) / 1e6, 1),
@@ -275,7 +280,7 @@ def run_batch(args, ctx, lib):
         log(f"warmup {w}")
         ctx.batch_solve(problems, params)
     lib.pdlp_get_profile(ctx.handle, C.byref(_abi.pdlp_kernel_profile()))
-    lib.pdlp_set_profiling(ctx.handle, 1)
+    lib.pdlp_set_profiling(ctx.handle, 2 if args.profile_fallback else 1)
     lat, iters, bad, fallbacks = [], [], 0, 0
     dt, load_s = 0.0, 0.0
     for k in range(args.steps):
@@ -554,6 +559,9 @@ def main():
     ap.add_argument("--serial-budget", type=float, default=60.0,
                     help="seconds of plain FP64 one-at-a-time solves (the batch: stops early "
                          "only if its LPs take longer; the served leg: half of it)")
+    ap.add_argument("--profile-fallback", action="store_true",
+                    help="batch: time the FP64 fallback engines' launches too (events on "
+                         "every launch; slows the run)")
     ap.add_argument("--dump-instances", default="",
                     help="batch: write each instance's size, iterations and latency (last "
                          "timed step) to this JSON file")

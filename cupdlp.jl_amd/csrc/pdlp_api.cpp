@@ -28,6 +28,7 @@ struct pdlp_ctx {
   DevicePool pool;
   Profiler prof;
   std::string err;
+  bool prof_fallback_timed = false;  // pdlp_set_profiling(ctx, 2)
   int scratch_rc = 0;             // set-up of the no-allocation region (pdlp_ctx_create)
   std::string scratch_err;
 };
@@ -157,6 +158,7 @@ const char *pdlp_last_error(const pdlp_ctx *ctx) {
 int pdlp_set_profiling(pdlp_ctx *ctx, int32_t enable) {
   if (!ctx) return 1;
   ctx->prof.enabled = enable != 0;
+  ctx->prof_fallback_timed = enable >= 2;
   return 0;
 }
 
@@ -683,7 +685,10 @@ static int run_batch(pdlp_batch *b, pdlp_result *results, pdlp_stats_cb cb, void
   const int G = (int)b->groups.size();
   // kernel timing creates HIP events: not in a no-allocation solve
   const bool prof_on = ctx->prof.enabled && !b->own_pool;
-  for (auto &grp : b->groups) grp->prof.enabled = grp->fbprof.enabled = prof_on;
+  for (auto &grp : b->groups) {
+    grp->prof.enabled = grp->fbprof.enabled = prof_on;
+    grp->fbprof.timed = ctx->prof_fallback_timed;
+  }
   if (b->prm.precision == PDLP_PRECISION_FP32_REFINE) {
     for (auto &grp : b->groups) {
       hipStream_t s;

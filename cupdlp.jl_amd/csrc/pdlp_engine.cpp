@@ -891,8 +891,9 @@ int Engine::launch_attempts(int64_t count, std::string *err) {
   //   N (s_A + 4) + 4 (rows + 1) + s_v (gathered) + 9 s_v rows,
   // with s_A = s_v = 8 (FP64) or 4 (FP32).  Traffic the engine adds beyond
   // this (A^T y and dx stored, FP64 sums on the FP32 path) is not counted.
+  const bool timed = prof && prof_->timed;
   if (prof) {
-    const size_t need = 4 * (size_t)count;
+    const size_t need = timed ? 4 * (size_t)count : 0;
     while (prof_->events.size() < need) {
       hipEvent_t ev;
       HIPCHK(hipEventCreate(&ev));
@@ -909,7 +910,7 @@ int Engine::launch_attempts(int64_t count, std::string *err) {
   }
   const double t0 = now_sec();
   for (int64_t a = 0; a < count; ++a) {
-    hipEvent_t *ev = prof ? &prof_->events[4 * a] : nullptr;
+    hipEvent_t *ev = timed ? &prof_->events[4 * a] : nullptr;
     int e = launch_primal(s_, prec_, nps_, d_pslices_, npl_, d_pslices_ + nps_, d_longs_,
                           d_long_cnt_, d_long_part_, d_insts_, d_ctl_, d_ppart_,
                           ev ? ev[0] : nullptr, ev ? ev[1] : nullptr);
@@ -949,8 +950,10 @@ int Engine::complete_attempts(std::string *err) {
   if (prof) {
     for (int64_t a = 0; a < count; ++a) {
       float ms_p = 0.f, ms_d = 0.f;
-      HIPCHK(hipEventElapsedTime(&ms_p, prof_->events[4 * a + 0], prof_->events[4 * a + 1]));
-      HIPCHK(hipEventElapsedTime(&ms_d, prof_->events[4 * a + 2], prof_->events[4 * a + 3]));
+      if (prof_->timed) {
+        HIPCHK(hipEventElapsedTime(&ms_p, prof_->events[4 * a + 0], prof_->events[4 * a + 1]));
+        HIPCHK(hipEventElapsedTime(&ms_d, prof_->events[4 * a + 2], prof_->events[4 * a + 3]));
+      }
       double bp = 0.0, bd = 0.0;
       for (size_t q = 0; q < ninst; ++q)
         if (a < remaining[q]) {

@@ -123,6 +123,7 @@ struct HostInst {
 // Event-based timing of the iteration kernels (see pdlp_kernel_profile).
 struct Profiler {
   bool enabled = false;
+  bool timed = true;  // false: launches and bytes only, no events
   pdlp_kernel_profile acc{};
   std::vector<hipEvent_t> events;  // reusable pool
 };

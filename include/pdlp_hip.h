@@ -388,6 +388,10 @@ typedef struct pdlp_kernel_profile {
   double fallback_primal_bytes;
   double fallback_dual_bytes;
 } pdlp_kernel_profile;
+/* enable: 0 off; 1 the main engines' launches timed, the fallback engines'
+ * launches and bytes counted without events (their fallback_*_ms stay 0);
+ * 2 the fallback engines' launches timed as well (events on every FP64
+ * fallback launch: slower). */
 int pdlp_set_profiling(pdlp_ctx *ctx, int32_t enable);
 /* Copies the accumulated profile and resets it. */
 int pdlp_get_profile(pdlp_ctx *ctx, pdlp_kernel_profile *out);
