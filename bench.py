@@ -82,6 +82,8 @@ def log(*a):
 BATCH_GROUPS = 0  # --batch-groups
 REFINE_INNER_TOL = 0.0  # --refine-inner-tol (0: the library default)
 REFINE_EARLY = 0  # --refine-early-fallback (0: the library default)
+REFINE_WARM = -1  # --refine-warm-start (-1: the library default)
+REFINE_CAP = 0  # --refine-cap (0: the library default)
 
 
 def make_params(tol, precision, iteration_limit=2**31 - 1, time_limit=float("inf")):
@@ -94,6 +96,10 @@ def make_params(tol, precision, iteration_limit=2**31 - 1, time_limit=float("inf
     if REFINE_INNER_TOL > 0:
         p.refine_inner_tolerance = REFINE_INNER_TOL
     p.refine_early_fallback = REFINE_EARLY
+    if REFINE_WARM >= 0:
+        p.refine_warm_start = REFINE_WARM
+    if REFINE_CAP > 0:
+        p.refine_inner_iteration_limit = REFINE_CAP
     return p
 
 
@@ -555,6 +561,12 @@ def main():
     ap.add_argument("--refine-early-fallback", type=int, default=0,
                     help="FP32 + refinement, batches: iterations into a correction solve after "
                          "which the FP64 fallback starts beside it (0 = never, the default)")
+    ap.add_argument("--refine-cap", type=int, default=0,
+                    help="FP32 + refinement: iteration cap of each correction solve "
+                         "(0 = the library default)")
+    ap.add_argument("--refine-warm-start", type=int, default=-1,
+                    help="FP32 + refinement: correction solves start from the previous "
+                         "solve's primal weight (1), step size (2) or both (3)")
     ap.add_argument("--iters", type=int, default=1000)
     ap.add_argument("--serial-budget", type=float, default=60.0,
                     help="seconds of plain FP64 one-at-a-time solves (the batch: stops early "
@@ -572,7 +584,10 @@ def main():
     ap.add_argument("--time-limit", type=float, default=float("inf"),
                     help="per-solve time limit (a guard; instances that hit it count as not optimal)")
     args = ap.parse_args()
-    global BATCH_GROUPS, REFINE_INNER_TOL, REFINE_EARLY
+    global BATCH_GROUPS, REFINE_INNER_TOL, REFINE_EARLY, REFINE_WARM
+    REFINE_WARM = args.refine_warm_start
+    global REFINE_CAP
+    REFINE_CAP = args.refine_cap
     BATCH_GROUPS = args.batch_groups
     REFINE_INNER_TOL = args.refine_inner_tol
     REFINE_EARLY = args.refine_early_fallback

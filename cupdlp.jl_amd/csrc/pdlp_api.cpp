@@ -96,6 +96,7 @@ void pdlp_default_params(pdlp_params *p) {
   p->refine_max_scale = 1.0e6;
   p->device_preconditioning = 1;
   p->refine_inner_iteration_limit = 0;
+  p->refine_warm_start = 3;
 }
 
 int pdlp_ctx_create(int device, pdlp_ctx **out) {

@@ -1015,6 +1015,9 @@ static int check_termination(const pdlp_termination_criteria &tc, const QpCache 
   return 0;
 }
 
+// Iteration cap of the first FP32 solve of FP32 + refinement.
+constexpr int64_t kFirstSolveCap = 100000;
+
 static int64_t next_eval_iteration(int64_t it, int64_t freq, int64_t limit) {
   int64_t j = it + 1;
   while (!((j - 1) % freq == 0 || j == limit + 1 || j <= 10)) ++j;
@@ -1101,6 +1104,15 @@ void Engine::host_eval_logic(size_t q, const double *o) {
   // a correction solve that has not met its tolerance within its cap is
   // refined anyway (the outer rounds are bounded by refine_max_outer)
   if (reason == 0 && prec_ && h.inner_cap > 0 && (int64_t)st.iteration_number >= h.inner_cap) {
+    reason = PDLP_TERMINATION_REASON_ITERATION_LIMIT;
+    ++h.capped_rounds;
+  }
+  // the first FP32 solve is capped too (an inner tolerance FP32 data cannot
+  // reach would otherwise never end it): the refinement then counts as
+  // stalled and the caller's FP64 fallback takes over
+  if (reason == 0 && prec_ && h.ref_round == 0 &&
+      (int64_t)st.iteration_number >=
+          std::max<int64_t>(kFirstSolveCap, prm_.refine_inner_iteration_limit)) {
     reason = PDLP_TERMINATION_REASON_ITERATION_LIMIT;
     ++h.capped_rounds;
   }
@@ -1473,6 +1485,7 @@ void Engine::refine_logic(size_t q, const double *o) {
   const double ep = jl_max(M[1], N[1]), edl = jl_max(N[6], M[3]);
   const double dp = ep > 0.0 ? std::min(1.0 / ep, alpha * h.dp) : alpha * h.dp;
   const double ddn = edl > 0.0 ? std::min(1.0 / edl, alpha * h.dd) : alpha * h.dd;
+  const double dp_prev = h.dp, dd_prev = h.dd;
   h.dp = dp;
   h.dd = ddn;
   if (h.ref_round == 0)
@@ -1493,6 +1506,10 @@ void Engine::refine_logic(size_t q, const double *o) {
   double omega0 = prm_.primal_importance;
   if (prm_.scale_invariant_initial_primal_weight && nc > 0.0 && nbn > 0.0)
     omega0 = prm_.primal_importance * (nc / nbn);
+  // refine_warm_start bit 1: the previous solve's final primal weight
+  // instead, carried to the correction's scaling (a weight scales as dD / dP)
+  if ((prm_.refine_warm_start & 1) && C.omega > 0.0)
+    omega0 = C.omega * (ddn / dp) / (dd_prev / dp_prev);
   C.ref_flag = 2;
   C.ref_keep = improved ? 1 : 0;
   C.ref_dp = dp;
@@ -1512,7 +1529,11 @@ void Engine::refine_logic(size_t q, const double *o) {
 void Engine::reset_inner(size_t q, double omega0) {
   HostInst &h = *inst_[q];
   Ctl &C = ctl_[q];
-  const double eta0 = h.P.max_abs_scaled > 0.0 ? 1.0 / h.P.max_abs_scaled : 1.0;
+  // refine_warm_start bit 2: the previous solve's final step size (the
+  // correction LP has the same scaled matrix)
+  const double eta0 = (prm_.refine_warm_start & 2) && C.eta_entry > 0.0 ? C.eta_entry
+                      : h.P.max_abs_scaled > 0.0 ? 1.0 / h.P.max_abs_scaled
+                                                 : 1.0;
   const double kkt = C.kkt_passes;
   const double dp = C.ref_dp, ddn = C.ref_dd;
   const int32_t flag = C.ref_flag, keep = C.ref_keep;

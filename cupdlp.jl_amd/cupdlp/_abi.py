@@ -78,7 +78,8 @@ class pdlp_params(C.Structure):
         ("batch_groups", C.c_int32),
         ("refine_early_fallback", C.c_int32),
         ("debug_numerical_error_attempt", C.c_int32),
-        ("reserved", C.c_int32 * 3),
+        ("refine_warm_start", C.c_int32),
+        ("reserved", C.c_int32 * 2),
     ]
 
 

@@ -95,6 +95,7 @@ def to_c_params(p: PdhgParameters) -> _abi.pdlp_params:
     cp.batch_groups = int(p.batch_groups)
     cp.refine_early_fallback = int(p.refine_early_fallback)
     cp.debug_numerical_error_attempt = int(p.debug_numerical_error_attempt)
+    cp.refine_warm_start = int(p.refine_warm_start)
     return cp
 
 

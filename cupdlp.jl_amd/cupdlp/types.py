@@ -222,6 +222,7 @@ class PdhgParameters:
     batch_groups: int = 0
     refine_early_fallback: int = 0
     debug_numerical_error_attempt: int = 0  # test-only fault injection (movement = 0)
+    refine_warm_start: int = 3
 
 
 @dataclass

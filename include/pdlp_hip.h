@@ -174,7 +174,13 @@ typedef struct pdlp_params {
                                             see movement = 0, the reference's numerical
                                             error branch (src/primal_dual_hybrid_gradient_gpu.jl:
                                             384-390) when its interaction is > 0; 0 = off */
-  int32_t reserved[3];
+  int32_t refine_warm_start;         /* FP32_REFINE: a correction solve starts from the
+                                        previous solve's final primal weight (bit 1,
+                                        rescaled by the correction's scaling) and step
+                                        size (bit 2) instead of fresh ones; 0 = fresh;
+                                        default 3 (fewer stalled refinements on the
+                                        512-LP batch: 16 vs 21) */
+  int32_t reserved[2];
 } pdlp_params;
 
 /* ConvergenceInformation — src/solve_log.jl:51-155 (same field order). */

@@ -130,6 +130,23 @@ def test_malformed_colptr_refused(gpu_ctx):
         assert "colptr" in gpu_ctx.error()
 
 
+def test_fp32_first_solve_cap_falls_back(gpu_ctx):
+    """An inner tolerance FP32 data cannot reach (1e-13) stops the first FP32
+    solve at its 100,000-iteration cap; the refinement counts as stalled and
+    the FP64 solve finishes the LP, bitwise as a plain FP64 solve."""
+    say("case: FP32 + refinement with refine_inner_tolerance 1e-13 (first-solve cap)")
+    prob, _ = config_lp(0, scale=0.5)
+    g = gpu_ctx.solve(prob, params(1e-8, precision=Precision.FP32_REFINE,
+                                   refine_inner_tolerance=1e-13))
+    f = gpu_ctx.solve(prob, params(1e-8))
+    assert g.fp64_fallback and g.refinement_rounds == 0
+    assert g.termination_reason == TR.TERMINATION_REASON_OPTIMAL
+    # the cap acts at the first evaluation (every 64 iterations) at or past it
+    assert 100000 <= g.iteration_count - f.iteration_count < 100064
+    assert np.array_equal(g.primal_solution, f.primal_solution)
+    assert np.array_equal(g.dual_solution, f.dual_solution)
+
+
 def test_fp32_refine_stays_within_iteration_limit(gpu_ctx):
     """The default correction-solve cap (max(10000, 20 x round 0)) is far above
     the limit: the refinement rounds together must stop at it."""
