@@ -100,7 +100,27 @@ def make_params(tol, precision, iteration_limit=2**31 - 1, time_limit=float("inf
         p.refine_warm_start = REFINE_WARM
     if REFINE_CAP > 0:
         p.refine_inner_iteration_limit = REFINE_CAP
+    for k, v in TILES.get(precision, {}).items():
+        setattr(p, k, v)
     return p
+
+
+# Step-kernel tile parameters chosen by pdlp_tune, per precision (--tune).
+TILES = {}
+
+
+def tune(ctx, problems, precision, tol, trial_iterations=100):
+    """pdlp_tune on these LPs before anything is timed: sets TILES[precision]
+    and returns the tuner's report."""
+    t = ctx.tune(problems, make_params(tol, precision), trial_iterations)
+    TILES[precision] = {k: t[k] for k in ("lane_budget", "slice_unroll", "slice_waves")}
+    log(f"tuned {precision}: {TILES[precision]} ({t['kernel_us_per_iteration']:.1f} us vs "
+        f"{t['default_us_per_iteration']:.1f} us default per iteration, {t['candidates']} "
+        f"candidates, {t['seconds']:.1f} s)")
+    return {"chosen": TILES[precision], "kernel_us_per_iteration": round(t["kernel_us_per_iteration"], 2),
+            "default_us_per_iteration": round(t["default_us_per_iteration"], 2),
+            "candidates": t["candidates"], "trial_iterations": trial_iterations,
+            "seconds": round(t["seconds"], 2)}
 
 
 def batch_problems(count):
@@ -281,6 +301,11 @@ def run_batch(args, ctx, lib):
     problems, objectives = batch_problems(args.count)
     nnz = sum(p.constraint_matrix.nnz for p in problems)
     log(f"  {time.time() - t:.1f} s, total nnz {nnz}, total n {sum(p.num_variables for p in problems)}")
+    tuning = {}
+    if args.tune:
+        tuning[args.precision] = tune(ctx, problems, args.precision, args.tol)
+        if args.precision == "fp32" and not args.no_fp64_batch:
+            tuning["fp64"] = tune(ctx, problems, "fp64", args.tol)
     params = make_params(args.tol, args.precision, time_limit=args.time_limit)
     for w in range(args.warmup):
         log(f"warmup {w}")
@@ -305,6 +330,8 @@ def run_batch(args, ctx, lib):
     value = args.steps * len(problems) / dt
     extra = {"load_inclusive_solves_per_s": round(args.steps * len(problems) / (dt + load_s), 4),
              "load_s_per_step": round(load_s / args.steps, 3)}
+    if tuning:
+        extra["tuning"] = tuning
     if args.dump_instances:
         with open(args.dump_instances, "w") as f:
             json.dump([{"i": i, "n": p.num_variables, "nnz": int(p.constraint_matrix.nnz),
@@ -483,6 +510,7 @@ def run_single(args, ctx, lib, which):
     from cupdlp.generators import config_lp
     log(f"generating config {which} ...")
     prob, cert = config_lp(which)
+    tuning = tune(ctx, [prob], args.precision, args.tol) if args.tune else None
     params = make_params(args.tol, args.precision, iteration_limit=args.iters,
                          time_limit=args.time_limit)
     for w in range(args.warmup):
@@ -509,6 +537,8 @@ def run_single(args, ctx, lib, which):
              "relative_optimality_gap": ci.relative_optimality_gap,
              "primal_objective": ci.primal_objective,
              "refinement_rounds": out.refinement_rounds, "fp64_fallback": out.fp64_fallback}
+    if tuning:
+        extra["tuning"] = tuning
     ref = cert.objective if cert is not None else REFERENCE_OBJECTIVES.get(which)
     if ref is not None:
         extra["objective_reference"] = ref
@@ -571,6 +601,9 @@ def main():
     ap.add_argument("--serial-budget", type=float, default=60.0,
                     help="seconds of plain FP64 one-at-a-time solves (the batch: stops early "
                          "only if its LPs take longer; the served leg: half of it)")
+    ap.add_argument("--tune", type=int, default=1,
+                    help="batch and single-LP workloads: pick the step kernels' tile "
+                         "parameters with pdlp_tune before anything is timed (0 = defaults)")
     ap.add_argument("--profile-fallback", action="store_true",
                     help="batch: time the FP64 fallback engines' launches too (events on "
                          "every launch; slows the run)")

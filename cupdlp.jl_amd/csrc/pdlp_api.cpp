@@ -29,6 +29,7 @@ struct pdlp_ctx {
   Profiler prof;
   std::string err;
   bool prof_fallback_timed = false;  // pdlp_set_profiling(ctx, 2)
+  std::map<std::string, pdlp_tuning> tune_cache;  // pdlp_tune's choices per LP shape
   int scratch_rc = 0;             // set-up of the no-allocation region (pdlp_ctx_create)
   std::string scratch_err;
 };
@@ -207,6 +208,15 @@ static int check_params(const pdlp_params *p, std::string *err) {
     *err = "invalid refinement parameters";
     return 2;
   }
+  if (p->lane_budget != 0 && (p->lane_budget < 16 || p->lane_budget > 1024)) {
+    *err = "lane_budget must be 0 or in [16, 1024]";
+    return 2;
+  }
+  if ((p->slice_unroll != 0 && p->slice_unroll != 4 && p->slice_unroll != 8) ||
+      (p->slice_waves != 0 && p->slice_waves != 1 && p->slice_waves != 2 && p->slice_waves != 4)) {
+    *err = "slice_unroll must be 0, 4 or 8 and slice_waves 0, 1, 2 or 4";
+    return 2;
+  }
   return 0;
 }
 
@@ -219,7 +229,7 @@ int pdlp::prepare_instance(const pdlp_lp *lp, const pdlp_params *prm, HostInst *
   // Rescaling runs on the device unless disabled or the optional L2 pass is
   // requested (that pass is host-only).
   const bool dev = prm->device_preconditioning && !prm->l2_norm_rescaling;
-  int rc = load_problem(lp, &h->P, err, !dev);
+  int rc = load_problem(lp, &h->P, err, !dev, prm->lane_budget);
   if (rc) return rc;
   h->P.device_prep = dev;
   if (!dev) {
@@ -823,6 +833,107 @@ extern "C" int pdlp_batch_solve(pdlp_ctx *ctx, int32_t count, const pdlp_lp *lps
   rc = load_batch(ctx, count, lps, params, &b);
   if (rc == 0) rc = run_batch(b.get(), results, nullptr, nullptr, t0);
   return rc;
+}
+
+extern "C" int pdlp_tune(pdlp_ctx *ctx, int32_t count, const pdlp_lp *lps,
+                         const pdlp_params *params, int32_t trial_iterations, pdlp_tuning *out) {
+  if (!ctx) return 1;
+  ctx->err.clear();
+  if (!out || !params || count <= 0 || !lps || trial_iterations <= 0) {
+    ctx->err = "invalid tuning arguments";
+    return 1;
+  }
+  int rc = check_params(params, &ctx->err);
+  if (rc) return rc;
+  int64_t sn = 0, sm = 0, snz = 0;
+  for (int32_t i = 0; i < count; ++i) {
+    sn += lps[i].n;
+    sm += lps[i].m;
+    snz += lps[i].nnz;
+  }
+  const std::string key = std::to_string(count) + "/" + std::to_string(sn) + "/" +
+                          std::to_string(sm) + "/" + std::to_string(snz) + "/" +
+                          std::to_string(params->precision) + "/" +
+                          std::to_string(batch_group_count(count, params));
+  auto hit = ctx->tune_cache.find(key);
+  if (hit != ctx->tune_cache.end()) {
+    *out = hit->second;
+    out->candidates = 0;
+    out->seconds = 0.0;
+    return 0;
+  }
+  (void)hipSetDevice(ctx->device);
+  const double t0 = now_sec();
+  std::vector<std::vector<double>> xs(count), ys(count);
+  std::vector<pdlp_result> res(count);
+  for (int32_t i = 0; i < count; ++i) {
+    xs[i].assign((size_t)std::max<int64_t>(lps[i].n, 1), 0.0);
+    ys[i].assign((size_t)std::max<int64_t>(lps[i].m, 1), 0.0);
+    memset(&res[i], 0, sizeof res[i]);
+    res[i].x = xs[i].data();
+    res[i].y = ys[i].data();
+  }
+  int ntrials = 0;
+  // k_primal + k_dual time per launch pair over trial_iterations iterations
+  // with tolerances no point meets, timed with the kernels' own events
+  auto trial = [&](int lb, int u, int w, double *us) -> int {
+    pdlp_params p = *params;
+    p.lane_budget = lb;
+    p.slice_unroll = u;
+    p.slice_waves = w;
+    p.record_iteration_stats = 0;
+    p.termination_criteria.iteration_limit = trial_iterations;
+    p.termination_criteria.eps_optimal_absolute = 1e-300;
+    p.termination_criteria.eps_optimal_relative = 1e-300;
+    p.termination_criteria.time_sec_limit = INFINITY;
+    p.termination_criteria.kkt_matrix_pass_limit = INFINITY;
+    std::unique_ptr<pdlp_batch> b;
+    int r = load_batch(ctx, count, lps, &p, &b);
+    if (r) return r;
+    const pdlp_kernel_profile saved = ctx->prof.acc;
+    const bool saved_on = ctx->prof.enabled, saved_fb = ctx->prof_fallback_timed;
+    ctx->prof.enabled = true;
+    ctx->prof_fallback_timed = false;
+    memset(&ctx->prof.acc, 0, sizeof ctx->prof.acc);
+    r = run_batch(b.get(), res.data(), nullptr, nullptr, now_sec());
+    const pdlp_kernel_profile got = ctx->prof.acc;
+    ctx->prof.acc = saved;
+    ctx->prof.enabled = saved_on;
+    ctx->prof_fallback_timed = saved_fb;
+    if (r) return r;
+    ++ntrials;
+    *us = got.primal_launches > 0 ? (got.primal_ms + got.dual_ms) / (double)got.primal_launches * 1e3
+                                  : INFINITY;
+    return 0;
+  };
+  pdlp_tuning best{};
+  best.lane_budget = 64;
+  best.slice_unroll = 8;
+  best.slice_waves = 4;
+  rc = trial(64, 8, 4, &best.kernel_us_per_iteration);
+  if (rc) return rc;
+  best.default_us_per_iteration = best.kernel_us_per_iteration;
+  auto consider = [&](int lb, int u, int w) -> int {
+    double us = INFINITY;
+    int r = trial(lb, u, w, &us);
+    if (r) return r;
+    if (us < 0.99 * best.kernel_us_per_iteration) {
+      best.lane_budget = lb;
+      best.slice_unroll = u;
+      best.slice_waves = w;
+      best.kernel_us_per_iteration = us;
+    }
+    return 0;
+  };
+  for (int lb : {32, 128})
+    if ((rc = consider(lb, 8, 4))) return rc;
+  if ((rc = consider(best.lane_budget, 8, 2))) return rc;
+  if ((rc = consider(best.lane_budget, 4, best.slice_waves))) return rc;
+  best.candidates = ntrials;
+  best.seconds = now_sec() - t0;
+  ctx->tune_cache[key] = best;
+  *out = best;
+  return 0;
 }
 
 extern "C" int pdlp_batch_load(pdlp_ctx *ctx, int32_t count, const pdlp_lp *lps,

@@ -913,11 +913,12 @@ int Engine::launch_attempts(int64_t count, std::string *err) {
     hipEvent_t *ev = timed ? &prof_->events[4 * a] : nullptr;
     int e = launch_primal(s_, prec_, nps_, d_pslices_, npl_, d_pslices_ + nps_, d_longs_,
                           d_long_cnt_, d_long_part_, d_insts_, d_ctl_, d_ppart_,
-                          ev ? ev[0] : nullptr, ev ? ev[1] : nullptr);
+                          prm_.slice_unroll, prm_.slice_waves, ev ? ev[0] : nullptr,
+                          ev ? ev[1] : nullptr);
     if (!e)
       e = launch_dual(s_, prec_, nds_, d_dslices_, ndl_, d_dslices_ + nds_, d_longs_,
-                      d_long_cnt_, d_long_part_, d_insts_, d_ctl_, d_dpart_,
-                      ev ? ev[2] : nullptr, ev ? ev[3] : nullptr);
+                      d_long_cnt_, d_long_part_, d_insts_, d_ctl_, d_dpart_, prm_.slice_unroll,
+                      prm_.slice_waves, ev ? ev[2] : nullptr, ev ? ev[3] : nullptr);
     if (!e)
       e = launch_step(s_, (int)inst_.size(), d_insts_, d_ctl_, d_ppart_, d_dpart_, d_red_, d_grow_,
                       tab_len_, prm_.reduction_exponent, prm_.growth_exponent,

@@ -12,16 +12,20 @@ namespace pdlp {
 // Step kernels over the slice tables (one wave per slice): normal slices,
 // then the chunk slices of long rows (their partials follow the normal
 // slices' in part).
-// Optional events e0 / e1 are recorded at the start of the first and the end
-// of the last kernel's execution.
+// unroll (4 or 8: loads in flight per lane) and waves (1, 2 or 4 per
+// work-group) pick the variant of the normal-slice kernel; the variants give
+// the same results.  Optional events e0 / e1 are recorded at the start of
+// the first and the end of the last kernel's execution.
 int launch_primal(hipStream_t s, int prec, int nslices, const SliceDesc *slices, int nlong,
                   const SliceDesc *lslices, const LongRow *longs, int32_t *long_cnt,
                   double *long_part, const InstDev *insts, const Ctl *ctl, double *part,
-                  hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
+                  int unroll = 8, int waves = 4, hipEvent_t e0 = nullptr,
+                  hipEvent_t e1 = nullptr);
 int launch_dual(hipStream_t s, int prec, int nslices, const SliceDesc *slices, int nlong,
                 const SliceDesc *lslices, const LongRow *longs, int32_t *long_cnt,
                 double *long_part, const InstDev *insts, const Ctl *ctl, double *part,
-                hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
+                int unroll = 8, int waves = 4, hipEvent_t e0 = nullptr,
+                hipEvent_t e1 = nullptr);
 // Evaluation preparation over the row-block tables: pending weighted sums,
 // and A^T y of the current y when it is not valid.
 int launch_primal_flush(hipStream_t s, int prec, int nblocks, const BlockDesc *blocks,

@@ -276,14 +276,14 @@ __global__ __launch_bounds__(kThreads) void k_primal_flush(const BlockDesc *__re
 
 // One lane's share of a slice row: entries off + lane + 64 k, k < width,
 // gathered from xv, as a compensated sum.  Padding slots (column -1) are
-// skipped.  Entries go in groups of 8 whose index and value loads are all
-// issued before their gathers; the last group is predicated.
-template <typename T, typename V>
+// skipped.  Entries go in groups of U whose index and value loads are all
+// issued before their gathers; the last group is predicated.  U changes only
+// how many loads are in flight, not the order of the sum.
+template <typename T, typename V, int U = 8>
 __device__ __forceinline__ DD slice_lane_sum(const int32_t *__restrict__ scol,
                                              const T *__restrict__ sval,
                                              const V *__restrict__ xv, int off, int width,
                                              int lane) {
-  constexpr int U = 8;
   DD acc = dd_zero();
   const int32_t *cp = scol + off + lane;
   const T *vp = sval + off + lane;
@@ -410,7 +410,7 @@ __device__ __forceinline__ void primal_update(const PrimalView<T> &v, int j, con
 // slice's partial index p).
 // One wave per slice; each row's operands are loaded before the gathers of
 // its sum.
-template <typename T>
+template <typename T, int U = 8>
 __device__ __forceinline__ void primal_slice(const SliceDesc &sd, int lane,
                                              const InstDev *__restrict__ insts, const Ctl *ctl,
                                              double *part) {
@@ -425,7 +425,8 @@ __device__ __forceinline__ void primal_slice(const SliceDesc &sd, int lane,
   if (mine) r = primal_load(v, j);
   double s = 0.0;
   if (v.need) {
-    s = group_sum(slice_lane_sum(I.ts_col, (const T *)I.ts_val, v.yv, sd.off, sd.width, lane), sd.lg);
+    s = group_sum(slice_lane_sum<T, T, U>(I.ts_col, (const T *)I.ts_val, v.yv, sd.off, sd.width, lane),
+                  sd.lg);
     s = __shfl(s, (lane << sd.lg) & 63, 64);
   }
   DD acc = dd_zero();
@@ -437,14 +438,15 @@ __device__ __forceinline__ void primal_slice(const SliceDesc &sd, int lane,
   }
 }
 
-template <typename T>
-__global__ __launch_bounds__(kThreads) void k_primal(const SliceDesc *__restrict__ slices, int nsl,
-                                                     const InstDev *__restrict__ insts,
-                                                     const Ctl *__restrict__ ctl,
-                                                     double *__restrict__ part) {
-  const int wv = (int)blockIdx.x * (kThreads / 64) + ((int)threadIdx.x >> 6);
+// One wave per slice; W waves per work-group, U loads in flight per lane.
+template <typename T, int U, int W>
+__global__ __launch_bounds__(64 * W) void k_primal(const SliceDesc *__restrict__ slices, int nsl,
+                                                   const InstDev *__restrict__ insts,
+                                                   const Ctl *__restrict__ ctl,
+                                                   double *__restrict__ part) {
+  const int wv = (int)blockIdx.x * W + ((int)threadIdx.x >> 6);
   if (wv >= nsl) return;
-  primal_slice<T>(slices[wv], threadIdx.x & 63, insts, ctl, part);
+  primal_slice<T, U>(slices[wv], threadIdx.x & 63, insts, ctl, part);
 }
 
 // The chunk slices of rows longer than kSellChunk entries: each wave sums one
@@ -657,7 +659,7 @@ __device__ __forceinline__ void dual_update(const DualView<T> &v, int i, const D
 // Dual half-step over the slices of A: the product, the pending weighted-sum
 // update of y and A x, the projected dual step and the partial sums of dy^2
 // and Adx*dy of each slice (part[4p..4p+3] for its partial index p).
-template <typename T>
+template <typename T, int U = 8>
 __device__ __forceinline__ void dual_slice(const SliceDesc &sd, int lane,
                                            const InstDev *__restrict__ insts, const Ctl *ctl,
                                            double *part) {
@@ -670,7 +672,8 @@ __device__ __forceinline__ void dual_slice(const SliceDesc &sd, int lane,
   const bool mine = lane < sd.nrows;
   DualRow r{};
   if (mine) r = dual_load(v, i);
-  double prod = group_sum(slice_lane_sum(I.as_col, (const T *)I.as_val, v.dxv, sd.off, sd.width, lane), sd.lg);
+  double prod = group_sum(
+      slice_lane_sum<T, T, U>(I.as_col, (const T *)I.as_val, v.dxv, sd.off, sd.width, lane), sd.lg);
   prod = __shfl(prod, (lane << sd.lg) & 63, 64);
   DD acc_dy = dd_zero(), acc_int = dd_zero();
   if (mine) dual_update(v, i, r, prod, acc_dy, acc_int);
@@ -684,14 +687,14 @@ __device__ __forceinline__ void dual_slice(const SliceDesc &sd, int lane,
   }
 }
 
-template <typename T>
-__global__ __launch_bounds__(kThreads) void k_dual(const SliceDesc *__restrict__ slices, int nsl,
-                                                   const InstDev *__restrict__ insts,
-                                                   const Ctl *__restrict__ ctl,
-                                                   double *__restrict__ part) {
-  const int wv = (int)blockIdx.x * (kThreads / 64) + ((int)threadIdx.x >> 6);
+template <typename T, int U, int W>
+__global__ __launch_bounds__(64 * W) void k_dual(const SliceDesc *__restrict__ slices, int nsl,
+                                                 const InstDev *__restrict__ insts,
+                                                 const Ctl *__restrict__ ctl,
+                                                 double *__restrict__ part) {
+  const int wv = (int)blockIdx.x * W + ((int)threadIdx.x >> 6);
   if (wv >= nsl) return;
-  dual_slice<T>(slices[wv], threadIdx.x & 63, insts, ctl, part);
+  dual_slice<T, U>(slices[wv], threadIdx.x & 63, insts, ctl, part);
 }
 
 // Chunk slices of long rows of A (see k_primal_long).
@@ -1411,13 +1414,40 @@ static void launch_ev(F kernel, dim3 g, dim3 b, hipStream_t s, hipEvent_t e0, hi
     hipLaunchKernelGGL(kernel, g, b, 0, s, args...);
 }
 
+// The slice-kernel variant: U in {4, 8} loads in flight per lane, W in
+// {1, 2, 4} waves per work-group (anything else: 8 and 4).  The variants
+// compute the same sums in the same order.
+using SliceKernel = void (*)(const SliceDesc *, int, const InstDev *, const Ctl *, double *);
+template <typename T, int U>
+static SliceKernel primal_variant(int w) {
+  return w == 1 ? &k_primal<T, U, 1> : w == 2 ? &k_primal<T, U, 2> : &k_primal<T, U, 4>;
+}
+template <typename T, int U>
+static SliceKernel dual_variant(int w) {
+  return w == 1 ? &k_dual<T, U, 1> : w == 2 ? &k_dual<T, U, 2> : &k_dual<T, U, 4>;
+}
+static int variant_waves(int waves) { return waves == 1 || waves == 2 ? waves : 4; }
+static SliceKernel primal_kernel(int prec, int unroll, int waves) {
+  const int w = variant_waves(waves);
+  if (prec == 0) return unroll == 4 ? primal_variant<double, 4>(w) : primal_variant<double, 8>(w);
+  return unroll == 4 ? primal_variant<float, 4>(w) : primal_variant<float, 8>(w);
+}
+static SliceKernel dual_kernel(int prec, int unroll, int waves) {
+  const int w = variant_waves(waves);
+  if (prec == 0) return unroll == 4 ? dual_variant<double, 4>(w) : dual_variant<double, 8>(w);
+  return unroll == 4 ? dual_variant<float, 4>(w) : dual_variant<float, 8>(w);
+}
+static inline unsigned grid_of_waves(int nsl, int w) {
+  return (unsigned)(nsl > 0 ? (nsl + w - 1) / w : 0);
+}
+
 // Normal slices, then the chunk slices of long rows; each slice writes its
 // partial sums at its own partial index.  e0 / e1 (optional) bracket both
 // launches.
 int launch_primal(hipStream_t s, int prec, int nslices, const SliceDesc *slices, int nlong,
                   const SliceDesc *lslices, const LongRow *longs, int32_t *long_cnt,
                   double *long_part, const InstDev *insts, const Ctl *ctl, double *part,
-                  hipEvent_t e0, hipEvent_t e1) {
+                  int unroll, int waves, hipEvent_t e0, hipEvent_t e1) {
   const dim3 b(kThreads);
   if (nslices <= 0 && nlong <= 0) {
     hipError_t r = e0 ? hipEventRecord(e0, s) : hipSuccess;
@@ -1426,10 +1456,9 @@ int launch_primal(hipStream_t s, int prec, int nslices, const SliceDesc *slices,
   }
   if (nslices > 0) {
     hipEvent_t stop = nlong > 0 ? nullptr : e1;
-    if (prec == 0)
-      launch_ev(k_primal<double>, dim3(grid_of_slices(nslices)), b, s, e0, stop, slices, nslices, insts, ctl, part);
-    else
-      launch_ev(k_primal<float>, dim3(grid_of_slices(nslices)), b, s, e0, stop, slices, nslices, insts, ctl, part);
+    const int w = variant_waves(waves);
+    launch_ev(primal_kernel(prec, unroll, waves), dim3(grid_of_waves(nslices, w)), dim3(64 * w),
+              s, e0, stop, slices, nslices, insts, ctl, part);
   }
   if (nlong > 0) {
     hipEvent_t start = nslices > 0 ? nullptr : e0;
@@ -1444,7 +1473,7 @@ int launch_primal(hipStream_t s, int prec, int nslices, const SliceDesc *slices,
 int launch_dual(hipStream_t s, int prec, int nslices, const SliceDesc *slices, int nlong,
                 const SliceDesc *lslices, const LongRow *longs, int32_t *long_cnt,
                 double *long_part, const InstDev *insts, const Ctl *ctl, double *part,
-                hipEvent_t e0, hipEvent_t e1) {
+                int unroll, int waves, hipEvent_t e0, hipEvent_t e1) {
   const dim3 b(kThreads);
   if (nslices <= 0 && nlong <= 0) {
     hipError_t r = e0 ? hipEventRecord(e0, s) : hipSuccess;
@@ -1453,10 +1482,9 @@ int launch_dual(hipStream_t s, int prec, int nslices, const SliceDesc *slices, i
   }
   if (nslices > 0) {
     hipEvent_t stop = nlong > 0 ? nullptr : e1;
-    if (prec == 0)
-      launch_ev(k_dual<double>, dim3(grid_of_slices(nslices)), b, s, e0, stop, slices, nslices, insts, ctl, part);
-    else
-      launch_ev(k_dual<float>, dim3(grid_of_slices(nslices)), b, s, e0, stop, slices, nslices, insts, ctl, part);
+    const int w = variant_waves(waves);
+    launch_ev(dual_kernel(prec, unroll, waves), dim3(grid_of_waves(nslices, w)), dim3(64 * w), s,
+              e0, stop, slices, nslices, insts, ctl, part);
   }
   if (nlong > 0) {
     hipEvent_t start = nslices > 0 ? nullptr : e0;

@@ -98,7 +98,8 @@ static double norminf(const std::vector<double> &v) {
   return mx;
 }
 
-int load_problem(const pdlp_lp *lp, Prepared *P, std::string *err, bool build_csr) {
+int load_problem(const pdlp_lp *lp, Prepared *P, std::string *err, bool build_csr,
+                 int lane_budget) {
   int rc = validate_lp(lp, err);
   if (rc) return rc;
   const int64_t n = lp->n, m = lp->m, nnz = lp->nnz, neq = lp->num_eq;
@@ -162,8 +163,8 @@ int load_problem(const pdlp_lp *lp, Prepared *P, std::string *err, bool build_cs
   P->a_ptr.assign(m + 1, 0);
   for (int64_t k = 0; k < nnz; ++k) P->a_ptr[P->t_col[k] + 1]++;
   for (int64_t i = 0; i < m; ++i) P->a_ptr[i + 1] += P->a_ptr[i];
-  build_slices(P->t_ptr.data(), n, &P->ps);
-  build_slices(P->a_ptr.data(), m, &P->ds);
+  build_slices(P->t_ptr.data(), n, &P->ps, lane_budget);
+  build_slices(P->a_ptr.data(), m, &P->ds, lane_budget);
   if (P->ps.slots > INT32_MAX || P->ds.slots > INT32_MAX) {
     if (err) *err = "the constraint matrix is too large for 32-bit slot offsets";
     return 3;
@@ -188,7 +189,8 @@ int load_problem(const pdlp_lp *lp, Prepared *P, std::string *err, bool build_cs
   return 0;
 }
 
-void build_slices(const int32_t *ptr, int64_t R, SliceTable *out) {
+void build_slices(const int32_t *ptr, int64_t R, SliceTable *out, int lane_budget) {
+  const int64_t L = lane_budget > 0 ? lane_budget : kSellWidth;
   out->sl.clear();
   out->nchunks.clear();
   int64_t off = 0, r = 0;
@@ -208,8 +210,8 @@ void build_slices(const int32_t *ptr, int64_t R, SliceTable *out) {
       continue;
     }
     int lg = 0;
-    while (lg < 6 && len0 > ((int64_t)kSellWidth << lg)) ++lg;
-    const int64_t cap = lg == 6 ? kSellChunk : (int64_t)kSellWidth << lg, maxrows = 64 >> lg;
+    while (lg < 6 && len0 > (L << lg)) ++lg;
+    const int64_t cap = lg == 6 ? kSellChunk : L << lg, maxrows = 64 >> lg;
     int64_t nr = 0, w = 0;
     while (r + nr < R && nr < maxrows) {
       const int64_t len = ptr[r + nr + 1] - ptr[r + nr];

@@ -30,7 +30,7 @@ struct SliceTable {
 // count 2^lg chosen by its first row (and at most 64 >> lg rows); a row of
 // more than 64 * kSellWidth entries is a slice of its own, one of more than
 // kSellChunk entries becomes chunk slices.
-void build_slices(const int32_t *ptr, int64_t R, SliceTable *out);
+void build_slices(const int32_t *ptr, int64_t R, SliceTable *out, int lane_budget = 0);
 
 // An LP after rescale_problem, in the device layouts (0-based, int32).
 struct Prepared {
@@ -65,7 +65,8 @@ int validate_lp(const pdlp_lp *lp, std::string *err);
 // Loads the original problem and its cached norms.  With build_csr the CSR
 // of A is built on the host; otherwise only its row pointers (the device
 // builds the rest).  Rescaling is applied by rescale_host or on the device.
-int load_problem(const pdlp_lp *lp, Prepared *P, std::string *err, bool build_csr = true);
+int load_problem(const pdlp_lp *lp, Prepared *P, std::string *err, bool build_csr = true,
+                 int lane_budget = 0);
 
 // rescale_problem (src/preprocess.jl:619-675) on the host: Ruiz-inf,
 // optional L2, optional Pock-Chambolle.  Updates t_val/a_val and the vectors.

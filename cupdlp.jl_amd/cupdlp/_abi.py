@@ -79,7 +79,10 @@ class pdlp_params(C.Structure):
         ("refine_early_fallback", C.c_int32),
         ("debug_numerical_error_attempt", C.c_int32),
         ("refine_warm_start", C.c_int32),
-        ("reserved", C.c_int32 * 2),
+        ("lane_budget", C.c_int32),
+        ("slice_unroll", C.c_int32),
+        ("slice_waves", C.c_int32),
+        ("reserved", C.c_int32 * 3),
     ]
 
 
@@ -142,6 +145,13 @@ class pdlp_kernel_profile(C.Structure):
                 ("fallback_primal_bytes", C.c_double), ("fallback_dual_bytes", C.c_double)]
 
 
+class pdlp_tuning(C.Structure):
+    _fields_ = [("lane_budget", C.c_int32), ("slice_unroll", C.c_int32),
+                ("slice_waves", C.c_int32), ("candidates", C.c_int32),
+                ("kernel_us_per_iteration", C.c_double),
+                ("default_us_per_iteration", C.c_double), ("seconds", C.c_double)]
+
+
 STATS_CB = C.CFUNCTYPE(None, C.POINTER(pdlp_iteration_stats), C.c_void_p)
 
 # Order of the sizes reported by pdlp_abi_layout().
@@ -157,7 +167,7 @@ EXPORTED = (
     "pdlp_abi_layout", "pdlp_set_profiling", "pdlp_get_profile",
     "pdlp_batch_load", "pdlp_batch_run", "pdlp_batch_free",
     "pdlp_read_mps", "pdlp_mps_free", "pdlp_serve_submit_limits", "pdlp_serve_cancel",
-    "pdlp_solve_noalloc",
+    "pdlp_solve_noalloc", "pdlp_tune",
 )
 
 
@@ -176,6 +186,9 @@ def declare(lib):
     lib.pdlp_solve.restype = C.c_int
     lib.pdlp_solve_noalloc.argtypes = lib.pdlp_solve.argtypes
     lib.pdlp_solve_noalloc.restype = C.c_int
+    lib.pdlp_tune.argtypes = [C.c_void_p, C.c_int32, C.POINTER(pdlp_lp), C.POINTER(pdlp_params),
+                              C.c_int32, C.POINTER(pdlp_tuning)]
+    lib.pdlp_tune.restype = C.c_int
     lib.pdlp_batch_solve.argtypes = [C.c_void_p, C.c_int32, C.POINTER(pdlp_lp),
                                      C.POINTER(pdlp_params), C.POINTER(pdlp_result)]
     lib.pdlp_batch_solve.restype = C.c_int

@@ -96,6 +96,9 @@ def to_c_params(p: PdhgParameters) -> _abi.pdlp_params:
     cp.refine_early_fallback = int(p.refine_early_fallback)
     cp.debug_numerical_error_attempt = int(p.debug_numerical_error_attempt)
     cp.refine_warm_start = int(p.refine_warm_start)
+    cp.lane_budget = int(p.lane_budget)
+    cp.slice_unroll = int(p.slice_unroll)
+    cp.slice_waves = int(p.slice_waves)
     return cp
 
 
@@ -204,6 +207,27 @@ class Context:
         if rc != 0:
             raise PdlpError(rc, self.error())
         return out
+
+    def tune(self, problems, params, trial_iterations=100):
+        """pdlp_tune: the step-kernel tile parameters for these LPs (batched
+        with ``params``) from timed trials; a dict whose lane_budget,
+        slice_unroll and slice_waves go into PdhgParameters.  Cached per LP
+        shape on this context."""
+        k = len(problems)
+        lps = (_abi.pdlp_lp * k)()
+        keeps = []
+        for i, p in enumerate(problems):
+            lp, keep = to_c_lp(p)
+            lps[i] = lp
+            keeps.append(keep)
+        cp = to_c_params(params)
+        out = _abi.pdlp_tuning()
+        rc = self.lib.pdlp_tune(self.handle, k, lps, C.byref(cp), int(trial_iterations),
+                                C.byref(out))
+        del keeps
+        if rc != 0:
+            raise PdlpError(rc, self.error())
+        return {f: getattr(out, f) for f, _ in _abi.pdlp_tuning._fields_}
 
     def batch_solve(self, problems, params):
         k = len(problems)

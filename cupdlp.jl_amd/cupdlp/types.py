@@ -223,6 +223,9 @@ class PdhgParameters:
     refine_early_fallback: int = 0
     debug_numerical_error_attempt: int = 0  # test-only fault injection (movement = 0)
     refine_warm_start: int = 3
+    lane_budget: int = 0    # step-kernel tiles (0 = default; see pdlp_tune)
+    slice_unroll: int = 0
+    slice_waves: int = 0
 
 
 @dataclass

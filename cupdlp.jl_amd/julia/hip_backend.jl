@@ -171,7 +171,10 @@ struct CParams
     refine_early_fallback::Int32
     debug_numerical_error_attempt::Int32
     refine_warm_start::Int32
-    reserved::NTuple{2,Int32}
+    lane_budget::Int32
+    slice_unroll::Int32
+    slice_waves::Int32
+    reserved::NTuple{3,Int32}
 end
 
 struct CConvergence
@@ -264,7 +267,7 @@ function _c_params(p::PdhgParameters)
                    p.refine_max_outer, p.refine_inner_tolerance, p.refine_max_scale,
                    Int32(p.device_preconditioning), p.refine_inner_iteration_limit,
                    p.batch_groups, p.refine_early_fallback, 0, p.refine_warm_start,
-                   ntuple(_ -> Int32(0), 2))
+                   0, 0, 0, ntuple(_ -> Int32(0), 3))  # tiles: the library's defaults
 end
 
 function _stats(s::CIterationStats)

@@ -180,7 +180,17 @@ typedef struct pdlp_params {
                                         size (bit 2) instead of fresh ones; 0 = fresh;
                                         default 3 (fewer stalled refinements on the
                                         512-LP batch: 16 vs 21) */
-  int32_t reserved[2];
+  /* Tile parameters of the step kernels (pdlp_tune picks them by timed
+   * trials; 0 = the default): */
+  int32_t lane_budget;               /* most entries of a row one lane of a slice sums
+                                        (the lanes per row follow; 16..1024, default 64);
+                                        changes the slicing, hence the order of a row's
+                                        compensated sum */
+  int32_t slice_unroll;              /* loads in flight per lane: 4 or 8 (default 8) */
+  int32_t slice_waves;               /* waves per work-group of the slice kernels: 1, 2
+                                        or 4 (default 4); unroll and waves change no
+                                        result */
+  int32_t reserved[3];
 } pdlp_params;
 
 /* ConvergenceInformation — src/solve_log.jl:51-155 (same field order). */
@@ -401,6 +411,31 @@ typedef struct pdlp_kernel_profile {
 int pdlp_set_profiling(pdlp_ctx *ctx, int32_t enable);
 /* Copies the accumulated profile and resets it. */
 int pdlp_get_profile(pdlp_ctx *ctx, pdlp_kernel_profile *out);
+
+/* Tile parameters of the step kernels chosen by timed trials on the device
+ * (extension; the reference hard-wires its choices: cuSPARSE's CSR_ALG2 at
+ * src/primal_dual_hybrid_gradient_gpu.jl:225, 307 and ThreadPerBlock = 128
+ * at src/cuPDLP.jl:34). */
+typedef struct pdlp_tuning {
+  int32_t lane_budget;              /* the fields to copy into pdlp_params */
+  int32_t slice_unroll;
+  int32_t slice_waves;
+  int32_t candidates;               /* configurations timed (0: taken from the cache) */
+  double kernel_us_per_iteration;   /* k_primal + k_dual per launch pair, the choice */
+  double default_us_per_iteration;  /* the same with the defaults (64, 8, 4) */
+  double seconds;                   /* wall time of the trials */
+} pdlp_tuning;
+/* Solves the LPs (batched as pdlp_batch_solve would, with params) for
+ * trial_iterations iterations per candidate and times their step kernels
+ * with HIP events: lane budget 32 / 64 / 128, then 2 or 4 waves per
+ * work-group, then 4 or 8 loads in flight, each step keeping a candidate
+ * only if it is at least 1 % faster.  Writes the fastest to *out.  The
+ * choice is cached on the context per LP shape (count, total n, m and
+ * nonzeros, precision, size groups): a later call for the same shape
+ * returns it at once.  Unroll and waves change no result; the lane budget
+ * changes the order of each row's compensated sum. */
+int pdlp_tune(pdlp_ctx *ctx, int32_t count, const pdlp_lp *lps, const pdlp_params *params,
+              int32_t trial_iterations, pdlp_tuning *out);
 
 /* Library identification. */
 int pdlp_abi_version(void);
