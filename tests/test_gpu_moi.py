@@ -7,6 +7,7 @@ import math
 import numpy as np
 import pytest
 
+import moi_cases
 import oracle as O
 from cupdlp import Optimizer, construct_termination_criteria
 
@@ -67,3 +68,29 @@ def test_optimizer_gpu_statuses(gpu_ctx, kind):
                     opt.output.iteration_count))
     assert res[0] == res[1]
     assert res[0][0] == ("INFEASIBLE" if kind == "infeasible" else "DUAL_INFEASIBLE")
+
+
+def _with_tolerance(opt):
+    opt.silent = True
+    opt.set_raw("termination_criteria",
+                construct_termination_criteria(eps_optimal_absolute=1e-9, eps_optimal_relative=1e-9))
+    return opt
+
+
+@pytest.mark.parametrize("name", list(moi_cases.CASES))
+def test_moi_case_gpu_matches_oracle(gpu_ctx, name):
+    """Every stage of every known-answer model (interval and vector rows,
+    modifications, deletions, infeasible / unbounded with offsets, MAX sense)
+    solved on the GPU is bitwise the oracle-backed front end's, statuses
+    included."""
+    print(f"case: MOI known-answer model {name}, GPU vs oracle", flush=True)
+    gpu = moi_cases.CASES[name](_with_tolerance(Optimizer()))
+    ref = moi_cases.CASES[name](_with_tolerance(Optimizer(solve_fn=_oracle_solve)))
+    assert len(gpu) == len(ref)
+    for k, (g, c) in enumerate(zip(gpu, ref)):
+        assert g.termination == c.termination == g.expected_status, (name, k)
+        assert (g.primal_status, g.dual_status) == (c.primal_status, c.dual_status)
+        assert g.output.iteration_count == c.output.iteration_count, (name, k)
+        assert np.array_equal(g.output.primal_solution, c.output.primal_solution), (name, k)
+        assert np.array_equal(g.output.dual_solution, c.output.dual_solution), (name, k)
+        assert g.objective == c.objective and g.duals == c.duals
