@@ -3,7 +3,8 @@
 * The n = 1,000,000 Zipf-degree LP (m = 500,000, ~12.5M nonzeros, row and
   column scales over 6 and 4 decades): its first 96 iterations in FP64,
   bitwise against the CPU oracle (iterates, step sizes, primal weights,
-  restart choices, every KKT statistic).
+  restart choices, every KKT statistic); and its generator at n = 20,000
+  solved to 1e-4, bitwise the oracle's whole 474k-iteration solve.
 * The 512-LP batch (seeds 0..511, n log-uniform in [500, 200,000]) solved
   batched in FP32 + FP64 refinement to its tolerance 1e-6: every instance
   OPTIMAL; the reference's optimality test (src/termination.jl:142-170, L2,
@@ -44,6 +45,34 @@ def params(tol, iteration_limit=2**31 - 1, precision=Precision.FP64, **kw):
 def rel_kkt(prob, x, y):
     """The reference's relative KKT measure of (x, y) (cupdlp.kkt, numpy)."""
     return relative_kkt(prob, x, y)
+
+
+def test_zipf_lp_n20000_to_tolerance(gpu_ctx):
+    """The n = 1,000,000 LP's generator at n = 20,000, solved to 1e-4 in
+    FP64: OPTIMAL after exactly the oracle's 473,856-odd iterations with
+    bitwise the oracle's final objectives (tests/golden/zipf_scaled_oracle.json,
+    a 5-minute oracle run), and within the tolerance of HiGHS's optimum."""
+    import json
+    import os
+    here = os.path.dirname(os.path.abspath(__file__))
+    with open(os.path.join(here, "golden", "zipf_scaled_oracle.json")) as f:
+        ref = json.load(f)
+    with open(os.path.join(here, "golden", "highs_objectives.json")) as f:
+        h = json.load(f)["objectives"]["zipf_scaled_n20000"]["objective"]
+    say(f"case: Zipf-degree LP n=20,000 to {ref['tolerance']} in FP64 "
+        f"(the oracle: {ref['iteration_count']} iterations)")
+    prob, _ = config_lp(1, scale=0.02)
+    g = gpu_ctx.solve(prob, params(ref["tolerance"]))
+    ci = g.iteration_stats[-1].convergence_information[0]
+    say(f"  GPU: {g.termination_reason.name}, {g.iteration_count} iterations, "
+        f"{g.solve_time_sec:.1f} s")
+    assert int(g.termination_reason) == ref["termination_reason"]
+    assert g.termination_reason == TR.TERMINATION_REASON_OPTIMAL
+    assert g.iteration_count == ref["iteration_count"]
+    assert ci.primal_objective == ref["primal_objective"]
+    assert ci.dual_objective == ref["dual_objective"]
+    assert abs(ci.primal_objective - h) <= ref["tolerance"] * (1 + abs(h))
+    assert rel_kkt(prob, g.primal_solution, g.dual_solution) <= ref["tolerance"]
 
 
 def test_c1_full_size_first_96_iterations(gpu_ctx):

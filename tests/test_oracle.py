@@ -91,6 +91,23 @@ def test_generator_certificate_and_highs(name, make, tol):
     assert ci.relative_l2_dual_residual < 5 * tol
 
 
+def test_zipf_generator_certificate_is_optimal():
+    """The Zipf-degree generator of the n = 1,000,000 benchmark LP, at
+    n = 20,000: its certificate's objective is HiGHS's optimum, so the many
+    PDHG iterations that LP needs (about 474k to 1e-4 here, in the oracle)
+    are the algorithm's, not a generator defect."""
+    prob, cert = config_lp(1, scale=0.02)
+    h = HIGHS["zipf_scaled_n20000"]["objective"]
+    assert prob.num_variables == HIGHS["zipf_scaled_n20000"]["n"]
+    assert prob.constraint_matrix.nnz == HIGHS["zipf_scaled_n20000"]["nnz"]
+    assert abs(cert.objective - h) <= 1e-9 * (1 + abs(h))
+    # the certificate point itself is feasible and complementary in FP64
+    ci, _ = O.point_stats(prob, cert.x, cert.y, 1e-9, 1e-9)
+    assert ci.relative_l2_primal_residual < 1e-9
+    assert ci.relative_l2_dual_residual < 1e-9
+    assert ci.relative_optimality_gap < 1e-9
+
+
 def test_transportation_against_highs():
     prob = transportation_lp(60, 60, seed=5)
     out = O.solve(prob, params(1e-8))
