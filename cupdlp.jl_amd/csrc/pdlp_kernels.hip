@@ -822,6 +822,11 @@ __global__ __launch_bounds__(kStepThreads) void k_step(const InstDev *__restrict
 // Each accumulator is a compensated sum or an exact maximum; partials hold
 // (s, e) pairs per block.
 
+// Row blocks per work-group of k_eval: its 25 / 22 compensated accumulators
+// are reduced once per run of an instance's blocks in the group rather than
+// once per block (a row block holds about 128 rows of a random LP).
+constexpr int kEvalGroup = 8;
+
 constexpr uint32_t kEvalMaxN = (1u << 1) | (1u << 3) | (1u << 6) | (1u << 8) | (1u << 10) | (1u << 11);
 constexpr uint32_t kEvalMaxM = (1u << 1) | (1u << 3) | (1u << 5) | (1u << 7);
 __device__ __forceinline__ bool eval_is_max_n(int k) { return (kEvalMaxN >> k) & 1u; }
@@ -879,138 +884,165 @@ __device__ __forceinline__ void block_rows(const BlockDesc &bd, int *r0, int *r1
   }
 }
 
+// Row blocks whose rows k_eval summed into the partial of an earlier block
+// of the same instance get zero partials (k_eval_finish adds them exactly).
+__device__ __forceinline__ void zero_partials(double *ep, int b0, int b1, int K) {
+  double *p = ep + (size_t)b0 * 2 * K;
+  const int count = (b1 - b0) * 2 * K;
+  for (int i = (int)threadIdx.x; i < count; i += kThreads) p[i] = 0.0;
+}
+
 template <typename T>
 __global__ __launch_bounds__(kThreads) void k_eval(const BlockDesc *__restrict__ pblocks, int np,
-                                                   const BlockDesc *__restrict__ dblocks,
+                                                   const BlockDesc *__restrict__ dblocks, int nd,
                                                    const InstDev *__restrict__ insts,
                                                    const Ctl *__restrict__ ctl,
                                                    double *__restrict__ epn,
                                                    double *__restrict__ epm) {
   __shared__ double lds[2 * kEvalN * (kThreads / 64)];
-  const bool nside = (int)blockIdx.x < np;
-  const int blk = nside ? (int)blockIdx.x : (int)blockIdx.x - np;
-  const BlockDesc bd = nside ? pblocks[blk] : dblocks[blk];
-  const Ctl *C = &ctl[bd.inst];
-  if (!C->eval_flag) return;
-  const InstDev &I = insts[bd.inst];
-  const int cur = C->cur;
-  const bool usecur = C->use_current != 0;
-  const double wsum = C->wsum;
-  int r0, r1;
-  block_rows(bd, &r0, &r1);
-  if (nside) {
-    DD acc[kEvalN];
+  const int nsp = (np + kEvalGroup - 1) / kEvalGroup;
+  const bool nside = (int)blockIdx.x < nsp;
+  const int sb = nside ? (int)blockIdx.x : (int)blockIdx.x - nsp;
+  const BlockDesc *__restrict__ blocks = nside ? pblocks : dblocks;
+  const int bend = min(nside ? np : nd, (sb + 1) * kEvalGroup);
+  for (int b0 = sb * kEvalGroup; b0 < bend;) {
+    // the run [bfirst, blast) of this group's row blocks that belong to one instance
+    const int bfirst = b0;
+    const int qi = blocks[bfirst].inst;
+    int blast = bfirst + 1;
+    while (blast < bend && blocks[blast].inst == qi) ++blast;
+    b0 = blast;
+    const Ctl *C = &ctl[qi];
+    if (!C->eval_flag) continue;
+    const InstDev &I = insts[qi];
+    const int cur = C->cur;
+    const bool usecur = C->use_current != 0;
+    const double wsum = C->wsum;
+    if (nside) {
+      DD acc[kEvalN];
 #pragma unroll
-    for (int k = 0; k < kEvalN; ++k) acc[k] = dd_zero();
-    const void *xcur = cur ? I.x1 : I.x0;
-    for (int j = r0 + (int)threadIdx.x; j < r1; j += kThreads) {
-      const double xc = ld<T>(xcur, j);
-      const double cs = ld<T>(I.c, j), ls = ld<T>(I.l, j), us = ld<T>(I.u, j);
-      const double ax = usecur ? xc : ld<double>(I.sx, j) / wsum;
-      const double ag = usecur ? ld<T>(I.gbuf, j) : (-ld<double>(I.saty, j)) / wsum + cs;
-      const double gnew = cs - ld<T>(I.aty, j);
-      st<T>(I.avg_x, j, ax);
-      st<T>(I.avg_g, j, ag);
-      st<T>(I.gbuf, j, gnew);
-      // original space
-      const double Dj = I.D[j];
-      const double ox = ax / Dj, og = ag * Dj;
-      const double ol = I.ol[j], ou = I.ou[j], oc = I.oc[j];
-      const double fl = isfinite(ol) ? 1.0 : 0.0, fu = isfinite(ou) ? 1.0 : 0.0;
-      const double lvv = jl_max(ol - ox, 0.0), uvv = jl_max(ox - ou, 0.0);
-      dd_add_prod(acc[0], lvv, lvv);
-      dd_add_prod(acc[0], uvv, uvv);
-      acc[1].s = jl_max(acc[1].s, jl_max(lvv, uvv));
-      dd_add_prod(acc[2], oc, ox);
-      acc[3].s = jl_max(acc[3].s, fabs(ox));
-      dd_add_prod(acc[4], ox, ox);
-      const double r = jl_max(og, 0.0) * fl + jl_min(og, 0.0) * fu;
-      const double rv = og - r;
-      dd_add_prod(acc[5], rv, rv);
-      acc[6].s = jl_max(acc[6].s, fabs(rv));
-      dd_add(acc[7], contrib(ol, ou, r));
-      const double g2 = og - oc;
-      const double r2 = jl_max(g2, 0.0) * fl + jl_min(g2, 0.0) * fu;
-      const double rv2 = g2 - r2;
-      acc[8].s = jl_max(acc[8].s, fabs(rv2));
-      dd_add(acc[9], contrib(ol, ou, r2));
-      acc[10].s = jl_max(acc[10].s, fabs(r2));
-      acc[11].s = jl_max(acc[11].s, jl_max(fl != 0.0 ? jl_max(-ox, 0.0) : 0.0,
-                                            fu != 0.0 ? jl_max(ox, 0.0) : 0.0));
-      // restart candidates in the scaled space: current, average, last restart
-      const double xs3[3] = {xc, ax, ld<T>(I.lx, j)};
-      const double gs3[3] = {gnew, ag, ld<T>(I.lg, j)};
-      const double sfl = isfinite(ls) ? 1.0 : 0.0, sfu = isfinite(us) ? 1.0 : 0.0;
+      for (int k = 0; k < kEvalN; ++k) acc[k] = dd_zero();
+      const void *xcur = cur ? I.x1 : I.x0;
+      for (int bb = bfirst; bb < blast; ++bb) {
+        int r0, r1;
+        block_rows(blocks[bb], &r0, &r1);
+        for (int j = r0 + (int)threadIdx.x; j < r1; j += kThreads) {
+          const double xc = ld<T>(xcur, j);
+          const double cs = ld<T>(I.c, j), ls = ld<T>(I.l, j), us = ld<T>(I.u, j);
+          const double ax = usecur ? xc : ld<double>(I.sx, j) / wsum;
+          const double ag = usecur ? ld<T>(I.gbuf, j) : (-ld<double>(I.saty, j)) / wsum + cs;
+          const double gnew = cs - ld<T>(I.aty, j);
+          st<T>(I.avg_x, j, ax);
+          st<T>(I.avg_g, j, ag);
+          st<T>(I.gbuf, j, gnew);
+          // original space
+          const double Dj = I.D[j];
+          const double ox = ax / Dj, og = ag * Dj;
+          const double ol = I.ol[j], ou = I.ou[j], oc = I.oc[j];
+          const double fl = isfinite(ol) ? 1.0 : 0.0, fu = isfinite(ou) ? 1.0 : 0.0;
+          const double lvv = jl_max(ol - ox, 0.0), uvv = jl_max(ox - ou, 0.0);
+          dd_add_prod(acc[0], lvv, lvv);
+          dd_add_prod(acc[0], uvv, uvv);
+          acc[1].s = jl_max(acc[1].s, jl_max(lvv, uvv));
+          dd_add_prod(acc[2], oc, ox);
+          acc[3].s = jl_max(acc[3].s, fabs(ox));
+          dd_add_prod(acc[4], ox, ox);
+          const double r = jl_max(og, 0.0) * fl + jl_min(og, 0.0) * fu;
+          const double rv = og - r;
+          dd_add_prod(acc[5], rv, rv);
+          acc[6].s = jl_max(acc[6].s, fabs(rv));
+          dd_add(acc[7], contrib(ol, ou, r));
+          const double g2 = og - oc;
+          const double r2 = jl_max(g2, 0.0) * fl + jl_min(g2, 0.0) * fu;
+          const double rv2 = g2 - r2;
+          acc[8].s = jl_max(acc[8].s, fabs(rv2));
+          dd_add(acc[9], contrib(ol, ou, r2));
+          acc[10].s = jl_max(acc[10].s, fabs(r2));
+          acc[11].s = jl_max(acc[11].s, jl_max(fl != 0.0 ? jl_max(-ox, 0.0) : 0.0,
+                                                fu != 0.0 ? jl_max(ox, 0.0) : 0.0));
+          // restart candidates in the scaled space: current, average, last restart
+          const double xs3[3] = {xc, ax, ld<T>(I.lx, j)};
+          const double gs3[3] = {gnew, ag, ld<T>(I.lg, j)};
+          const double sfl = isfinite(ls) ? 1.0 : 0.0, sfu = isfinite(us) ? 1.0 : 0.0;
 #pragma unroll
-      for (int q = 0; q < 3; ++q) {
-        const double xs = xs3[q], gs = gs3[q];
-        const double a = jl_max(ls - xs, 0.0), b = jl_max(xs - us, 0.0);
-        dd_add_prod(acc[12 + 4 * q], a, a);
-        dd_add_prod(acc[12 + 4 * q], b, b);
-        dd_add_prod(acc[13 + 4 * q], cs, xs);
-        const double rr = jl_max(gs, 0.0) * sfl + jl_min(gs, 0.0) * sfu;
-        const double rvv = gs - rr;
-        dd_add_prod(acc[14 + 4 * q], rvv, rvv);
-        dd_add(acc[15 + 4 * q], contrib(ls, us, rr));
-      }
-      const double dxl = ax - xs3[2];
-      dd_add_prod(acc[24], dxl, dxl);
-    }
-    block_reduce_acc<kEvalN>(acc, kEvalMaxN, lds, epn + (size_t)blk * 2 * kEvalN);
-  } else {
-    DD acc[kEvalM];
-#pragma unroll
-    for (int k = 0; k < kEvalM; ++k) acc[k] = dd_zero();
-    const void *ycur = cur ? I.y1 : I.y0;
-    const void *axcur = cur ? I.ax1 : I.ax0;
-    const int neq = I.neq;
-    const bool has_ylo = I.ylo != nullptr;
-    for (int i = r0 + (int)threadIdx.x; i < r1; i += kThreads) {
-      const double yc = ld<T>(ycur, i), axc = ld<double>(axcur, i);
-      const double ay = usecur ? yc : ld<double>(I.sy, i) / wsum;
-      const double aax = usecur ? axc : ld<double>(I.sax, i) / wsum;
-      st<T>(I.avg_y, i, ay);
-      st<T>(I.avg_ax, i, aax);
-      const double Ei = I.E[i];
-      const double oy = ay / Ei, oax = aax * Ei;
-      const double ob = I.ob[i];
-      const double t = ob - oax;
-      const double cvv = i < neq ? t : jl_max(t, 0.0);
-      dd_add_prod(acc[0], cvv, cvv);
-      acc[1].s = jl_max(acc[1].s, fabs(cvv));
-      if (i >= neq) {
-        // dual residual max(ylo - y, 0); with the primal-objective term
-        // -ylo * max(Ax - b, 0) of a shifted dual bound (refinement only)
-        const double yl = has_ylo ? I.oylo[i] : 0.0;
-        const double dr = has_ylo ? jl_max(yl - oy, 0.0) : jl_max(-oy, 0.0);
-        dd_add_prod(acc[2], dr, dr);
-        acc[3].s = jl_max(acc[3].s, dr);
-        if (has_ylo) dd_add_prod(acc[18], yl, jl_max(oax - ob, 0.0));
-      }
-      dd_add_prod(acc[4], ob, oy);
-      acc[5].s = jl_max(acc[5].s, fabs(oy));
-      dd_add_prod(acc[6], oy, oy);
-      acc[7].s = jl_max(acc[7].s, i < neq ? fabs(oax) : jl_max(-oax, 0.0));
-      const double bs = ld<T>(I.b, i);
-      const double ys3[3] = {yc, ay, ld<T>(I.ly, i)};
-      const double as3[3] = {axc, aax, ld<T>(I.lax, i)};
-#pragma unroll
-      for (int q = 0; q < 3; ++q) {
-        const double tt = bs - as3[q];
-        const double cc = i < neq ? tt : jl_max(tt, 0.0);
-        dd_add_prod(acc[8 + 3 * q], cc, cc);
-        if (i >= neq) {
-          const double yl = has_ylo ? ld<T>(I.ylo, i) : 0.0;
-          const double dr = has_ylo ? jl_max(yl - ys3[q], 0.0) : jl_max(-ys3[q], 0.0);
-          dd_add_prod(acc[9 + 3 * q], dr, dr);
-          if (has_ylo) dd_add_prod(acc[19 + q], yl, jl_max(as3[q] - bs, 0.0));
+          for (int q = 0; q < 3; ++q) {
+            const double xs = xs3[q], gs = gs3[q];
+            const double a = jl_max(ls - xs, 0.0), b = jl_max(xs - us, 0.0);
+            dd_add_prod(acc[12 + 4 * q], a, a);
+            dd_add_prod(acc[12 + 4 * q], b, b);
+            dd_add_prod(acc[13 + 4 * q], cs, xs);
+            const double rr = jl_max(gs, 0.0) * sfl + jl_min(gs, 0.0) * sfu;
+            const double rvv = gs - rr;
+            dd_add_prod(acc[14 + 4 * q], rvv, rvv);
+            dd_add(acc[15 + 4 * q], contrib(ls, us, rr));
+          }
+          const double dxl = ax - xs3[2];
+          dd_add_prod(acc[24], dxl, dxl);
         }
-        dd_add_prod(acc[10 + 3 * q], bs, ys3[q]);
       }
-      const double dyl = ay - ys3[2];
-      dd_add_prod(acc[17], dyl, dyl);
+      block_reduce_acc<kEvalN>(acc, kEvalMaxN, lds, epn + (size_t)bfirst * 2 * kEvalN);
+      zero_partials(epn, bfirst + 1, blast, kEvalN);
+    } else {
+      DD acc[kEvalM];
+#pragma unroll
+      for (int k = 0; k < kEvalM; ++k) acc[k] = dd_zero();
+      const void *ycur = cur ? I.y1 : I.y0;
+      const void *axcur = cur ? I.ax1 : I.ax0;
+      const int neq = I.neq;
+      const bool has_ylo = I.ylo != nullptr;
+      for (int bb = bfirst; bb < blast; ++bb) {
+        int r0, r1;
+        block_rows(blocks[bb], &r0, &r1);
+        for (int i = r0 + (int)threadIdx.x; i < r1; i += kThreads) {
+          const double yc = ld<T>(ycur, i), axc = ld<double>(axcur, i);
+          const double ay = usecur ? yc : ld<double>(I.sy, i) / wsum;
+          const double aax = usecur ? axc : ld<double>(I.sax, i) / wsum;
+          st<T>(I.avg_y, i, ay);
+          st<T>(I.avg_ax, i, aax);
+          const double Ei = I.E[i];
+          const double oy = ay / Ei, oax = aax * Ei;
+          const double ob = I.ob[i];
+          const double t = ob - oax;
+          const double cvv = i < neq ? t : jl_max(t, 0.0);
+          dd_add_prod(acc[0], cvv, cvv);
+          acc[1].s = jl_max(acc[1].s, fabs(cvv));
+          if (i >= neq) {
+            // dual residual max(ylo - y, 0); with the primal-objective term
+            // -ylo * max(Ax - b, 0) of a shifted dual bound (refinement only)
+            const double yl = has_ylo ? I.oylo[i] : 0.0;
+            const double dr = has_ylo ? jl_max(yl - oy, 0.0) : jl_max(-oy, 0.0);
+            dd_add_prod(acc[2], dr, dr);
+            acc[3].s = jl_max(acc[3].s, dr);
+            if (has_ylo) dd_add_prod(acc[18], yl, jl_max(oax - ob, 0.0));
+          }
+          dd_add_prod(acc[4], ob, oy);
+          acc[5].s = jl_max(acc[5].s, fabs(oy));
+          dd_add_prod(acc[6], oy, oy);
+          acc[7].s = jl_max(acc[7].s, i < neq ? fabs(oax) : jl_max(-oax, 0.0));
+          const double bs = ld<T>(I.b, i);
+          const double ys3[3] = {yc, ay, ld<T>(I.ly, i)};
+          const double as3[3] = {axc, aax, ld<T>(I.lax, i)};
+#pragma unroll
+          for (int q = 0; q < 3; ++q) {
+            const double tt = bs - as3[q];
+            const double cc = i < neq ? tt : jl_max(tt, 0.0);
+            dd_add_prod(acc[8 + 3 * q], cc, cc);
+            if (i >= neq) {
+              const double yl = has_ylo ? ld<T>(I.ylo, i) : 0.0;
+              const double dr = has_ylo ? jl_max(yl - ys3[q], 0.0) : jl_max(-ys3[q], 0.0);
+              dd_add_prod(acc[9 + 3 * q], dr, dr);
+              if (has_ylo) dd_add_prod(acc[19 + q], yl, jl_max(as3[q] - bs, 0.0));
+            }
+            dd_add_prod(acc[10 + 3 * q], bs, ys3[q]);
+          }
+          const double dyl = ay - ys3[2];
+          dd_add_prod(acc[17], dyl, dyl);
+        }
+      }
+      block_reduce_acc<kEvalM>(acc, kEvalMaxM, lds, epm + (size_t)bfirst * 2 * kEvalM);
+      zero_partials(epm, bfirst + 1, blast, kEvalM);
     }
-    block_reduce_acc<kEvalM>(acc, kEvalMaxM, lds, epm + (size_t)blk * 2 * kEvalM);
+    __syncthreads();  // lds is reused by the next run
   }
 }
 
@@ -1372,6 +1404,7 @@ __global__ __launch_bounds__(kThreads) void k_ref_setup(const BlockDesc *__restr
 // Launchers.
 
 static inline unsigned grid_of(int n) { return (unsigned)(n > 0 ? n : 0); }
+static inline int eval_groups(int nblocks) { return (nblocks + kEvalGroup - 1) / kEvalGroup; }
 
 #define PDLP_CHECK_LAUNCH() return (int)hipGetLastError()
 
@@ -1561,11 +1594,11 @@ int launch_eval(hipStream_t s, int prec, int np, const BlockDesc *pblocks, int n
                 double *epm, double *epr, int ninst, double *out) {
   if (np + nd > 0) {
     if (prec == 0)
-      hipLaunchKernelGGL(k_eval<double>, dim3(grid_of(np + nd)), dim3(kThreads), 0, s, pblocks, np,
-                         dblocks, insts, ctl, epn, epm);
+      hipLaunchKernelGGL(k_eval<double>, dim3(grid_of(eval_groups(np) + eval_groups(nd))),
+                         dim3(kThreads), 0, s, pblocks, np, dblocks, nd, insts, ctl, epn, epm);
     else
-      hipLaunchKernelGGL(k_eval<float>, dim3(grid_of(np + nd)), dim3(kThreads), 0, s, pblocks, np,
-                         dblocks, insts, ctl, epn, epm);
+      hipLaunchKernelGGL(k_eval<float>, dim3(grid_of(eval_groups(np) + eval_groups(nd))),
+                         dim3(kThreads), 0, s, pblocks, np, dblocks, nd, insts, ctl, epn, epm);
   }
   if (ninst > 0)
     hipLaunchKernelGGL(k_eval_finish, dim3(grid_of(ninst), kEvalOut), dim3(kThreads), 0, s, insts,
