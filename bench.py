@@ -59,15 +59,18 @@ METRIC = "solves/s + p99 latency to 1e-8 rel KKT, FP32+refine batched vs plain F
 
 # HBM bytes per launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE
 # (2 * FETCH + WRITE, factors from a plain 1 GiB copy on the same box), see
-# profiles/hbm_traffic_counters.md.  Keyed by (workload, precision, kernel)
+# profiles/hbm_traffic_counters.md and profiles/batch_kernels_512lp_gbs.md
+# (the batch rows).  Keyed by (workload, precision, kernel)
 # and used only when this run's algorithmic bytes per launch are within
 # 0.5 % of the profiled run's (the same workload: when a batch instance
 # leaves the launches depends on the host's round timing, so the mix moves
 # by a few hundredths of a percent between runs); the profiled ratio of
 # traffic to algorithmic bytes is then applied to this run's bytes.
 MEASURED_TRAFFIC = {
-    ("batch", "fp64", "k_dual"): (69376581.66, 102.82e6),
-    ("batch", "fp32", "k_dual"): (180747947.88, 264.46e6),
+    ("batch", "fp64", "k_dual"): (69370296.0, 102.709e6),
+    ("batch", "fp64", "k_primal"): (85988368.0, 104.363e6),
+    ("batch", "fp32", "k_dual"): (172086233.0, 251.715e6),
+    ("batch", "fp32", "k_primal"): (207272000.0, 263.418e6),
     ("c1", "fp64", "k_dual"): (195460496.0, 283.94e6),
     ("c1", "fp64", "k_primal"): (229460496.0, 235.38e6),
     ("c1", "fp32", "k_dual"): (123640332.0, 171.69e6),

@@ -35,15 +35,20 @@
  *    as ((1/r_i) * a_ij) * (1/v_j).
  *  - ConstantStepsizeParams: the power method starts from a SplitMix64/
  *    Box-Muller normal vector (seed 1), not Julia's MersenneTwister(1)
- *    stream, so the estimated step size is not bit-identical (parity
- *    unpinned for that non-default policy).
+ *    stream, so the estimated step size is not bit-identical to the
+ *    reference's; what the reference's estimate satisfies is pinned instead
+ *    (0.8 sigma_max <= sigma_hat <= sigma_max against numpy's SVD, and the
+ *    iteration count of the power method's stopping rule).
  *
- * Pinning (tests/test_oracle.py): closed-form LPs (optimal, infeasible,
- * unbounded), generator certificates and HiGHS objectives (committed
- * fixtures), Ruiz / Pock-Chambolle closed forms and the Pock-Chambolle
- * operator-norm bound, KKT statistics at certified optima and under
- * perturbations with closed-form residuals, and the step-size rule on
- * scripted inputs.  The constant-step policy is "parity unpinned".
+ * Pinning (tests/test_oracle.py, tests/test_oracle_pinning.py): closed-form
+ * LPs (optimal, infeasible, unbounded), generator certificates and HiGHS
+ * objectives (committed fixtures), Ruiz / Pock-Chambolle / L2 rescaling
+ * closed forms and the Pock-Chambolle operator-norm bound, the initial step
+ * size 1 / max|A~_ij| and primal weight ||c~|| / ||b~||, KKT statistics at
+ * certified optima and under perturbations with closed-form residuals (L2
+ * and L_INF), the step-size rule on scripted inputs, the KKT-pass, time and
+ * iteration limits and their precedence, and the numerical-error branch
+ * (forced movement = 0).
  */
 #include <math.h>
 #include <stdint.h>
