@@ -417,15 +417,18 @@ def serve_stream(count, nmax, seed=4):
     return probs, tols
 
 
-def serve_once(ctx, probs, tols, rate, precision, hbm_budget_gb):
+def serve_once(ctx, probs, tols, rate, precision, hbm_budget_gb, request_time_limit=0.0):
     """Streams the requests through the request loop with Poisson arrivals
-    at `rate`: returns (seconds, latencies, results not OPTIMAL)."""
+    at `rate`, each with its own time limit when request_time_limit > 0
+    (seconds from submission): returns (seconds, latencies, results not
+    OPTIMAL, results that ended at their time limit)."""
     from cupdlp.solver import Server
     rng = np.random.default_rng(5)
     gaps = rng.exponential(1.0 / rate, size=len(probs)) if rate > 0 else np.zeros(len(probs))
     arrive = np.cumsum(gaps) - gaps[0]
     srv = Server(ctx, make_params(1e-4, precision), hbm_budget_bytes=hbm_budget_gb * 1e9)
-    lat, bad, done = [], 0, 0
+    lat, bad, timed_out, done = [], 0, 0, 0
+    limit = request_time_limit if request_time_limit > 0 else None
     nxt = 0
     t0 = time.time()
     last_log = t0
@@ -435,28 +438,31 @@ def serve_once(ctx, probs, tols, rate, precision, hbm_budget_gb):
             log(f"  {last_log - t0:.0f} s: {nxt} submitted, {done} done")
         now = time.time() - t0
         while nxt < len(probs) and arrive[nxt] <= now:
-            srv.submit(probs[nxt], tols[nxt])
+            srv.submit(probs[nxt], tols[nxt], time_sec_limit=limit)
             nxt += 1
         if done < nxt:
             for f in srv.poll():
                 o = srv.result(f)
                 lat.append(o.solve_time_sec)
                 bad += int(o.termination_string != "OPTIMAL")
+                timed_out += int(o.termination_string == "TIME_LIMIT")
                 done += 1
         elif nxt < len(probs):
             time.sleep(max(0.0, min(arrive[nxt] - now, 0.05)))
     dt = time.time() - t0
     srv.close()
-    return dt, lat, bad
+    return dt, lat, bad, timed_out
 
 
-def serial_stream(ctx, probs, tols, budget):
+def serial_stream(ctx, probs, tols, budget, request_time_limit=0.0):
     """Plain FP64 one request at a time, in arrival order, until the budget
-    is spent: (solves/s, requests solved)."""
+    is spent, each within request_time_limit seconds when that is > 0:
+    (solves/s, requests solved)."""
     t1 = time.time()
     nser = 0
+    limit = request_time_limit if request_time_limit > 0 else float("inf")
     for p, t in zip(probs, tols):
-        ctx.solve(p, make_params(t, "fp64"))
+        ctx.solve(p, make_params(t, "fp64", time_limit=limit))
         nser += 1
         if time.time() - t1 > budget:
             break
@@ -470,7 +476,7 @@ def served_leg(args, ctx):
     precision, against plain FP64 one at a time on the same requests."""
     log(f"served leg: generating {args.serve_count} requests ...")
     probs, tols = serve_stream(args.serve_count, args.serve_nmax)
-    dt, lat, bad = serve_once(ctx, probs, tols, args.rate, args.precision, args.hbm_budget)
+    dt, lat, bad, _ = serve_once(ctx, probs, tols, args.rate, args.precision, args.hbm_budget)
     ser, nser = serial_stream(ctx, probs, tols, args.serial_budget / 2)
     value = len(probs) / dt
     log(f"served leg: {value:.2f} solves/s, FP64 one at a time {ser:.2f}")
@@ -492,12 +498,15 @@ def run_serve(args, ctx, lib):
     for w in range(args.warmup):  # warm-up: the first few requests alone
         for p, t in list(zip(probs, tols))[:4]:
             ctx.solve(p, make_params(t, args.precision))
-    dt, lat, bad = serve_once(ctx, probs, tols, args.rate, args.precision, args.hbm_budget)
+    dt, lat, bad, timed_out = serve_once(ctx, probs, tols, args.rate, args.precision,
+                                         args.hbm_budget, args.request_time_limit)
     value = len(probs) / dt
-    ser, nser = serial_stream(ctx, probs, tols, args.serial_budget)
+    ser, nser = serial_stream(ctx, probs, tols, args.serial_budget, args.request_time_limit)
     extra = {"p50_latency_s": round(float(np.percentile(lat, 50)), 4),
              "p99_latency_s": round(float(np.percentile(lat, 99)), 4),
-             "not_optimal": bad, "fp64_serial_solves_per_s": round(ser, 3),
+             "not_optimal": bad, "time_limited": timed_out,
+             "request_time_limit_s": args.request_time_limit or None,
+             "fp64_serial_solves_per_s": round(ser, 3),
              "fp64_serial_sample": nser, "speedup_vs_fp64_serial": round(value / ser, 3)}
     cfg = {"workload": f"served stream of {len(probs)} requests (90% random LPs n log-uniform "
                        f"1000..{args.serve_nmax}, 10% transportation 200..1000 per side; tolerance "
@@ -580,6 +589,9 @@ def main():
     ap.add_argument("--rate", type=float, default=50.0, help="serve: Poisson arrival rate (requests/s)")
     ap.add_argument("--serve-nmax", type=int, default=200000, help="serve: largest random-LP n")
     ap.add_argument("--hbm-budget", type=float, default=200.0, help="serve: HBM budget (GB)")
+    ap.add_argument("--request-time-limit", type=float, default=0.0,
+                    help="serve: each request's own time limit in seconds from its "
+                         "submission (pdlp_serve_submit_limits; 0 = none)")
     ap.add_argument("--precision", choices=("fp64", "fp32"), default=None,
                     help="default: fp32 (FP32 + FP64 refinement) for the batch, fp64 otherwise")
     ap.add_argument("--no-fp64-batch", action="store_true",
