@@ -1007,12 +1007,27 @@ struct ServeDone {
   RefinedPoint refined;        // the stalled FP32 solve's point
 };
 
+// Serving size classes: a request goes to the engine of its class by
+// nonzeros (class c holds nnz in [2^(17+3(c-1)), 2^(17+3c)), the last one
+// everything above), so a small request does not advance at the pace of the
+// large requests' launches; the engines run their rounds side by side on
+// streams of their own.  params.batch_groups sets the number of classes
+// (0 = 3, at most kServeClasses); results do not depend on it.
+constexpr int kServeClasses = 4;
+
+static int serve_class(int64_t nnz, int classes) {
+  int c = 0;
+  while (c + 1 < classes && nnz >= (int64_t{1} << (17 + 3 * c))) ++c;
+  return c;
+}
+
 struct pdlp_server {
   pdlp_ctx *ctx = nullptr;
   pdlp_params prm{};
   double budget = 0;
   int32_t max_slots = 0;
-  std::unique_ptr<Engine> eng;
+  int classes = 1;
+  std::vector<std::unique_ptr<Engine>> engs;  // one per size class, created on first use
   std::unique_ptr<Engine> fb;  // FP64 solves of stalled FP32 requests (context's second stream)
   std::deque<ServeReq> queue;
   std::map<int64_t, std::unique_ptr<ServeDone>> running;  // result storage of admitted ids
@@ -1044,9 +1059,30 @@ extern "C" int pdlp_serve_open(pdlp_ctx *ctx, const pdlp_params *params, double 
   s->prm = *params;
   s->budget = hbm_budget_bytes > 0 ? hbm_budget_bytes : 1e18;
   s->max_slots = max_slots > 0 ? max_slots : 1 << 30;
-  s->eng = std::make_unique<Engine>(ctx->stream, &ctx->pool, *params);
+  s->classes = params->batch_groups > 0 ? std::min(params->batch_groups, kServeClasses) : 3;
+  s->engs.resize(s->classes);
   *out = s;
   return 0;
+}
+
+// The engine of size class c (its stream: the context's first for class 0,
+// a stream of its own for the others).
+static int serve_engine(pdlp_server *s, int c, Engine **out, std::string *err) {
+  if (!s->engs[c]) {
+    hipStream_t st;
+    int rc = ctx_stream(s->ctx, c == 0 ? 0 : 16 + c, &st, err);
+    if (rc) return rc;
+    s->engs[c] = std::make_unique<Engine>(st, &s->ctx->pool, s->prm);
+  }
+  *out = s->engs[c].get();
+  return 0;
+}
+
+static int32_t serve_running(const pdlp_server *s) {
+  size_t k = 0;
+  for (const auto &e : s->engs)
+    if (e) k += e->count();
+  return (int32_t)std::min<size_t>(k, INT32_MAX);
 }
 
 static int serve_submit(pdlp_server *s, const pdlp_lp *lp, double tolerance, double time_limit,
@@ -1136,7 +1172,7 @@ extern "C" int pdlp_serve_poll(pdlp_server *s, int64_t *ids, int32_t max_ids, in
     }
   }
   // admit by footprint and slot count
-  while (!s->queue.empty() && (int32_t)s->eng->count() < s->max_slots &&
+  while (!s->queue.empty() && serve_running(s) < s->max_slots &&
          s->in_use + (double)s->queue.front().bytes <= s->budget) {
     ServeReq r = std::move(s->queue.front());
     s->queue.pop_front();
@@ -1151,7 +1187,9 @@ extern "C" int pdlp_serve_poll(pdlp_server *s, int64_t *ids, int32_t max_ids, in
     // counted as running only once the engine holds it; a request the
     // engine cannot take is finished with reason OTHER (its result says so)
     // and the error is returned
-    int rc = s->eng->add(std::move(r.inst), err);
+    Engine *eng = nullptr;
+    int rc = serve_engine(s, serve_class(r.inst->P.nnz, s->classes), &eng, err);
+    if (rc == 0) rc = eng->add(std::move(r.inst), err);
     if (rc) {
       d->res.termination_reason = PDLP_TERMINATION_REASON_OTHER;
       d->res.solve_time_sec = now_sec() - d->t_submit;
@@ -1163,11 +1201,25 @@ extern "C" int pdlp_serve_poll(pdlp_server *s, int64_t *ids, int32_t max_ids, in
     s->running[r.id] = std::move(d);
   }
   const bool refine = s->prm.precision == PDLP_PRECISION_FP32_REFINE;
-  if (s->eng->count() > 0) {
-    int rc = s->eng->step_round(err);
+  // one round of every engine, enqueued back to back so their kernels overlap
+  std::vector<Engine *> active;
+  for (auto &e : s->engs)
+    if (e && e->count() > 0) active.push_back(e.get());
+  if (s->fb && s->fb->count() > 0) active.push_back(s->fb.get());
+  for (Engine *e : active) {
+    int rc = e->begin_round(err);
     if (rc < 0) return rc;
-    for (size_t q = 0; q < s->eng->count(); ++q) {
-      HostInst &h = s->eng->inst(q);
+  }
+  for (Engine *e : active) {
+    int rc = e->end_round(err);
+    if (rc < 0) return rc;
+  }
+  for (auto &ep : s->engs) {
+    if (!ep || ep->count() == 0) continue;
+    Engine *eng = ep.get();
+    int rc = 0;
+    for (size_t q = 0; q < eng->count(); ++q) {
+      HostInst &h = eng->inst(q);
       if (!h.done) continue;
       auto it = s->running.find(h.tag);
       if (it == s->running.end()) continue;
@@ -1214,11 +1266,9 @@ extern "C" int pdlp_serve_poll(pdlp_server *s, int64_t *ids, int32_t max_ids, in
       s->running.erase(it);
       s->finished.push_back(h.tag);
     }
-    s->eng->remove_done();
+    eng->remove_done();
   }
   if (s->fb && s->fb->count() > 0) {
-    int rc = s->fb->step_round(err);
-    if (rc < 0) return rc;
     for (size_t q = 0; q < s->fb->count(); ++q) {
       HostInst &h = s->fb->inst(q);
       if (!h.done) continue;
@@ -1287,9 +1337,13 @@ extern "C" int pdlp_serve_cancel(pdlp_server *s, int64_t id) {
   }
   ServeDone &d = *rit->second;
   const bool refine = s->prm.precision == PDLP_PRECISION_FP32_REFINE;
-  // the request is on the FP32 engine, or (stalled) on the FP64 one
-  for (int which = 0; which < 2; ++which) {
-    Engine *e = which == 0 ? s->eng.get() : s->fb.get();
+  // the request is on its class's engine, or (a stalled FP32 request) on the FP64 one
+  std::vector<Engine *> engines;
+  for (auto &e : s->engs) engines.push_back(e.get());
+  engines.push_back(s->fb.get());
+  for (size_t ei = 0; ei < engines.size(); ++ei) {
+    Engine *e = engines[ei];
+    const int which = ei + 1 == engines.size() ? 1 : 0;
     if (!e) continue;
     for (size_t q = 0; q < e->count(); ++q) {
       HostInst &h = e->inst(q);
@@ -1319,7 +1373,7 @@ extern "C" int pdlp_serve_cancel(pdlp_server *s, int64_t id) {
 extern "C" int pdlp_serve_close(pdlp_server *s) {
   if (!s) return 1;
   s->fb.reset();
-  s->eng.reset();
+  s->engs.clear();
   delete s;
   return 0;
 }

@@ -161,7 +161,10 @@ typedef struct pdlp_params {
                                            solve; 0 = max(10000, 20 x the first solve's) */
   int32_t batch_groups;              /* batched solves: size groups run side by side, each
                                         with its own launches and streams; 0 = automatic
-                                        (3 from 24 instances up), 1 = one group */
+                                        (3 from 24 instances up), 1 = one group.  Servers:
+                                        size classes by nonzeros (< 2^17, < 2^20, < 2^23,
+                                        above), each with its own engine and stream;
+                                        0 = 3, at most 4 */
   int32_t refine_early_fallback;     /* FP32_REFINE, pdlp_batch_solve / pdlp_batch_run:
                                         iterations into a correction solve after which
                                         the instance's plain FP64 solve starts beside it,

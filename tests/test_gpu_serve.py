@@ -121,6 +121,38 @@ def test_serve_fp32_refine_alone_vs_mixed(gpu_ctx):
         same_result(out[rid], alone[k])
 
 
+@pytest.mark.parametrize("precision", [Precision.FP64, Precision.FP32_REFINE])
+def test_serve_size_classes_bitwise(gpu_ctx, precision):
+    """Requests of three size classes (nonzeros below 2^17, below 2^20 and
+    above) run in engines of their own, side by side: each result is bitwise
+    the lone solve's and the same as with a single engine (batch_groups 1)."""
+    print(f"case: served requests n = 1,000..140,000 in 3 size classes vs 1, {precision.name}",
+          flush=True)
+    sizes = [1000, 140_000, 3000, 20_000, 40_000, 1500]
+    probs = [random_lp(n=n, m=n // 2, seed=80 + i)[0] for i, n in enumerate(sizes)]
+    tols = [1e-6, 1e-4, 1e-8, 1e-6, 1e-4, 1e-8]
+    alone = [gpu_ctx.solve(p, params(t, precision)) for p, t in zip(probs, tols)]
+    res = {}
+    for classes in (3, 1):
+        sp = params(1e-4, precision)
+        sp.batch_groups = classes
+        srv = Server(gpu_ctx, sp)
+        try:
+            out = {}
+            ids = [srv.submit(probs[i], tols[i]) for i in range(3)]
+            for _ in range(3):
+                for f in srv.poll():
+                    out[f] = srv.result(f)
+            ids += [srv.submit(probs[i], tols[i]) for i in range(3, len(probs))]
+            drain(srv, ids, out)
+        finally:
+            srv.close()
+        res[classes] = [out[i] for i in ids]
+    for k in range(len(probs)):
+        same_result(res[3][k], alone[k])
+        same_result(res[1][k], alone[k])
+
+
 def test_serve_fp32_stalled_falls_back_to_fp64(gpu_ctx):
     print("case: FP32 + refinement server with inner cap 64: stalled requests finish in FP64, "
           "bitwise the plain FP64 solve", flush=True)
