@@ -352,6 +352,12 @@ def run_batch(args, ctx, lib):
                                      make_params(args.tol, "fp64", time_limit=args.time_limit))
         extra["fp64_batched_solves_per_s"] = round(len(problems) / d64, 4)
         extra["fp64_batched_p99_latency_s"] = round(float(np.percentile([o.solve_time_sec for o in o64], 99)), 4)
+        if args.dump_instances:
+            with open(args.dump_instances + ".fp64.json", "w") as f:
+                json.dump([{"i": i, "n": p.num_variables, "nnz": int(p.constraint_matrix.nnz),
+                            "iterations": o.iteration_count, "attempts": o.total_attempts,
+                            "latency_s": round(o.solve_time_sec, 4)}
+                           for i, (p, o) in enumerate(zip(problems, o64))], f)
         a64 = accuracy(problems, o64, objectives)
         extra["fp64_batched_max_rel_kkt"] = a64["max_rel_kkt"]
         extra["fp64_batched_max_rel_obj_error_vs_certificate"] = a64["max_rel_obj_error_vs_certificate"]

@@ -121,13 +121,30 @@ __device__ __forceinline__ double wave_max(double v) {
   return v;
 }
 
+// Global-address-space views of device pointers.  Pointers read from the
+// instance tables are generic, and loads through them compile to flat
+// loads, which also count against lgkmcnt: every LDS or cross-lane wait then
+// waits for the flat loads in flight as well.
+template <typename T>
+using gptr = const __attribute__((address_space(1))) T *;
+template <typename T>
+using gptr_w = __attribute__((address_space(1))) T *;
+template <typename T>
+__device__ __forceinline__ gptr<T> gp(const T *p) {
+  return (gptr<T>)p;
+}
+template <typename T>
+__device__ __forceinline__ gptr_w<T> gpw(T *p) {
+  return (gptr_w<T>)p;
+}
+
 template <typename T>
 __device__ __forceinline__ double ld(const void *p, int i) {
-  return (double)((const T *)p)[i];
+  return (double)gp((const T *)p)[i];
 }
 template <typename T>
 __device__ __forceinline__ void st(void *p, int i, double v) {
-  ((T *)p)[i] = (T)v;
+  gpw((T *)p)[i] = (T)v;
 }
 
 // Row sums of one row block.  Phase 1: the block's nonzeros are read as one
@@ -142,12 +159,15 @@ __device__ __forceinline__ void rowblock_sums(const int32_t *__restrict__ ptr,
                                               const T *__restrict__ val,
                                               const V *__restrict__ xv, int row0, int nrows,
                                               double *lds_p, double *lds_q, double *rowsum) {
-  const int k0 = ptr[row0];
-  const int nz = ptr[row0 + nrows] - k0;
+  const gptr<int32_t> pg = gp(ptr), cg = gp(col);
+  const gptr<T> vg = gp(val);
+  const gptr<V> xg = gp(xv);
+  const int k0 = pg[row0];
+  const int nz = pg[row0 + nrows] - k0;
 #pragma unroll 4
   for (int k = threadIdx.x; k < nz; k += kThreads) {
-    const double a = (double)__builtin_nontemporal_load(&val[k0 + k]);
-    const double x = (double)xv[__builtin_nontemporal_load(&col[k0 + k])];
+    const double a = (double)__builtin_nontemporal_load(vg + k0 + k);
+    const double x = (double)xg[__builtin_nontemporal_load(cg + k0 + k)];
     const double pr = a * x;
     lds_p[k] = pr;
     lds_q[k] = __builtin_fma(a, x, -pr);
@@ -159,7 +179,7 @@ __device__ __forceinline__ void rowblock_sums(const int32_t *__restrict__ ptr,
   const int grp = threadIdx.x / tpr;
   const int ngrp = kThreads / tpr;
   for (int r = grp; r < nrows; r += ngrp) {
-    const int rs = ptr[row0 + r] - k0, re = ptr[row0 + r + 1] - k0;
+    const int rs = pg[row0 + r] - k0, re = pg[row0 + r + 1] - k0;
     DD acc = dd_zero();
     for (int k = rs + lane; k < re; k += tpr) {
       dd_add(acc, lds_p[k]);
@@ -279,43 +299,55 @@ __global__ __launch_bounds__(kThreads) void k_primal_flush(const BlockDesc *__re
 // skipped.  Entries go in groups of U whose index and value loads are all
 // issued before their gathers; the last group is predicated.  U changes only
 // how many loads are in flight, not the order of the sum.
+// s += x*y for x, y widened from FP32: the product of two floats is exact in
+// FP64 (48 significant bits, and the exponent range of a double holds every
+// product of two floats), so TwoProd's error term is +0 and is not formed.
+template <typename T, typename V>
+__device__ __forceinline__ void dd_add_prod_of(DD &a, T x, V y) {
+  if constexpr (std::is_same<T, float>::value && std::is_same<V, float>::value)
+    dd_add(a, (double)x * (double)y);
+  else
+    dd_add_prod(a, (double)x, (double)y);
+}
+
 template <typename T, typename V, int U = 8>
 __device__ __forceinline__ DD slice_lane_sum(const int32_t *__restrict__ scol,
                                              const T *__restrict__ sval,
-                                             const V *__restrict__ xv, int off, int width,
-                                             int lane) {
+                                             gptr<V> xg, int off, int width, int lane) {
   DD acc = dd_zero();
-  const int32_t *cp = scol + off + lane;
-  const T *vp = sval + off + lane;
+  const gptr<int32_t> cp = gp(scol + off + lane);
+  const gptr<T> vp = gp(sval + off + lane);
   int k = 0;
   for (; k + U <= width; k += U) {
     int32_t c[U];
-    double v[U], g[U];
+    T v[U];
+    V g[U];
 #pragma unroll
     for (int i = 0; i < U; ++i) {
       c[i] = __builtin_nontemporal_load(cp + 64 * (k + i));
-      v[i] = (double)__builtin_nontemporal_load(vp + 64 * (k + i));
+      v[i] = __builtin_nontemporal_load(vp + 64 * (k + i));
     }
 #pragma unroll
-    for (int i = 0; i < U; ++i) g[i] = c[i] >= 0 ? (double)xv[c[i]] : 0.0;
+    for (int i = 0; i < U; ++i) g[i] = c[i] >= 0 ? xg[c[i]] : V(0);
 #pragma unroll
     for (int i = 0; i < U; ++i)
-      if (c[i] >= 0) dd_add_prod(acc, v[i], g[i]);
+      if (c[i] >= 0) dd_add_prod_of(acc, v[i], g[i]);
   }
   if (k < width) {
     int32_t c[U];
-    double v[U], g[U];
+    T v[U];
+    V g[U];
 #pragma unroll
     for (int i = 0; i < U; ++i) {
       const bool in = k + i < width;
       c[i] = in ? __builtin_nontemporal_load(cp + 64 * (k + i)) : -1;
-      v[i] = in ? (double)__builtin_nontemporal_load(vp + 64 * (k + i)) : 0.0;
+      v[i] = in ? __builtin_nontemporal_load(vp + 64 * (k + i)) : T(0);
     }
 #pragma unroll
-    for (int i = 0; i < U; ++i) g[i] = c[i] >= 0 ? (double)xv[c[i]] : 0.0;
+    for (int i = 0; i < U; ++i) g[i] = c[i] >= 0 ? xg[c[i]] : V(0);
 #pragma unroll
     for (int i = 0; i < U; ++i)
-      if (c[i] >= 0) dd_add_prod(acc, v[i], g[i]);
+      if (c[i] >= 0) dd_add_prod_of(acc, v[i], g[i]);
   }
   return acc;
 }
@@ -333,9 +365,9 @@ __device__ __forceinline__ double group_sum(DD acc, int lg) {
 // Per-launch view of one instance for the primal step.
 template <typename T>
 struct PrimalView {
-  const T *yv, *xv, *cv, *lv, *uv;
-  T *xn, *aty, *dx;
-  double *sx, *saty;
+  gptr<T> yv, xv, cv, lv, uv;
+  gptr_w<T> xn, aty, dx;
+  gptr_w<double> sx, saty;
   bool pending, need;
   double w, tau;
   __device__ PrimalView(const InstDev &I, const Ctl *C) {
@@ -344,16 +376,16 @@ struct PrimalView {
     need = C->aty_valid == 0;
     w = C->w_pending;
     tau = C->eta / C->omega;
-    yv = (const T *)(cur ? I.y1 : I.y0);
-    xv = (const T *)(cur ? I.x1 : I.x0);
-    xn = (T *)(cur ? I.x0 : I.x1);
-    aty = (T *)I.aty;
-    dx = (T *)I.dx;
-    sx = (double *)I.sx;
-    saty = (double *)I.saty;
-    cv = (const T *)I.c;
-    lv = (const T *)I.l;
-    uv = (const T *)I.u;
+    yv = gp((const T *)(cur ? I.y1 : I.y0));
+    xv = gp((const T *)(cur ? I.x1 : I.x0));
+    xn = gpw((T *)(cur ? I.x0 : I.x1));
+    aty = gpw((T *)I.aty);
+    dx = gpw((T *)I.dx);
+    sx = gpw((double *)I.sx);
+    saty = gpw((double *)I.saty);
+    cv = gp((const T *)I.c);
+    lv = gp((const T *)I.l);
+    uv = gp((const T *)I.u);
   }
 };
 
@@ -585,10 +617,10 @@ __global__ __launch_bounds__(kThreads) void k_dual_flush(const BlockDesc *__rest
 // from the stored point).
 template <typename T>
 struct DualView {
-  const T *yv, *bv, *dxv, *ylo;
-  T *yn;
-  const double *axv;
-  double *axn, *sy, *sax;
+  gptr<T> yv, bv, dxv, ylo;
+  gptr_w<T> yn;
+  gptr<double> axv;
+  gptr_w<double> axn, sy, sax;
   bool pending;
   double w, sig;
   int neq;
@@ -598,15 +630,15 @@ struct DualView {
     w = C->w_pending;
     sig = C->omega * C->eta;
     neq = I.neq;
-    yv = (const T *)(cur ? I.y1 : I.y0);
-    yn = (T *)(cur ? I.y0 : I.y1);
-    axv = (const double *)(cur ? I.ax1 : I.ax0);
-    axn = (double *)(cur ? I.ax0 : I.ax1);
-    sy = (double *)I.sy;
-    sax = (double *)I.sax;
-    bv = (const T *)I.b;
-    dxv = (const T *)(std::is_same<T, float>::value ? (cur ? I.x0 : I.x1) : I.dx);
-    ylo = (const T *)I.ylo;
+    yv = gp((const T *)(cur ? I.y1 : I.y0));
+    yn = gpw((T *)(cur ? I.y0 : I.y1));
+    axv = gp((const double *)(cur ? I.ax1 : I.ax0));
+    axn = gpw((double *)(cur ? I.ax0 : I.ax1));
+    sy = gpw((double *)I.sy);
+    sax = gpw((double *)I.sax);
+    bv = gp((const T *)I.b);
+    dxv = gp((const T *)(std::is_same<T, float>::value ? (cur ? I.x0 : I.x1) : I.dx));
+    ylo = gp((const T *)I.ylo);
   }
 };
 
@@ -936,9 +968,9 @@ __global__ __launch_bounds__(kThreads) void k_eval(const BlockDesc *__restrict__
           st<T>(I.avg_g, j, ag);
           st<T>(I.gbuf, j, gnew);
           // original space
-          const double Dj = I.D[j];
+          const double Dj = gp(I.D)[j];
           const double ox = ax / Dj, og = ag * Dj;
-          const double ol = I.ol[j], ou = I.ou[j], oc = I.oc[j];
+          const double ol = gp(I.ol)[j], ou = gp(I.ou)[j], oc = gp(I.oc)[j];
           const double fl = isfinite(ol) ? 1.0 : 0.0, fu = isfinite(ou) ? 1.0 : 0.0;
           const double lvv = jl_max(ol - ox, 0.0), uvv = jl_max(ox - ou, 0.0);
           dd_add_prod(acc[0], lvv, lvv);
@@ -999,9 +1031,9 @@ __global__ __launch_bounds__(kThreads) void k_eval(const BlockDesc *__restrict__
           const double aax = usecur ? axc : ld<double>(I.sax, i) / wsum;
           st<T>(I.avg_y, i, ay);
           st<T>(I.avg_ax, i, aax);
-          const double Ei = I.E[i];
+          const double Ei = gp(I.E)[i];
           const double oy = ay / Ei, oax = aax * Ei;
-          const double ob = I.ob[i];
+          const double ob = gp(I.ob)[i];
           const double t = ob - oax;
           const double cvv = i < neq ? t : jl_max(t, 0.0);
           dd_add_prod(acc[0], cvv, cvv);
@@ -1009,7 +1041,7 @@ __global__ __launch_bounds__(kThreads) void k_eval(const BlockDesc *__restrict__
           if (i >= neq) {
             // dual residual max(ylo - y, 0); with the primal-objective term
             // -ylo * max(Ax - b, 0) of a shifted dual bound (refinement only)
-            const double yl = has_ylo ? I.oylo[i] : 0.0;
+            const double yl = has_ylo ? gp(I.oylo)[i] : 0.0;
             const double dr = has_ylo ? jl_max(yl - oy, 0.0) : jl_max(-oy, 0.0);
             dd_add_prod(acc[2], dr, dr);
             acc[3].s = jl_max(acc[3].s, dr);
