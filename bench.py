@@ -87,6 +87,7 @@ REFINE_INNER_TOL = 0.0  # --refine-inner-tol (0: the library default)
 REFINE_EARLY = 0  # --refine-early-fallback (0: the library default)
 REFINE_WARM = -1  # --refine-warm-start (-1: the library default)
 REFINE_CAP = 0  # --refine-cap (0: the library default)
+REFINE_CONTINUE = -1  # --refine-continue (-1: the library default)
 
 
 def make_params(tol, precision, iteration_limit=2**31 - 1, time_limit=float("inf")):
@@ -103,6 +104,8 @@ def make_params(tol, precision, iteration_limit=2**31 - 1, time_limit=float("inf
         p.refine_warm_start = REFINE_WARM
     if REFINE_CAP > 0:
         p.refine_inner_iteration_limit = REFINE_CAP
+    if REFINE_CONTINUE >= 0:
+        p.refine_continue_progress = REFINE_CONTINUE
     for k, v in TILES.get(precision, {}).items():
         setattr(p, k, v)
     return p
@@ -479,14 +482,21 @@ def served_leg(args, ctx):
     """A bounded served stream beside the batch line: args.serve_count
     requests of the served mix (random LPs n <= args.serve_nmax, 10 %
     transportation), Poisson arrivals at args.rate, in the batch line's
-    precision, against plain FP64 one at a time on the same requests."""
+    precision, each request with a time limit of args.serve_time_limit
+    seconds from its submission (0: none), against plain FP64 one at a time
+    on the same requests with the same limit.  The default stream holds one
+    LP that needs 470,592 iterations to 1e-8 (21 s alone): without a limit
+    it alone sets the stream's makespan."""
     log(f"served leg: generating {args.serve_count} requests ...")
     probs, tols = serve_stream(args.serve_count, args.serve_nmax)
-    dt, lat, bad, _ = serve_once(ctx, probs, tols, args.rate, args.precision, args.hbm_budget)
-    ser, nser = serial_stream(ctx, probs, tols, args.serial_budget / 2)
+    lim = args.serve_time_limit
+    dt, lat, bad, timed_out = serve_once(ctx, probs, tols, args.rate, args.precision,
+                                         args.hbm_budget, lim)
+    ser, nser = serial_stream(ctx, probs, tols, args.serial_budget / 2, lim)
     value = len(probs) / dt
     log(f"served leg: {value:.2f} solves/s, FP64 one at a time {ser:.2f}")
     return {"requests": len(probs), "completed": len(lat), "not_optimal": bad,
+            "time_limited": timed_out, "request_time_limit_s": lim or None,
             "solves_per_s": round(value, 3), "seconds": round(dt, 2),
             "p50_latency_s": round(float(np.percentile(lat, 50)), 4),
             "p99_latency_s": round(float(np.percentile(lat, 99)), 4),
@@ -495,7 +505,8 @@ def served_leg(args, ctx):
             "workload": f"{len(probs)} requests: 90% random LPs n log-uniform 1000..{args.serve_nmax}, "
                         f"10% transportation 200..1000 per side; tolerance 1e-4 (70%) / 1e-8 "
                         f"(30%); Poisson arrivals at {args.rate:g}/s; {args.precision}; HBM budget "
-                        f"{args.hbm_budget:g} GB"}
+                        f"{args.hbm_budget:g} GB"
+                        + (f"; per-request time limit {lim:g} s" if lim else "")}
 
 
 def run_serve(args, ctx, lib):
@@ -615,6 +626,10 @@ def main():
     ap.add_argument("--refine-cap", type=int, default=0,
                     help="FP32 + refinement: iteration cap of each correction solve "
                          "(0 = the library default)")
+    ap.add_argument("--refine-continue", type=int, default=-1,
+                    help="FP32 + refinement: a capped correction solve that cut the error this "
+                         "many times continues the refinement (0: stall at once; -1: the "
+                         "library default)")
     ap.add_argument("--refine-warm-start", type=int, default=-1,
                     help="FP32 + refinement: correction solves start from the previous "
                          "solve's primal weight (1), step size (2) or both (3)")
@@ -631,6 +646,9 @@ def main():
     ap.add_argument("--dump-instances", default="",
                     help="batch: write each instance's size, iterations and latency (last "
                          "timed step) to this JSON file")
+    ap.add_argument("--serve-time-limit", type=float, default=10.0,
+                    help="served leg of the batch line: time limit of each request (s, from "
+                         "its submission; 0: none)")
     ap.add_argument("--serve-count", type=int, default=160,
                     help="batch: requests of the served leg reported beside it (0 = none)")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
@@ -640,8 +658,9 @@ def main():
     args = ap.parse_args()
     global BATCH_GROUPS, REFINE_INNER_TOL, REFINE_EARLY, REFINE_WARM
     REFINE_WARM = args.refine_warm_start
-    global REFINE_CAP
+    global REFINE_CAP, REFINE_CONTINUE
     REFINE_CAP = args.refine_cap
+    REFINE_CONTINUE = args.refine_continue
     BATCH_GROUPS = args.batch_groups
     REFINE_INNER_TOL = args.refine_inner_tol
     REFINE_EARLY = args.refine_early_fallback

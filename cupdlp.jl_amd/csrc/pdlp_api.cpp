@@ -98,6 +98,7 @@ void pdlp_default_params(pdlp_params *p) {
   p->device_preconditioning = 1;
   p->refine_inner_iteration_limit = 0;
   p->refine_warm_start = 3;
+  p->refine_continue_progress = 0;
 }
 
 int pdlp_ctx_create(int device, pdlp_ctx **out) {
@@ -204,7 +205,8 @@ static int check_params(const pdlp_params *p, std::string *err) {
   }
   if (p->precision == PDLP_PRECISION_FP32_REFINE &&
       (p->refine_max_outer < 0 || !(p->refine_inner_tolerance > 0) ||
-       !(p->refine_max_scale >= 1.0) || p->refine_inner_iteration_limit < 0)) {
+       !(p->refine_max_scale >= 1.0) || p->refine_inner_iteration_limit < 0 ||
+       p->refine_continue_progress < 0)) {
     *err = "invalid refinement parameters";
     return 2;
   }

@@ -1450,7 +1450,11 @@ void Engine::refine_logic(size_t q, const double *o) {
   // the point's error as a multiple of the tolerance (optimal iff < 1)
   const double err = jl_max(jl_max(pe / (ea + er * pb), de / (ea + er * db)),
                             gap / (ea + er * abs_obj));
+  const double prev_err = h.last_err;
   h.last_err = err;
+  // whether the solve that produced this point stopped at its iteration cap
+  const bool capped_now = h.capped_rounds > h.capped_seen;
+  h.capped_seen = h.capped_rounds;
   const bool improved = err < h.best_err;
   if (improved) {
     h.best_err = err;
@@ -1463,9 +1467,15 @@ void Engine::refine_logic(size_t q, const double *o) {
     reason = PDLP_TERMINATION_REASON_TIME_LIMIT;
   else if (h.iters_total >= (int64_t)tc.iteration_limit)
     reason = PDLP_TERMINATION_REASON_ITERATION_LIMIT;
-  else if (h.capped_rounds >= 1 || h.ref_round >= prm_.refine_max_outer) {
-    // refinement stalled (a correction solve stopped by its cap, or the
-    // round limit): the caller may solve the LP again in FP64
+  else if (h.ref_round >= prm_.refine_max_outer ||
+           (capped_now && !(h.ref_round > 0 && prm_.refine_continue_progress > 0 &&
+                            err * (double)prm_.refine_continue_progress <= prev_err))) {
+    // refinement stalled (the round limit, the first FP32 solve stopped by
+    // its cap, or a correction solve stopped by its cap without cutting the
+    // error refine_continue_progress times): the caller may solve the LP
+    // again in FP64.  A capped correction solve that did cut the error that
+    // much moved the point closer to optimal, and the next round starts
+    // from there.
     reason = PDLP_TERMINATION_REASON_ITERATION_LIMIT;
     h.stalled = true;
   }

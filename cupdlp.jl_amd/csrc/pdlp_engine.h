@@ -95,6 +95,7 @@ struct HostInst {
   double tol_cur = -1.0;             // tolerance of the current (inner) solve, -1 = user's
   int32_t ref_round = 0;             // correction LPs solved so far
   int32_t capped_rounds = 0;         // correction solves stopped by their iteration cap
+  int32_t capped_seen = 0;           // capped_rounds at the last refinement test
   bool stalled = false;              // refinement stalled: finished for an FP64 solve
   bool refine_now = false;           // inner solve finished in this evaluation
   bool refined = false;              // (xk, yk) holds a refined point

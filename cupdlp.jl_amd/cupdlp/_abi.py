@@ -82,7 +82,8 @@ class pdlp_params(C.Structure):
         ("lane_budget", C.c_int32),
         ("slice_unroll", C.c_int32),
         ("slice_waves", C.c_int32),
-        ("reserved", C.c_int32 * 3),
+        ("refine_continue_progress", C.c_int32),
+        ("reserved", C.c_int32 * 2),
     ]
 
 

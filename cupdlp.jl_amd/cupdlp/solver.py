@@ -99,6 +99,7 @@ def to_c_params(p: PdhgParameters) -> _abi.pdlp_params:
     cp.lane_budget = int(p.lane_budget)
     cp.slice_unroll = int(p.slice_unroll)
     cp.slice_waves = int(p.slice_waves)
+    cp.refine_continue_progress = int(p.refine_continue_progress)
     return cp
 
 

@@ -226,6 +226,8 @@ class PdhgParameters:
     lane_budget: int = 0    # step-kernel tiles (0 = default; see pdlp_tune)
     slice_unroll: int = 0
     slice_waves: int = 0
+    refine_continue_progress: int = 0  # > 0: a capped correction solve that cut the
+                                       # error this many times continues the refinement
 
 
 @dataclass

@@ -86,13 +86,15 @@ mutable struct PdhgParameters
     batch_groups::Int32
     refine_early_fallback::Int32
     refine_warm_start::Int32
+    refine_continue_progress::Int32
 end
 
 # The reference's positional constructor (src/MOI_wrapper.jl:58-78 calls it
 # with eleven arguments): extensions take the library's defaults.
 PdhgParameters(a1, a2, a3, a4, a5, a6, a7, a8, a9, a10, a11) =
     PdhgParameters(a1, a2, a3, a4, a5, a6, a7, a8, a9, a10, a11,
-                   :fp64, Int32(12), 1e-4, 1e6, true, Int32(0), Int32(0), Int32(0), Int32(3))
+                   :fp64, Int32(12), 1e-4, 1e6, true, Int32(0), Int32(0), Int32(0), Int32(3),
+                   Int32(0))
 
 struct SaddlePointOutput
     primal_solution::Vector{Float64}
@@ -174,7 +176,8 @@ struct CParams
     lane_budget::Int32
     slice_unroll::Int32
     slice_waves::Int32
-    reserved::NTuple{3,Int32}
+    refine_continue_progress::Int32
+    reserved::NTuple{2,Int32}
 end
 
 struct CConvergence
@@ -267,7 +270,8 @@ function _c_params(p::PdhgParameters)
                    p.refine_max_outer, p.refine_inner_tolerance, p.refine_max_scale,
                    Int32(p.device_preconditioning), p.refine_inner_iteration_limit,
                    p.batch_groups, p.refine_early_fallback, 0, p.refine_warm_start,
-                   0, 0, 0, ntuple(_ -> Int32(0), 3))  # tiles: the library's defaults
+                   0, 0, 0,  # tiles: the library's defaults
+                   p.refine_continue_progress, ntuple(_ -> Int32(0), 2))
 end
 
 function _stats(s::CIterationStats)

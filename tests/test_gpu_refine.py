@@ -16,7 +16,7 @@ import lp_cases
 import oracle as O
 from cupdlp import (PdhgParameters, PdlpError, Precision, TerminationReason as TR,
                    construct_termination_criteria)
-from cupdlp.generators import config_lp, random_lp, transportation_lp
+from cupdlp.generators import batch_sizes, config_lp, random_lp, transportation_lp
 
 pytestmark = pytest.mark.gpu
 HERE = os.path.dirname(os.path.abspath(__file__))
@@ -251,3 +251,35 @@ def test_refined_batch_alone_vs_mixed(gpu_ctx):
         assert np.array_equal(a.primal_solution, b.primal_solution)
         assert np.array_equal(a.dual_solution, b.dual_solution)
         check_kkt(prob, b, 1e-8)
+
+
+def test_capped_round_continues_when_it_made_progress(gpu_ctx):
+    """refine_continue_progress = 10 on two LPs of the 512-LP batch (seeds
+    41 and 510) whose first correction solve stops at its 10,000-iteration
+    cap after cutting the refined point's error more than ten times: the
+    refinement goes on from that point instead of falling back at once.
+    Results are the same alone and in a batch (bitwise) and meet the
+    tolerance; more rounds run than with the fallback at the first capped
+    round, and seed 510 then refines to optimal without an FP64 solve."""
+    say("case: batch LPs 41 and 510, FP32 + refinement, continue when a capped round cut the error 10x")
+    sizes = batch_sizes(512)
+    probs = [random_lp(n=int(sizes[s]), m=int(sizes[s]) // 2, seed=s)[0] for s in (41, 510)]
+    p = params(1e-8, refine_continue_progress=10)
+    p0 = params(1e-8)
+    alone = [gpu_ctx.solve(q, p) for q in probs]
+    mixed = gpu_ctx.batch_solve(probs, p)
+    stop_at_once = [gpu_ctx.solve(q, p0) for q in probs]
+    say(f"  rounds {[o.refinement_rounds for o in alone]} (stall at once: "
+        f"{[o.refinement_rounds for o in stop_at_once]}), fallbacks "
+        f"{[o.fp64_fallback for o in alone]}")
+    for prob, a, b in zip(probs, alone, mixed):
+        assert a.termination_reason == b.termination_reason == TR.TERMINATION_REASON_OPTIMAL
+        assert a.iteration_count == b.iteration_count
+        assert a.refinement_rounds == b.refinement_rounds
+        assert a.fp64_fallback == b.fp64_fallback
+        assert np.array_equal(a.primal_solution, b.primal_solution)
+        assert np.array_equal(a.dual_solution, b.dual_solution)
+        check_kkt(prob, a, 1e-8)
+    assert all(o.fp64_fallback for o in stop_at_once)
+    assert all(a.refinement_rounds > s.refinement_rounds for a, s in zip(alone, stop_at_once))
+    assert not alone[1].fp64_fallback
