@@ -88,6 +88,7 @@ REFINE_EARLY = 0  # --refine-early-fallback (0: the library default)
 REFINE_WARM = -1  # --refine-warm-start (-1: the library default)
 REFINE_CAP = 0  # --refine-cap (0: the library default)
 REFINE_CONTINUE = -1  # --refine-continue (-1: the library default)
+OCCUPANCY = -1  # --slice-occupancy (-1: the tuner's choice or the library default)
 
 
 def make_params(tol, precision, iteration_limit=2**31 - 1, time_limit=float("inf")):
@@ -108,6 +109,8 @@ def make_params(tol, precision, iteration_limit=2**31 - 1, time_limit=float("inf
         p.refine_continue_progress = REFINE_CONTINUE
     for k, v in TILES.get(precision, {}).items():
         setattr(p, k, v)
+    if OCCUPANCY >= 0:
+        p.primal_occupancy = p.dual_occupancy = OCCUPANCY
     return p
 
 
@@ -119,7 +122,8 @@ def tune(ctx, problems, precision, tol, trial_iterations=100):
     """pdlp_tune on these LPs before anything is timed: sets TILES[precision]
     and returns the tuner's report."""
     t = ctx.tune(problems, make_params(tol, precision), trial_iterations)
-    TILES[precision] = {k: t[k] for k in ("lane_budget", "slice_unroll", "slice_waves")}
+    TILES[precision] = {k: t[k] for k in ("lane_budget", "slice_unroll", "slice_waves",
+                                          "primal_occupancy", "dual_occupancy")}
     log(f"tuned {precision}: {TILES[precision]} ({t['kernel_us_per_iteration']:.1f} us vs "
         f"{t['default_us_per_iteration']:.1f} us default per iteration, {t['candidates']} "
         f"candidates, {t['seconds']:.1f} s)")
@@ -626,6 +630,10 @@ def main():
     ap.add_argument("--refine-cap", type=int, default=0,
                     help="FP32 + refinement: iteration cap of each correction solve "
                          "(0 = the library default)")
+    ap.add_argument("--slice-occupancy", type=int, default=-1,
+                    help="most waves per SIMD of both slice kernels (0: the library's "
+                         "defaults; 8: no cap; -1: the tuner's choice, or the defaults "
+                         "without tuning)")
     ap.add_argument("--refine-continue", type=int, default=-1,
                     help="FP32 + refinement: a capped correction solve that cut the error this "
                          "many times continues the refinement (0: stall at once; -1: the "
@@ -658,9 +666,10 @@ def main():
     args = ap.parse_args()
     global BATCH_GROUPS, REFINE_INNER_TOL, REFINE_EARLY, REFINE_WARM
     REFINE_WARM = args.refine_warm_start
-    global REFINE_CAP, REFINE_CONTINUE
+    global REFINE_CAP, REFINE_CONTINUE, OCCUPANCY
     REFINE_CAP = args.refine_cap
     REFINE_CONTINUE = args.refine_continue
+    OCCUPANCY = args.slice_occupancy
     BATCH_GROUPS = args.batch_groups
     REFINE_INNER_TOL = args.refine_inner_tol
     REFINE_EARLY = args.refine_early_fallback

@@ -219,6 +219,11 @@ static int check_params(const pdlp_params *p, std::string *err) {
     *err = "slice_unroll must be 0, 4 or 8 and slice_waves 0, 1, 2 or 4";
     return 2;
   }
+  if (p->primal_occupancy < 0 || p->primal_occupancy > 16 || p->dual_occupancy < 0 ||
+      p->dual_occupancy > 16) {
+    *err = "primal_occupancy and dual_occupancy must be in [0, 16]";
+    return 2;
+  }
   return 0;
 }
 
@@ -876,13 +881,15 @@ extern "C" int pdlp_tune(pdlp_ctx *ctx, int32_t count, const pdlp_lp *lps,
     res[i].y = ys[i].data();
   }
   int ntrials = 0;
-  // k_primal + k_dual time per launch pair over trial_iterations iterations
+  // k_primal and k_dual time per launch over trial_iterations iterations
   // with tolerances no point meets, timed with the kernels' own events
-  auto trial = [&](int lb, int u, int w, double *us) -> int {
+  auto trial = [&](int lb, int u, int w, int po, int dov, double *pus, double *dus) -> int {
     pdlp_params p = *params;
     p.lane_budget = lb;
     p.slice_unroll = u;
     p.slice_waves = w;
+    p.primal_occupancy = po;
+    p.dual_occupancy = dov;
     p.record_iteration_stats = 0;
     p.termination_criteria.iteration_limit = trial_iterations;
     p.termination_criteria.eps_optimal_absolute = 1e-300;
@@ -904,26 +911,31 @@ extern "C" int pdlp_tune(pdlp_ctx *ctx, int32_t count, const pdlp_lp *lps,
     ctx->prof_fallback_timed = saved_fb;
     if (r) return r;
     ++ntrials;
-    *us = got.primal_launches > 0 ? (got.primal_ms + got.dual_ms) / (double)got.primal_launches * 1e3
-                                  : INFINITY;
+    const double nl = (double)got.primal_launches;
+    *pus = nl > 0 ? got.primal_ms / nl * 1e3 : INFINITY;
+    *dus = nl > 0 ? got.dual_ms / nl * 1e3 : INFINITY;
     return 0;
   };
   pdlp_tuning best{};
   best.lane_budget = 64;
   best.slice_unroll = 8;
   best.slice_waves = 4;
-  rc = trial(64, 8, 4, &best.kernel_us_per_iteration);
+  double bp = 0.0, bd = 0.0;  // per-kernel times of the current best
+  rc = trial(64, 8, 4, 0, 0, &bp, &bd);
   if (rc) return rc;
+  best.kernel_us_per_iteration = bp + bd;
   best.default_us_per_iteration = best.kernel_us_per_iteration;
   auto consider = [&](int lb, int u, int w) -> int {
-    double us = INFINITY;
-    int r = trial(lb, u, w, &us);
+    double pus = INFINITY, dus = INFINITY;
+    int r = trial(lb, u, w, 0, 0, &pus, &dus);
     if (r) return r;
-    if (us < 0.99 * best.kernel_us_per_iteration) {
+    if (pus + dus < 0.99 * best.kernel_us_per_iteration) {
       best.lane_budget = lb;
       best.slice_unroll = u;
       best.slice_waves = w;
-      best.kernel_us_per_iteration = us;
+      best.kernel_us_per_iteration = pus + dus;
+      bp = pus;
+      bd = dus;
     }
     return 0;
   };
@@ -931,6 +943,18 @@ extern "C" int pdlp_tune(pdlp_ctx *ctx, int32_t count, const pdlp_lp *lps,
     if ((rc = consider(lb, 8, 4))) return rc;
   if ((rc = consider(best.lane_budget, 8, 2))) return rc;
   if ((rc = consider(best.lane_budget, 4, best.slice_waves))) return rc;
+  // occupancy caps, each kernel by its own launches (0 = the default caps)
+  int po = 0, dov = 0;
+  for (int occ : {3, 4, 5, 6, 8}) {
+    double pus = INFINITY, dus = INFINITY;
+    if ((rc = trial(best.lane_budget, best.slice_unroll, best.slice_waves, occ, occ, &pus, &dus)))
+      return rc;
+    if (pus < 0.99 * bp) bp = pus, po = occ;
+    if (dus < 0.99 * bd) bd = dus, dov = occ;
+  }
+  best.primal_occupancy = po;
+  best.dual_occupancy = dov;
+  best.kernel_us_per_iteration = bp + bd;
   best.candidates = ntrials;
   best.seconds = now_sec() - t0;
   ctx->tune_cache[key] = best;

@@ -139,6 +139,19 @@ void DevicePool::release_all() {
 Engine::Engine(hipStream_t s, DevicePool *pool, const pdlp_params &prm, Profiler *prof)
     : s_(s), pool_(pool), prof_(prof) , prm_(prm) {
   prec_ = prm.precision == PDLP_PRECISION_FP32_REFINE ? 1 : 0;
+  // LDS reserved per slice-kernel work-group for the occupancy cap, from the
+  // current device's LDS per CU and per work-group
+  int dev = 0, per_cu = 0, per_block = 0;
+  if (hipGetDevice(&dev) == hipSuccess &&
+      hipDeviceGetAttribute(&per_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev) ==
+          hipSuccess &&
+      hipDeviceGetAttribute(&per_block, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) ==
+          hipSuccess) {
+    const int po = prm.primal_occupancy > 0 ? prm.primal_occupancy : kPrimalOccupancy;
+    const int dov = prm.dual_occupancy > 0 ? prm.dual_occupancy : kDualOccupancy;
+    primal_lds_ = slice_lds_bytes(po, prm.slice_waves, per_cu, per_block);
+    dual_lds_ = slice_lds_bytes(dov, prm.slice_waves, per_cu, per_block);
+  }
 }
 
 Engine::~Engine() {
@@ -913,12 +926,12 @@ int Engine::launch_attempts(int64_t count, std::string *err) {
     hipEvent_t *ev = timed ? &prof_->events[4 * a] : nullptr;
     int e = launch_primal(s_, prec_, nps_, d_pslices_, npl_, d_pslices_ + nps_, d_longs_,
                           d_long_cnt_, d_long_part_, d_insts_, d_ctl_, d_ppart_,
-                          prm_.slice_unroll, prm_.slice_waves, ev ? ev[0] : nullptr,
-                          ev ? ev[1] : nullptr);
+                          prm_.slice_unroll, prm_.slice_waves, primal_lds_,
+                          ev ? ev[0] : nullptr, ev ? ev[1] : nullptr);
     if (!e)
       e = launch_dual(s_, prec_, nds_, d_dslices_, ndl_, d_dslices_ + nds_, d_longs_,
                       d_long_cnt_, d_long_part_, d_insts_, d_ctl_, d_dpart_, prm_.slice_unroll,
-                      prm_.slice_waves, ev ? ev[2] : nullptr, ev ? ev[3] : nullptr);
+                      prm_.slice_waves, dual_lds_, ev ? ev[2] : nullptr, ev ? ev[3] : nullptr);
     if (!e)
       e = launch_step(s_, (int)inst_.size(), d_insts_, d_ctl_, d_ppart_, d_dpart_, d_red_, d_grow_,
                       tab_len_, prm_.reduction_exponent, prm_.growth_exponent,

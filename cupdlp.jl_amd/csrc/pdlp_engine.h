@@ -194,6 +194,8 @@ class Engine {
   bool pageable_host_ = false;
   pdlp_params prm_;
   int prec_ = 0;
+  // LDS reserved per work-group of the primal / dual slice kernels (occupancy caps)
+  unsigned primal_lds_ = 0, dual_lds_ = 0;
   std::vector<std::unique_ptr<HostInst>> inst_;
   std::vector<Ctl> ctl_;  // host mirror (pinned copy below)
   Ctl *ctl_pinned_ = nullptr;

@@ -14,18 +14,29 @@ namespace pdlp {
 // slices' in part).
 // unroll (4 or 8: loads in flight per lane) and waves (1, 2 or 4 per
 // work-group) pick the variant of the normal-slice kernel; the variants give
-// the same results.  Optional events e0 / e1 are recorded at the start of
+// the same results.  lds_bytes (unused by the kernel) is reserved per
+// work-group of the normal-slice kernel to cap how many share a CU
+// (slice_lds_bytes).  Optional events e0 / e1 are recorded at the start of
 // the first and the end of the last kernel's execution.
 int launch_primal(hipStream_t s, int prec, int nslices, const SliceDesc *slices, int nlong,
                   const SliceDesc *lslices, const LongRow *longs, int32_t *long_cnt,
                   double *long_part, const InstDev *insts, const Ctl *ctl, double *part,
-                  int unroll = 8, int waves = 4, hipEvent_t e0 = nullptr,
-                  hipEvent_t e1 = nullptr);
+                  int unroll = 8, int waves = 4, unsigned lds_bytes = 0,
+                  hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
+// Default occupancy caps of the slice kernels (waves per SIMD), measured on
+// MI355X: n = 1M LP FP64 k_dual 79 us at 3 vs 88 us uncapped, k_primal 91 us
+// at 4-5 vs 94; 512-LP batch 60.1 vs 52.7 solves/s (FP64).
+constexpr int kPrimalOccupancy = 5;
+constexpr int kDualOccupancy = 3;
+// LDS bytes per work-group of `waves` waves that hold the slice kernels to
+// at most `occupancy` waves per SIMD on a device with lds_per_cu bytes of
+// LDS per CU and lds_per_block per work-group (0 for occupancy <= 0).
+unsigned slice_lds_bytes(int occupancy, int waves, int lds_per_cu, int lds_per_block);
 int launch_dual(hipStream_t s, int prec, int nslices, const SliceDesc *slices, int nlong,
                 const SliceDesc *lslices, const LongRow *longs, int32_t *long_cnt,
                 double *long_part, const InstDev *insts, const Ctl *ctl, double *part,
-                int unroll = 8, int waves = 4, hipEvent_t e0 = nullptr,
-                hipEvent_t e1 = nullptr);
+                int unroll = 8, int waves = 4, unsigned lds_bytes = 0,
+                hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
 // Evaluation preparation over the row-block tables: pending weighted sums,
 // and A^T y of the current y when it is not valid.
 int launch_primal_flush(hipStream_t s, int prec, int nblocks, const BlockDesc *blocks,

@@ -362,6 +362,29 @@ __device__ __forceinline__ double group_sum(DD acc, int lg) {
   return dd_val(acc);
 }
 
+// The descriptor of wave-uniform slice wv as eight 32-bit words, loaded
+// together (one scalar load), its 16-bit fields unpacked from the fifth
+// (little-endian layout of SliceDesc: lg, then nrows).
+__device__ __forceinline__ SliceDesc load_slice(const SliceDesc *slices, int wv) {
+  static_assert(sizeof(SliceDesc) == 32, "SliceDesc is eight 32-bit words");
+  const gptr<int32_t> w = (gptr<int32_t>)(slices + wv);
+  const int32_t w0 = w[0], w1 = w[1], w2 = w[2], w3 = w[3];
+  const int32_t w4 = w[4], w5 = w[5], w6 = w[6], w7 = w[7];
+  // no instruction: keeps the words from being loaded one test at a time
+  asm volatile("" ::"s"(w0), "s"(w1), "s"(w2), "s"(w3), "s"(w4), "s"(w5), "s"(w6), "s"(w7));
+  SliceDesc sd;
+  sd.inst = w0;
+  sd.row0 = w1;
+  sd.off = w2;
+  sd.width = w3;
+  sd.lg = (int16_t)(w4 & 0xffff);
+  sd.nrows = (int16_t)((uint32_t)w4 >> 16);
+  sd.lid = w5;
+  sd.chunk = w6;
+  sd.part = w7;
+  return sd;
+}
+
 // Per-launch view of one instance for the primal step.
 template <typename T>
 struct PrimalView {
@@ -448,17 +471,24 @@ __device__ __forceinline__ void primal_slice(const SliceDesc &sd, int lane,
                                              double *part) {
   if (sd.nrows < 0) return;
   const Ctl *C = &ctl[sd.inst];
-  if (C->active == 0) return;
+  const int active = C->active;
   const InstDev &I = insts[sd.inst];
   const PrimalView<T> v(I, C);
+  const int32_t *scol = I.ts_col;
+  const T *sval = (const T *)I.ts_val;
+  // issue the control block's and the instance's loads together (the empty
+  // asm holds no instruction; it keeps them from being sunk below the test)
+  asm volatile("" ::"s"(active), "s"(v.yv), "s"(v.xv), "s"(v.cv), "s"(v.lv), "s"(v.uv),
+               "s"(v.xn), "s"(v.sx), "s"(v.saty), "s"(scol), "s"(sval), "s"(C->eta),
+               "s"(C->omega), "s"(C->w_pending), "s"(C->pending), "s"(C->aty_valid));
+  if (active == 0) return;
   const int j = sd.row0 + lane;
   const bool mine = lane < sd.nrows;
   PrimalRow r{};
   if (mine) r = primal_load(v, j);
   double s = 0.0;
   if (v.need) {
-    s = group_sum(slice_lane_sum<T, T, U>(I.ts_col, (const T *)I.ts_val, v.yv, sd.off, sd.width, lane),
-                  sd.lg);
+    s = group_sum(slice_lane_sum<T, T, U>(scol, sval, v.yv, sd.off, sd.width, lane), sd.lg);
     s = __shfl(s, (lane << sd.lg) & 63, 64);
   }
   DD acc = dd_zero();
@@ -476,9 +506,11 @@ __global__ __launch_bounds__(64 * W) void k_primal(const SliceDesc *__restrict__
                                                    const InstDev *__restrict__ insts,
                                                    const Ctl *__restrict__ ctl,
                                                    double *__restrict__ part) {
-  const int wv = (int)blockIdx.x * W + ((int)threadIdx.x >> 6);
+  // wave-uniform: the descriptor is one scalar load, and the wave's
+  // instance-wide values live in scalar registers
+  const int wv = __builtin_amdgcn_readfirstlane((int)blockIdx.x * W + ((int)threadIdx.x >> 6));
   if (wv >= nsl) return;
-  primal_slice<T, U>(slices[wv], threadIdx.x & 63, insts, ctl, part);
+  primal_slice<T, U>(load_slice(slices, wv), threadIdx.x & 63, insts, ctl, part);
 }
 
 // The chunk slices of rows longer than kSellChunk entries: each wave sums one
@@ -521,9 +553,10 @@ __global__ __launch_bounds__(kThreads) void k_primal_long(const SliceDesc *__res
 // applied to the reduced sums of one attempt; runs on one thread.
 // debug_attempt > 0: the attempt with that number sees movement = 0 (fault
 // injection of the numerical-error branch for tests).
+// fr_tab / fg_tab: the step factors of this attempt read from the host's
+// tables when the attempt number is below tab_len.
 __device__ __forceinline__ void step_rule(Ctl *C, double ssx, double ssy, double sxy,
-                                          const double *__restrict__ red_tab,
-                                          const double *__restrict__ grow_tab, int64_t tab_len,
+                                          double fr_tab, double fg_tab, int64_t tab_len,
                                           double red_exp, double grow_exp,
                                           int64_t debug_attempt) {
   const double omega = C->omega;
@@ -565,8 +598,8 @@ __device__ __forceinline__ void step_rule(Ctl *C, double ssx, double ssy, double
   }
   double fr, fg;
   if (k < tab_len) {
-    fr = red_tab[k];
-    fg = grow_tab[k];
+    fr = fr_tab;
+    fg = fg_tab;
   } else {
     const double k1 = (double)(k + 1);
     fr = 1.0 - 1.0 / pow(k1, red_exp);
@@ -697,15 +730,21 @@ __device__ __forceinline__ void dual_slice(const SliceDesc &sd, int lane,
                                            double *part) {
   if (sd.nrows < 0) return;
   const Ctl *C = &ctl[sd.inst];
-  if (C->active == 0) return;
+  const int active = C->active;
   const InstDev &I = insts[sd.inst];
   const DualView<T> v(I, C);
+  const int32_t *scol = I.as_col;
+  const T *sval = (const T *)I.as_val;
+  // as in primal_slice
+  asm volatile("" ::"s"(active), "s"(v.yv), "s"(v.bv), "s"(v.dxv), "s"(v.yn), "s"(v.axv),
+               "s"(v.axn), "s"(v.sy), "s"(v.sax), "s"(scol), "s"(sval), "s"(C->eta),
+               "s"(C->omega), "s"(C->w_pending), "s"(C->pending), "s"(I.neq));
+  if (active == 0) return;
   const int i = sd.row0 + lane;
   const bool mine = lane < sd.nrows;
   DualRow r{};
   if (mine) r = dual_load(v, i);
-  double prod = group_sum(
-      slice_lane_sum<T, T, U>(I.as_col, (const T *)I.as_val, v.dxv, sd.off, sd.width, lane), sd.lg);
+  double prod = group_sum(slice_lane_sum<T, T, U>(scol, sval, v.dxv, sd.off, sd.width, lane), sd.lg);
   prod = __shfl(prod, (lane << sd.lg) & 63, 64);
   DD acc_dy = dd_zero(), acc_int = dd_zero();
   if (mine) dual_update(v, i, r, prod, acc_dy, acc_int);
@@ -724,9 +763,11 @@ __global__ __launch_bounds__(64 * W) void k_dual(const SliceDesc *__restrict__ s
                                                  const InstDev *__restrict__ insts,
                                                  const Ctl *__restrict__ ctl,
                                                  double *__restrict__ part) {
-  const int wv = (int)blockIdx.x * W + ((int)threadIdx.x >> 6);
+  // wave-uniform: the descriptor is one scalar load, and the wave's
+  // instance-wide values live in scalar registers
+  const int wv = __builtin_amdgcn_readfirstlane((int)blockIdx.x * W + ((int)threadIdx.x >> 6));
   if (wv >= nsl) return;
-  dual_slice<T, U>(slices[wv], threadIdx.x & 63, insts, ctl, part);
+  dual_slice<T, U>(load_slice(slices, wv), threadIdx.x & 63, insts, ctl, part);
 }
 
 // Chunk slices of long rows of A (see k_primal_long).
@@ -792,24 +833,46 @@ __device__ __forceinline__ void step_instance(int q, const InstDev *__restrict__
                                               int constant, int64_t debug_attempt,
                                               double (*sc)[2 * kStepThreads / 64]) {
   Ctl *C = &ctl[q];
-  if (!C->active) return;
+  // The control block and the instance's partial ranges are independent
+  // loads: issue them together, then the step factors of the next attempt
+  // beside the partials, so the launch waits for two memory round trips
+  // before its reduction rather than four.
+  const gptr<Ctl> cg = gp(ctl) + q;
+  const gptr<InstDev> ig = gp(insts) + q;
+  const int active = cg->active;
+  const int64_t next = cg->attempts + 1;
+  int ps0 = ig->ps0, pl0 = ig->pl0, ds0 = ig->ds0, dl0 = ig->dl0;
+  int ps1 = ig->ps1, pl1 = ig->pl1, ds1 = ig->ds1, dl1 = ig->dl1;
+  // An empty asm that takes every value keeps the compiler from sinking the
+  // range loads below the test of `active` (it would issue them one round
+  // trip later); it holds no instruction.
+  asm volatile("" ::"s"(active), "s"(ps0), "s"(ps1), "s"(pl0), "s"(pl1), "s"(ds0), "s"(ds1),
+               "s"(dl0), "s"(dl1));
+  const bool run = active && !constant;
+  if (!run) ps1 = ps0, pl1 = pl0, ds1 = ds0, dl1 = dl0;
+  double fr = 0.0, fg = 0.0;
+  if (run && next < tab_len) {  // uniform: scalar loads, waited for at their use
+    fr = gp(red_tab)[next];
+    fg = gp(grow_tab)[next];
+  }
+  const gptr<double> pp = gp(ppart), dp = gp(dpart);
+  DD sx = dd_zero(), sy = dd_zero(), sxy = dd_zero();
+  for (int b = ps0 + (int)threadIdx.x; b < ps1; b += kStepThreads)
+    dd_merge(sx, DD{pp[2 * b], pp[2 * b + 1]});
+  for (int b = pl0 + (int)threadIdx.x; b < pl1; b += kStepThreads)
+    dd_merge(sx, DD{pp[2 * b], pp[2 * b + 1]});
+  for (int b = ds0 + (int)threadIdx.x; b < ds1; b += kStepThreads) {
+    dd_merge(sy, DD{dp[4 * b], dp[4 * b + 1]});
+    dd_merge(sxy, DD{dp[4 * b + 2], dp[4 * b + 3]});
+  }
+  for (int b = dl0 + (int)threadIdx.x; b < dl1; b += kStepThreads) {
+    dd_merge(sy, DD{dp[4 * b], dp[4 * b + 1]});
+    dd_merge(sxy, DD{dp[4 * b + 2], dp[4 * b + 3]});
+  }
+  if (!active) return;
   if (constant) {
     if (threadIdx.x == 0) step_constant(C);
     return;
-  }
-  const InstDev &I = insts[q];
-  DD sx = dd_zero(), sy = dd_zero(), sxy = dd_zero();
-  for (int b = I.ps0 + (int)threadIdx.x; b < I.ps1; b += kStepThreads)
-    dd_merge(sx, DD{ppart[2 * b], ppart[2 * b + 1]});
-  for (int b = I.pl0 + (int)threadIdx.x; b < I.pl1; b += kStepThreads)
-    dd_merge(sx, DD{ppart[2 * b], ppart[2 * b + 1]});
-  for (int b = I.ds0 + (int)threadIdx.x; b < I.ds1; b += kStepThreads) {
-    dd_merge(sy, DD{dpart[4 * b], dpart[4 * b + 1]});
-    dd_merge(sxy, DD{dpart[4 * b + 2], dpart[4 * b + 3]});
-  }
-  for (int b = I.dl0 + (int)threadIdx.x; b < I.dl1; b += kStepThreads) {
-    dd_merge(sy, DD{dpart[4 * b], dpart[4 * b + 1]});
-    dd_merge(sxy, DD{dpart[4 * b + 2], dpart[4 * b + 3]});
   }
   sx = wave_dd(sx);
   sy = wave_dd(sy);
@@ -827,8 +890,8 @@ __device__ __forceinline__ void step_instance(int q, const InstDev *__restrict__
     r[v] = DD{sc[v][0], sc[v][1]};
     for (int k = 1; k < kStepThreads / 64; ++k) dd_merge(r[v], DD{sc[v][2 * k], sc[v][2 * k + 1]});
   }
-  step_rule(C, dd_val(r[0]), dd_val(r[1]), dd_val(r[2]), red_tab, grow_tab, tab_len, red_exp,
-            grow_exp, debug_attempt);
+  step_rule(C, dd_val(r[0]), dd_val(r[1]), dd_val(r[2]), fr, fg, tab_len, red_exp, grow_exp,
+            debug_attempt);
 }
 
 __global__ __launch_bounds__(kStepThreads) void k_step(const InstDev *__restrict__ insts,
@@ -1471,12 +1534,23 @@ static inline unsigned grid_of_slices(int nsl) {
 // the kernel's own execution (hipExtLaunchKernelGGL), so profiled durations
 // exclude the dispatch gaps between kernels.
 template <typename F, typename... Args>
-static void launch_ev(F kernel, dim3 g, dim3 b, hipStream_t s, hipEvent_t e0, hipEvent_t e1,
-                      Args... args) {
+static void launch_ev(F kernel, dim3 g, dim3 b, unsigned shm, hipStream_t s, hipEvent_t e0,
+                      hipEvent_t e1, Args... args) {
   if (e0 || e1)
-    hipExtLaunchKernelGGL(kernel, g, b, 0, s, e0, e1, 0, args...);
+    hipExtLaunchKernelGGL(kernel, g, b, shm, s, e0, e1, 0, args...);
   else
-    hipLaunchKernelGGL(kernel, g, b, 0, s, args...);
+    hipLaunchKernelGGL(kernel, g, b, shm, s, args...);
+}
+
+unsigned slice_lds_bytes(int occupancy, int waves, int lds_per_cu, int lds_per_block) {
+  if (occupancy <= 0 || lds_per_cu <= 0) return 0;
+  const int w = waves == 1 || waves == 2 ? waves : 4;
+  // one wave of a work-group per SIMD (four SIMDs per CU): at most
+  // occupancy * 4 / w work-groups of w waves keep occupancy waves per SIMD
+  const int groups = std::max(1, occupancy * 4 / w);
+  unsigned bytes = (unsigned)(lds_per_cu / groups) & ~1023u;
+  if (lds_per_block > 0 && bytes > (unsigned)lds_per_block) bytes = (unsigned)lds_per_block & ~1023u;
+  return bytes;
 }
 
 // The slice-kernel variant: U in {4, 8} loads in flight per lane, W in
@@ -1512,7 +1586,7 @@ static inline unsigned grid_of_waves(int nsl, int w) {
 int launch_primal(hipStream_t s, int prec, int nslices, const SliceDesc *slices, int nlong,
                   const SliceDesc *lslices, const LongRow *longs, int32_t *long_cnt,
                   double *long_part, const InstDev *insts, const Ctl *ctl, double *part,
-                  int unroll, int waves, hipEvent_t e0, hipEvent_t e1) {
+                  int unroll, int waves, unsigned lds_bytes, hipEvent_t e0, hipEvent_t e1) {
   const dim3 b(kThreads);
   if (nslices <= 0 && nlong <= 0) {
     hipError_t r = e0 ? hipEventRecord(e0, s) : hipSuccess;
@@ -1523,14 +1597,14 @@ int launch_primal(hipStream_t s, int prec, int nslices, const SliceDesc *slices,
     hipEvent_t stop = nlong > 0 ? nullptr : e1;
     const int w = variant_waves(waves);
     launch_ev(primal_kernel(prec, unroll, waves), dim3(grid_of_waves(nslices, w)), dim3(64 * w),
-              s, e0, stop, slices, nslices, insts, ctl, part);
+              lds_bytes, s, e0, stop, slices, nslices, insts, ctl, part);
   }
   if (nlong > 0) {
     hipEvent_t start = nslices > 0 ? nullptr : e0;
     if (prec == 0)
-      launch_ev(k_primal_long<double>, dim3(grid_of_slices(nlong)), b, s, start, e1, lslices, nlong, longs, long_cnt, long_part, insts, ctl, part);
+      launch_ev(k_primal_long<double>, dim3(grid_of_slices(nlong)), b, 0, s, start, e1, lslices, nlong, longs, long_cnt, long_part, insts, ctl, part);
     else
-      launch_ev(k_primal_long<float>, dim3(grid_of_slices(nlong)), b, s, start, e1, lslices, nlong, longs, long_cnt, long_part, insts, ctl, part);
+      launch_ev(k_primal_long<float>, dim3(grid_of_slices(nlong)), b, 0, s, start, e1, lslices, nlong, longs, long_cnt, long_part, insts, ctl, part);
   }
   PDLP_CHECK_LAUNCH();
 }
@@ -1538,7 +1612,7 @@ int launch_primal(hipStream_t s, int prec, int nslices, const SliceDesc *slices,
 int launch_dual(hipStream_t s, int prec, int nslices, const SliceDesc *slices, int nlong,
                 const SliceDesc *lslices, const LongRow *longs, int32_t *long_cnt,
                 double *long_part, const InstDev *insts, const Ctl *ctl, double *part,
-                int unroll, int waves, hipEvent_t e0, hipEvent_t e1) {
+                int unroll, int waves, unsigned lds_bytes, hipEvent_t e0, hipEvent_t e1) {
   const dim3 b(kThreads);
   if (nslices <= 0 && nlong <= 0) {
     hipError_t r = e0 ? hipEventRecord(e0, s) : hipSuccess;
@@ -1548,15 +1622,15 @@ int launch_dual(hipStream_t s, int prec, int nslices, const SliceDesc *slices, i
   if (nslices > 0) {
     hipEvent_t stop = nlong > 0 ? nullptr : e1;
     const int w = variant_waves(waves);
-    launch_ev(dual_kernel(prec, unroll, waves), dim3(grid_of_waves(nslices, w)), dim3(64 * w), s,
-              e0, stop, slices, nslices, insts, ctl, part);
+    launch_ev(dual_kernel(prec, unroll, waves), dim3(grid_of_waves(nslices, w)), dim3(64 * w),
+              lds_bytes, s, e0, stop, slices, nslices, insts, ctl, part);
   }
   if (nlong > 0) {
     hipEvent_t start = nslices > 0 ? nullptr : e0;
     if (prec == 0)
-      launch_ev(k_dual_long<double>, dim3(grid_of_slices(nlong)), b, s, start, e1, lslices, nlong, longs, long_cnt, long_part, insts, ctl, part);
+      launch_ev(k_dual_long<double>, dim3(grid_of_slices(nlong)), b, 0, s, start, e1, lslices, nlong, longs, long_cnt, long_part, insts, ctl, part);
     else
-      launch_ev(k_dual_long<float>, dim3(grid_of_slices(nlong)), b, s, start, e1, lslices, nlong, longs, long_cnt, long_part, insts, ctl, part);
+      launch_ev(k_dual_long<float>, dim3(grid_of_slices(nlong)), b, 0, s, start, e1, lslices, nlong, longs, long_cnt, long_part, insts, ctl, part);
   }
   PDLP_CHECK_LAUNCH();
 }

@@ -83,7 +83,8 @@ class pdlp_params(C.Structure):
         ("slice_unroll", C.c_int32),
         ("slice_waves", C.c_int32),
         ("refine_continue_progress", C.c_int32),
-        ("reserved", C.c_int32 * 2),
+        ("primal_occupancy", C.c_int32),
+        ("dual_occupancy", C.c_int32),
     ]
 
 
@@ -150,7 +151,8 @@ class pdlp_tuning(C.Structure):
     _fields_ = [("lane_budget", C.c_int32), ("slice_unroll", C.c_int32),
                 ("slice_waves", C.c_int32), ("candidates", C.c_int32),
                 ("kernel_us_per_iteration", C.c_double),
-                ("default_us_per_iteration", C.c_double), ("seconds", C.c_double)]
+                ("default_us_per_iteration", C.c_double), ("seconds", C.c_double),
+                ("primal_occupancy", C.c_int32), ("dual_occupancy", C.c_int32)]
 
 
 STATS_CB = C.CFUNCTYPE(None, C.POINTER(pdlp_iteration_stats), C.c_void_p)

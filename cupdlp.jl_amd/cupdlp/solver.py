@@ -100,6 +100,8 @@ def to_c_params(p: PdhgParameters) -> _abi.pdlp_params:
     cp.slice_unroll = int(p.slice_unroll)
     cp.slice_waves = int(p.slice_waves)
     cp.refine_continue_progress = int(p.refine_continue_progress)
+    cp.primal_occupancy = int(p.primal_occupancy)
+    cp.dual_occupancy = int(p.dual_occupancy)
     return cp
 
 
@@ -212,8 +214,8 @@ class Context:
     def tune(self, problems, params, trial_iterations=100):
         """pdlp_tune: the step-kernel tile parameters for these LPs (batched
         with ``params``) from timed trials; a dict whose lane_budget,
-        slice_unroll and slice_waves go into PdhgParameters.  Cached per LP
-        shape on this context."""
+        slice_unroll, slice_waves, primal_occupancy and dual_occupancy go into
+        PdhgParameters.  Cached per LP shape on this context."""
         k = len(problems)
         lps = (_abi.pdlp_lp * k)()
         keeps = []

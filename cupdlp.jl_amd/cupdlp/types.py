@@ -226,6 +226,8 @@ class PdhgParameters:
     lane_budget: int = 0    # step-kernel tiles (0 = default; see pdlp_tune)
     slice_unroll: int = 0
     slice_waves: int = 0
+    primal_occupancy: int = 0  # most waves per SIMD of the slice kernels (0: the
+    dual_occupancy: int = 0    # defaults 5 / 3; 8 or more: no cap)
     refine_continue_progress: int = 0  # > 0: a capped correction solve that cut the
                                        # error this many times continues the refinement
 

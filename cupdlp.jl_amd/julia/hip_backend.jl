@@ -177,7 +177,8 @@ struct CParams
     slice_unroll::Int32
     slice_waves::Int32
     refine_continue_progress::Int32
-    reserved::NTuple{2,Int32}
+    primal_occupancy::Int32
+    dual_occupancy::Int32
 end
 
 struct CConvergence
@@ -271,7 +272,7 @@ function _c_params(p::PdhgParameters)
                    Int32(p.device_preconditioning), p.refine_inner_iteration_limit,
                    p.batch_groups, p.refine_early_fallback, 0, p.refine_warm_start,
                    0, 0, 0,  # tiles: the library's defaults
-                   p.refine_continue_progress, ntuple(_ -> Int32(0), 2))
+                   p.refine_continue_progress, 0, 0)  # occupancy caps: the defaults
 end
 
 function _stats(s::CIterationStats)

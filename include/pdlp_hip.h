@@ -202,7 +202,13 @@ typedef struct pdlp_params {
                                         512-LP batch at 1e-8 a second capped round
                                         rarely finishes (factor 10: 14 instead of 16
                                         fallbacks, 42.9 vs 47.9 solves/s) */
-  int32_t reserved[2];
+  int32_t primal_occupancy;          /* most waves per SIMD of the primal / dual slice
+                                        kernels, held by reserving LDS per work-group at
+                                        launch (fewer waves in flight thrash the caches
+                                        less with their gathers); 0 = the default (5
+                                        primal, 3 dual), 8 or more = as many as the
+                                        registers allow.  Changes no result */
+  int32_t dual_occupancy;
 } pdlp_params;
 
 /* ConvergenceInformation — src/solve_log.jl:51-155 (same field order). */
@@ -436,12 +442,16 @@ typedef struct pdlp_tuning {
   double kernel_us_per_iteration;   /* k_primal + k_dual per launch pair, the choice */
   double default_us_per_iteration;  /* the same with the defaults (64, 8, 4) */
   double seconds;                   /* wall time of the trials */
+  int32_t primal_occupancy;         /* the fields to copy into pdlp_params */
+  int32_t dual_occupancy;
 } pdlp_tuning;
 /* Solves the LPs (batched as pdlp_batch_solve would, with params) for
  * trial_iterations iterations per candidate and times their step kernels
  * with HIP events: lane budget 32 / 64 / 128, then 2 or 4 waves per
  * work-group, then 4 or 8 loads in flight, each step keeping a candidate
- * only if it is at least 1 % faster.  Writes the fastest to *out.  The
+ * only if it is at least 1 % faster; then, with the best of those, at most
+ * 3, 4, 5, 6 or 8 waves per SIMD, taken for the primal and the dual kernel
+ * separately (each the fastest of its own launches).  Writes the fastest to *out.  The
  * choice is cached on the context per LP shape (count, total n, m and
  * nonzeros, precision, size groups): a later call for the same shape
  * returns it at once.  Unroll and waves change no result; the lane budget
