@@ -88,7 +88,7 @@ REFINE_EARLY = 0  # --refine-early-fallback (0: the library default)
 REFINE_WARM = -1  # --refine-warm-start (-1: the library default)
 REFINE_CAP = 0  # --refine-cap (0: the library default)
 REFINE_CONTINUE = -1  # --refine-continue (-1: the library default)
-OCCUPANCY = -1  # --slice-occupancy (-1: the tuner's choice or the library default)
+OCCUPANCY = ()  # --slice-occupancy P,D (empty: the tuner's choice or the library default)
 
 
 def make_params(tol, precision, iteration_limit=2**31 - 1, time_limit=float("inf")):
@@ -109,8 +109,8 @@ def make_params(tol, precision, iteration_limit=2**31 - 1, time_limit=float("inf
         p.refine_continue_progress = REFINE_CONTINUE
     for k, v in TILES.get(precision, {}).items():
         setattr(p, k, v)
-    if OCCUPANCY >= 0:
-        p.primal_occupancy = p.dual_occupancy = OCCUPANCY
+    if OCCUPANCY:
+        p.primal_occupancy, p.dual_occupancy = OCCUPANCY
     return p
 
 
@@ -630,10 +630,10 @@ def main():
     ap.add_argument("--refine-cap", type=int, default=0,
                     help="FP32 + refinement: iteration cap of each correction solve "
                          "(0 = the library default)")
-    ap.add_argument("--slice-occupancy", type=int, default=-1,
-                    help="most waves per SIMD of both slice kernels (0: the library's "
-                         "defaults; 8: no cap; -1: the tuner's choice, or the defaults "
-                         "without tuning)")
+    ap.add_argument("--slice-occupancy", default="",
+                    help="P[,D]: most waves per SIMD of the primal [and dual] slice kernels "
+                         "(0: the library's defaults; 8: no cap; empty: the tuner's choice, "
+                         "or the defaults without tuning)")
     ap.add_argument("--refine-continue", type=int, default=-1,
                     help="FP32 + refinement: a capped correction solve that cut the error this "
                          "many times continues the refinement (0: stall at once; -1: the "
@@ -669,7 +669,9 @@ def main():
     global REFINE_CAP, REFINE_CONTINUE, OCCUPANCY
     REFINE_CAP = args.refine_cap
     REFINE_CONTINUE = args.refine_continue
-    OCCUPANCY = args.slice_occupancy
+    if args.slice_occupancy:
+        occ = [int(v) for v in args.slice_occupancy.split(",")]
+        OCCUPANCY = (occ[0], occ[-1])
     BATCH_GROUPS = args.batch_groups
     REFINE_INNER_TOL = args.refine_inner_tol
     REFINE_EARLY = args.refine_early_fallback

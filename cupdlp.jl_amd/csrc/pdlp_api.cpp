@@ -842,6 +842,11 @@ extern "C" int pdlp_batch_solve(pdlp_ctx *ctx, int32_t count, const pdlp_lp *lps
   return rc;
 }
 
+// A candidate replaces the current choice only when at least 3 % faster:
+// repeated trials of one configuration differ by 1-2 % on the 512-LP batch,
+// and a smaller margin switched between near-equal choices from run to run.
+constexpr double kTuneGain = 0.97;
+
 extern "C" int pdlp_tune(pdlp_ctx *ctx, int32_t count, const pdlp_lp *lps,
                          const pdlp_params *params, int32_t trial_iterations, pdlp_tuning *out) {
   if (!ctx) return 1;
@@ -929,7 +934,7 @@ extern "C" int pdlp_tune(pdlp_ctx *ctx, int32_t count, const pdlp_lp *lps,
     double pus = INFINITY, dus = INFINITY;
     int r = trial(lb, u, w, 0, 0, &pus, &dus);
     if (r) return r;
-    if (pus + dus < 0.99 * best.kernel_us_per_iteration) {
+    if (pus + dus < kTuneGain * best.kernel_us_per_iteration) {
       best.lane_budget = lb;
       best.slice_unroll = u;
       best.slice_waves = w;
@@ -949,8 +954,8 @@ extern "C" int pdlp_tune(pdlp_ctx *ctx, int32_t count, const pdlp_lp *lps,
     double pus = INFINITY, dus = INFINITY;
     if ((rc = trial(best.lane_budget, best.slice_unroll, best.slice_waves, occ, occ, &pus, &dus)))
       return rc;
-    if (pus < 0.99 * bp) bp = pus, po = occ;
-    if (dus < 0.99 * bd) bd = dus, dov = occ;
+    if (pus < kTuneGain * bp) bp = pus, po = occ;
+    if (dus < kTuneGain * bd) bd = dus, dov = occ;
   }
   best.primal_occupancy = po;
   best.dual_occupancy = dov;

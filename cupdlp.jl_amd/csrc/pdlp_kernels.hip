@@ -823,6 +823,57 @@ __device__ __forceinline__ void step_constant(Ctl *C) {
   if (C->accepted >= C->target) C->active = 0;
 }
 
+// This thread's share of the partials [b0, b1) of one kernel (S = 2: one
+// compensated pair per partial, into a; S = 4: two, into a and c), merged
+// in the order b = b0 + thread, + kStepThreads, ...  While eight more
+// remain, their loads are issued together, so a long range costs an eighth
+// of the memory round trips of one load per merge (the n = 1M LP: 31 primal
+// partials per thread); two to seven left are loaded together as well (the
+// slots past b1 re-read the last partial and are left out of the merge by a
+// select), and a single one alone.
+template <int S>
+__device__ __forceinline__ void merge_partial(const double *v, DD *a, DD *c) {
+  dd_merge(*a, DD{v[0], v[1]});
+  if constexpr (S == 4) dd_merge(*c, DD{v[2], v[3]});
+}
+template <int S>
+__device__ __forceinline__ void step_partials(gptr<double> part, int b0, int b1, DD *a, DD *c) {
+  constexpr int G = 8;
+  int b = b0 + (int)threadIdx.x;
+  for (; b + (G - 1) * kStepThreads < b1; b += G * kStepThreads) {
+    double v[G][S];
+#pragma unroll
+    for (int g = 0; g < G; ++g)
+#pragma unroll
+      for (int k = 0; k < S; ++k) v[g][k] = part[S * (b + g * kStepThreads) + k];
+#pragma unroll
+    for (int g = 0; g < G; ++g) merge_partial<S>(v[g], a, c);
+  }
+  if (b + kStepThreads < b1) {
+    double v[G][S];
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+      const int q = min(b + g * kStepThreads, b1 - 1);
+#pragma unroll
+      for (int k = 0; k < S; ++k) v[g][k] = part[S * q + k];
+    }
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+      DD ta = *a, tc = c ? *c : dd_zero();
+      merge_partial<S>(v[g], &ta, &tc);
+      if (b + g * kStepThreads < b1) {
+        *a = ta;
+        if constexpr (S == 4) *c = tc;
+      }
+    }
+  } else if (b < b1) {
+    double v[S];
+#pragma unroll
+    for (int k = 0; k < S; ++k) v[k] = part[S * b + k];
+    merge_partial<S>(v, a, c);
+  }
+}
+
 // The step of instance q by a work-group of kStepThreads threads; sc is
 // LDS scratch of 3 x 2 kStepThreads/64 doubles.  Contains a barrier.
 __device__ __forceinline__ void step_instance(int q, const InstDev *__restrict__ insts, Ctl *ctl,
@@ -857,18 +908,10 @@ __device__ __forceinline__ void step_instance(int q, const InstDev *__restrict__
   }
   const gptr<double> pp = gp(ppart), dp = gp(dpart);
   DD sx = dd_zero(), sy = dd_zero(), sxy = dd_zero();
-  for (int b = ps0 + (int)threadIdx.x; b < ps1; b += kStepThreads)
-    dd_merge(sx, DD{pp[2 * b], pp[2 * b + 1]});
-  for (int b = pl0 + (int)threadIdx.x; b < pl1; b += kStepThreads)
-    dd_merge(sx, DD{pp[2 * b], pp[2 * b + 1]});
-  for (int b = ds0 + (int)threadIdx.x; b < ds1; b += kStepThreads) {
-    dd_merge(sy, DD{dp[4 * b], dp[4 * b + 1]});
-    dd_merge(sxy, DD{dp[4 * b + 2], dp[4 * b + 3]});
-  }
-  for (int b = dl0 + (int)threadIdx.x; b < dl1; b += kStepThreads) {
-    dd_merge(sy, DD{dp[4 * b], dp[4 * b + 1]});
-    dd_merge(sxy, DD{dp[4 * b + 2], dp[4 * b + 3]});
-  }
+  step_partials<2>(pp, ps0, ps1, &sx, nullptr);
+  step_partials<2>(pp, pl0, pl1, &sx, nullptr);
+  step_partials<4>(dp, ds0, ds1, &sy, &sxy);
+  step_partials<4>(dp, dl0, dl1, &sy, &sxy);
   if (!active) return;
   if (constant) {
     if (threadIdx.x == 0) step_constant(C);

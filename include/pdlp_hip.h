@@ -449,7 +449,7 @@ typedef struct pdlp_tuning {
  * trial_iterations iterations per candidate and times their step kernels
  * with HIP events: lane budget 32 / 64 / 128, then 2 or 4 waves per
  * work-group, then 4 or 8 loads in flight, each step keeping a candidate
- * only if it is at least 1 % faster; then, with the best of those, at most
+ * only if it is at least 3 % faster; then, with the best of those, at most
  * 3, 4, 5, 6 or 8 waves per SIMD, taken for the primal and the dual kernel
  * separately (each the fastest of its own launches).  Writes the fastest to *out.  The
  * choice is cached on the context per LP shape (count, total n, m and
