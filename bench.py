@@ -67,14 +67,14 @@ METRIC = "solves/s + p99 latency to 1e-8 rel KKT, FP32+refine batched vs plain F
 # by a few hundredths of a percent between runs); the profiled ratio of
 # traffic to algorithmic bytes is then applied to this run's bytes.
 MEASURED_TRAFFIC = {
-    ("batch", "fp64", "k_dual"): (69370296.0, 102.709e6),
-    ("batch", "fp64", "k_primal"): (85988368.0, 104.363e6),
-    ("batch", "fp32", "k_dual"): (172086233.0, 251.715e6),
-    ("batch", "fp32", "k_primal"): (207272000.0, 263.418e6),
-    ("c1", "fp64", "k_dual"): (195460496.0, 283.94e6),
-    ("c1", "fp64", "k_primal"): (229460496.0, 235.38e6),
-    ("c1", "fp32", "k_dual"): (123640332.0, 171.69e6),
-    ("c2", "fp64", "k_primal"): (400032004.0, 421.45e6),
+    ("batch", "fp64", "k_dual"): (69376582.0, 96.257e6),
+    ("batch", "fp64", "k_primal"): (85985015.0, 104.653e6),
+    ("batch", "fp32", "k_dual"): (172089502.0, 250.579e6),
+    ("batch", "fp32", "k_primal"): (207292512.0, 264.746e6),
+    ("c1", "fp64", "k_dual"): (195460496.0, 285.789e6),
+    ("c1", "fp64", "k_primal"): (229460496.0, 242.441e6),
+    ("c1", "fp32", "k_dual"): (123640332.0, 175.597e6),
+    ("c2", "fp64", "k_primal"): (400032004.0, 423.443e6),
 }
 
 
