@@ -559,11 +559,15 @@ def run_single(args, ctx, lib, which):
     lib.pdlp_get_profile(ctx.handle, C.byref(prof))
     roof = profile_roofline(prof, ("k_primal", "k_dual"), f"config {which}, {args.precision}",
                             f"c{which}", args.precision)
+    # the same solve once more without the kernel events of the timed steps,
+    # which cost a launch-bound (small) LP a large share of its iteration
+    (out_u,), d_u, _ = solve_resident(ctx, [prob], params)
     n, m, nnz = prob.num_variables, prob.num_constraints, prob.constraint_matrix.nnz
     cfg = {"workload": f"single LP n={n} m={m} nnz={nnz}", "iteration_limit": args.iters,
            "tolerance": args.tol, "precision": args.precision}
     ci = out.iteration_stats[-1].convergence_information[0]
     extra = {"iterations_per_s": round(its / dt, 1), "iterations": its,
+             "iterations_per_s_unprofiled": round(out_u.iteration_count / d_u, 1),
              "termination": out.termination_string, "solve_time_s": round(out.solve_time_sec, 3),
              "relative_l2_primal_residual": ci.relative_l2_primal_residual,
              "relative_l2_dual_residual": ci.relative_l2_dual_residual,
