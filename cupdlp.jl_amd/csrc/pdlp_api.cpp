@@ -214,9 +214,10 @@ static int check_params(const pdlp_params *p, std::string *err) {
     *err = "lane_budget must be 0 or in [16, 1024]";
     return 2;
   }
-  if ((p->slice_unroll != 0 && p->slice_unroll != 4 && p->slice_unroll != 8) ||
+  if ((p->slice_unroll != 0 && p->slice_unroll != 4 && p->slice_unroll != 8 &&
+       p->slice_unroll != 16) ||
       (p->slice_waves != 0 && p->slice_waves != 1 && p->slice_waves != 2 && p->slice_waves != 4)) {
-    *err = "slice_unroll must be 0, 4 or 8 and slice_waves 0, 1, 2 or 4";
+    *err = "slice_unroll must be 0, 4, 8 or 16 and slice_waves 0, 1, 2 or 4";
     return 2;
   }
   if (p->primal_occupancy < 0 || p->primal_occupancy > 16 || p->dual_occupancy < 0 ||
@@ -923,10 +924,10 @@ extern "C" int pdlp_tune(pdlp_ctx *ctx, int32_t count, const pdlp_lp *lps,
   };
   pdlp_tuning best{};
   best.lane_budget = 64;
-  best.slice_unroll = 8;
+  best.slice_unroll = 0;  // each kernel's default
   best.slice_waves = 4;
   double bp = 0.0, bd = 0.0;  // per-kernel times of the current best
-  rc = trial(64, 8, 4, 0, 0, &bp, &bd);
+  rc = trial(64, 0, 4, 0, 0, &bp, &bd);
   if (rc) return rc;
   best.kernel_us_per_iteration = bp + bd;
   best.default_us_per_iteration = best.kernel_us_per_iteration;
@@ -945,9 +946,13 @@ extern "C" int pdlp_tune(pdlp_ctx *ctx, int32_t count, const pdlp_lp *lps,
     return 0;
   };
   for (int lb : {32, 128})
-    if ((rc = consider(lb, 8, 4))) return rc;
-  if ((rc = consider(best.lane_budget, 8, 2))) return rc;
+    if ((rc = consider(lb, 0, 4))) return rc;
+  if ((rc = consider(best.lane_budget, 0, 2))) return rc;
   if ((rc = consider(best.lane_budget, 4, best.slice_waves))) return rc;
+  // FP64: 8 for the dual kernel too (the default gives it 16)
+  if (params->precision != PDLP_PRECISION_FP32_REFINE && best.slice_unroll == 0 &&
+      (rc = consider(best.lane_budget, 8, best.slice_waves)))
+    return rc;
   // occupancy caps, each kernel by its own launches (0 = the default caps)
   int po = 0, dov = 0;
   for (int occ : {3, 4, 5, 6, 8}) {

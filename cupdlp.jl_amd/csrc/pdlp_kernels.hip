@@ -1596,9 +1596,13 @@ unsigned slice_lds_bytes(int occupancy, int waves, int lds_per_cu, int lds_per_b
   return bytes;
 }
 
-// The slice-kernel variant: U in {4, 8} loads in flight per lane, W in
-// {1, 2, 4} waves per work-group (anything else: 8 and 4).  The variants
-// compute the same sums in the same order.
+// The slice-kernel variant: U in {4, 8, 16} loads in flight per lane, W in
+// {1, 2, 4} waves per work-group (anything else: 4).  unroll 0 takes each
+// kernel's default: 8, except 16 for the FP64 dual kernel, whose rows (A's,
+// about twice as long as A^T's on the LPs measured) then issue a whole row's
+// gathers at once; at its default cap of 3 waves per SIMD its 103 VGPRs cost
+// no occupancy.  The FP32 dual kernel and both primal kernels are slower with
+// 16.  The variants compute the same sums in the same order.
 using SliceKernel = void (*)(const SliceDesc *, int, const InstDev *, const Ctl *, double *);
 template <typename T, int U>
 static SliceKernel primal_variant(int w) {
@@ -1611,13 +1615,23 @@ static SliceKernel dual_variant(int w) {
 static int variant_waves(int waves) { return waves == 1 || waves == 2 ? waves : 4; }
 static SliceKernel primal_kernel(int prec, int unroll, int waves) {
   const int w = variant_waves(waves);
-  if (prec == 0) return unroll == 4 ? primal_variant<double, 4>(w) : primal_variant<double, 8>(w);
-  return unroll == 4 ? primal_variant<float, 4>(w) : primal_variant<float, 8>(w);
+  if (prec == 0)
+    return unroll == 4    ? primal_variant<double, 4>(w)
+           : unroll == 16 ? primal_variant<double, 16>(w)
+                          : primal_variant<double, 8>(w);
+  return unroll == 4    ? primal_variant<float, 4>(w)
+         : unroll == 16 ? primal_variant<float, 16>(w)
+                        : primal_variant<float, 8>(w);
 }
 static SliceKernel dual_kernel(int prec, int unroll, int waves) {
   const int w = variant_waves(waves);
-  if (prec == 0) return unroll == 4 ? dual_variant<double, 4>(w) : dual_variant<double, 8>(w);
-  return unroll == 4 ? dual_variant<float, 4>(w) : dual_variant<float, 8>(w);
+  if (prec == 0)
+    return unroll == 4   ? dual_variant<double, 4>(w)
+           : unroll == 8 ? dual_variant<double, 8>(w)
+                         : dual_variant<double, 16>(w);
+  return unroll == 4    ? dual_variant<float, 4>(w)
+         : unroll == 16 ? dual_variant<float, 16>(w)
+                        : dual_variant<float, 8>(w);
 }
 static inline unsigned grid_of_waves(int nsl, int w) {
   return (unsigned)(nsl > 0 ? (nsl + w - 1) / w : 0);

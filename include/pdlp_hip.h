@@ -189,7 +189,9 @@ typedef struct pdlp_params {
                                         (the lanes per row follow; 16..1024, default 64);
                                         changes the slicing, hence the order of a row's
                                         compensated sum */
-  int32_t slice_unroll;              /* loads in flight per lane: 4 or 8 (default 8) */
+  int32_t slice_unroll;              /* loads in flight per lane: 4, 8 or 16 for both
+                                        slice kernels; 0 (default): 8, and 16 for the
+                                        FP64 dual kernel */
   int32_t slice_waves;               /* waves per work-group of the slice kernels: 1, 2
                                         or 4 (default 4); unroll and waves change no
                                         result */
@@ -440,7 +442,7 @@ typedef struct pdlp_tuning {
   int32_t slice_waves;
   int32_t candidates;               /* configurations timed (0: taken from the cache) */
   double kernel_us_per_iteration;   /* k_primal + k_dual per launch pair, the choice */
-  double default_us_per_iteration;  /* the same with the defaults (64, 8, 4) */
+  double default_us_per_iteration;  /* the same with the defaults (64, 0, 4) */
   double seconds;                   /* wall time of the trials */
   int32_t primal_occupancy;         /* the fields to copy into pdlp_params */
   int32_t dual_occupancy;
@@ -448,7 +450,8 @@ typedef struct pdlp_tuning {
 /* Solves the LPs (batched as pdlp_batch_solve would, with params) for
  * trial_iterations iterations per candidate and times their step kernels
  * with HIP events: lane budget 32 / 64 / 128, then 2 or 4 waves per
- * work-group, then 4 or 8 loads in flight, each step keeping a candidate
+ * work-group, then the default loads in flight or 4 (FP64: also 8 for the
+ * dual kernel as for the primal one), each step keeping a candidate
  * only if it is at least 3 % faster; then, with the best of those, at most
  * 3, 4, 5, 6 or 8 waves per SIMD, taken for the primal and the dual kernel
  * separately (each the fastest of its own launches).  Writes the fastest to *out.  The

@@ -1,7 +1,8 @@
 """The step kernels' tile parameters and pdlp_tune.
 
-* slice_unroll, slice_waves and the occupancy caps change only how the work
-  is scheduled: the results are bitwise those of the defaults.
+* slice_unroll (4, 8, 16 or each kernel's default), slice_waves and the
+  occupancy caps change only how the work is scheduled: the results are
+  bitwise those of the defaults.
 * lane_budget changes the slicing, hence the order of each row's
   compensated sum: the results are still bitwise equal to the CPU oracle's
   on these LPs (the bar of tests/test_gpu_parity.py).
@@ -35,13 +36,14 @@ def params(tol=1e-12, iteration_limit=2**31 - 1, **kw):
 
 
 def test_unroll_and_waves_change_no_result(gpu_ctx):
-    say("case: slice_unroll 4/8 x slice_waves 1/2/4 x occupancy caps, n=2000 LP and scaled "
-        "Zipf LP, FP64 and FP32")
+    say("case: slice_unroll 4/8/16 x slice_waves 1/2/4 x occupancy caps against the defaults "
+        "(unroll 8, 16 for the FP64 dual kernel), n=2000 LP and scaled Zipf LP, FP64 and FP32")
     for prob in (config_lp(0)[0], config_lp(1, scale=0.02)[0]):
         for prec in (Precision.FP64, Precision.FP32_REFINE):
             base = gpu_ctx.solve(prob, params(1e-8, iteration_limit=600, precision=prec))
             for u, w, po, do in ((4, 4, 0, 0), (8, 2, 0, 0), (4, 1, 0, 0), (8, 1, 0, 0),
-                                 (8, 4, 8, 8), (8, 4, 3, 6), (8, 2, 4, 2), (4, 1, 1, 16)):
+                                 (8, 4, 8, 8), (8, 4, 3, 6), (8, 2, 4, 2), (4, 1, 1, 16),
+                                 (8, 4, 0, 0), (16, 4, 0, 0), (16, 1, 5, 3)):
                 g = gpu_ctx.solve(prob, params(1e-8, iteration_limit=600, precision=prec,
                                                slice_unroll=u, slice_waves=w,
                                                primal_occupancy=po, dual_occupancy=do))
@@ -66,8 +68,9 @@ def test_tune_picks_a_candidate_and_caches_it(gpu_ctx):
              for k, n in enumerate([500, 900, 2000, 4000, 8000, 16000] * 4)]
     p = params(1e-6)
     t = gpu_ctx.tune(probs, p, trial_iterations=50)
-    assert t["candidates"] == 10
-    assert t["lane_budget"] in (32, 64, 128) and t["slice_unroll"] in (4, 8)
+    # FP64: one more unroll trial (8 for the dual kernel too) unless 4 was taken
+    assert t["candidates"] == (10 if t["slice_unroll"] == 4 else 11)
+    assert t["lane_budget"] in (32, 64, 128) and t["slice_unroll"] in (0, 4, 8)
     assert t["slice_waves"] in (2, 4)
     assert t["primal_occupancy"] in (0, 3, 4, 5, 6, 8) and t["dual_occupancy"] in (0, 3, 4, 5, 6, 8)
     assert 0 < t["kernel_us_per_iteration"] <= t["default_us_per_iteration"]
