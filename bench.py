@@ -67,14 +67,14 @@ METRIC = "solves/s + p99 latency to 1e-8 rel KKT, FP32+refine batched vs plain F
 # by a few hundredths of a percent between runs); the profiled ratio of
 # traffic to algorithmic bytes is then applied to this run's bytes.
 MEASURED_TRAFFIC = {
-    ("batch", "fp64", "k_dual"): (69376582.0, 96.257e6),
-    ("batch", "fp64", "k_primal"): (85985015.0, 104.653e6),
-    ("batch", "fp32", "k_dual"): (172089502.0, 250.579e6),
-    ("batch", "fp32", "k_primal"): (207292512.0, 264.746e6),
-    ("c1", "fp64", "k_dual"): (195460496.0, 285.789e6),
-    ("c1", "fp64", "k_primal"): (229460496.0, 242.441e6),
-    ("c1", "fp32", "k_dual"): (123640332.0, 175.597e6),
-    ("c2", "fp64", "k_primal"): (400032004.0, 423.443e6),
+    ("batch", "fp64", "k_dual"): (69376582.0, 96.260e6),
+    ("batch", "fp64", "k_primal"): (85985015.0, 104.654e6),
+    ("batch", "fp32", "k_dual"): (172089502.0, 250.583e6),
+    ("batch", "fp32", "k_primal"): (207292512.0, 264.742e6),
+    ("c1", "fp64", "k_dual"): (195460496.0, 287.163e6),
+    ("c1", "fp64", "k_primal"): (229460496.0, 242.544e6),
+    ("c1", "fp32", "k_dual"): (123640332.0, 175.638e6),
+    ("c2", "fp64", "k_primal"): (400032004.0, 423.450e6),
 }
 
 
@@ -537,11 +537,16 @@ def run_serve(args, ctx, lib):
     return value, dt, cfg, extra, None, probs
 
 
+SINGLE_LP_NAMES = {0: "n=2,000 random LP", 1: "n=1M Zipf-degree LP",
+                   2: "2,000 x 2,000 transportation LP"}
+
+
 def run_single(args, ctx, lib, which):
     import ctypes as C
     from cupdlp import _abi
     from cupdlp.generators import config_lp
-    log(f"generating config {which} ...")
+    name = SINGLE_LP_NAMES[which]
+    log(f"generating the {name} ...")
     prob, cert = config_lp(which)
     tuning = tune(ctx, [prob], args.precision, args.tol) if args.tune else None
     params = make_params(args.tol, args.precision, iteration_limit=args.iters,
@@ -557,13 +562,13 @@ def run_single(args, ctx, lib, which):
     lib.pdlp_set_profiling(ctx.handle, 0)
     prof = _abi.pdlp_kernel_profile()
     lib.pdlp_get_profile(ctx.handle, C.byref(prof))
-    roof = profile_roofline(prof, ("k_primal", "k_dual"), f"config {which}, {args.precision}",
+    roof = profile_roofline(prof, ("k_primal", "k_dual"), f"{name}, {args.precision}",
                             f"c{which}", args.precision)
     # the same solve once more without the kernel events of the timed steps,
     # which cost a launch-bound (small) LP a large share of its iteration
     (out_u,), d_u, _ = solve_resident(ctx, [prob], params)
     n, m, nnz = prob.num_variables, prob.num_constraints, prob.constraint_matrix.nnz
-    cfg = {"workload": f"single LP n={n} m={m} nnz={nnz}", "iteration_limit": args.iters,
+    cfg = {"workload": f"{name}: n={n} m={m} nnz={nnz}", "iteration_limit": args.iters,
            "tolerance": args.tol, "precision": args.precision}
     ci = out.iteration_stats[-1].convergence_information[0]
     extra = {"iterations_per_s": round(its / dt, 1), "iterations": its,
