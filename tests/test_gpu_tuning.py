@@ -50,6 +50,22 @@ def test_unroll_and_waves_change_no_result(gpu_ctx):
                 compare_runs(g, base)
 
 
+def test_tile_parameters_out_of_range_are_refused(gpu_ctx):
+    say("case: tile parameters out of range refused (code 2) before any device work; "
+        "slice_unroll 16 accepted")
+    from cupdlp.solver import PdlpError
+    prob = config_lp(0, scale=0.1)[0]
+    for kw in ({"slice_unroll": 12}, {"slice_unroll": 2}, {"slice_waves": 3},
+               {"lane_budget": 8}, {"lane_budget": 2048}, {"primal_occupancy": 17},
+               {"dual_occupancy": -1}):
+        with pytest.raises(PdlpError) as ei:
+            gpu_ctx.solve(prob, params(1e-6, iteration_limit=10, **kw))
+        assert ei.value.code == 2, (kw, ei.value.code)
+    ok = gpu_ctx.solve(prob, params(1e-12, iteration_limit=10, slice_unroll=16))
+    assert ok.termination_reason == TR.TERMINATION_REASON_ITERATION_LIMIT
+    assert ok.iteration_count == 10
+
+
 @pytest.mark.parametrize("lane_budget", [16, 32, 128])
 def test_lane_budget_matches_the_oracle(gpu_ctx, lane_budget):
     say(f"case: lane_budget {lane_budget} vs the oracle (n=2000 LP to 1e-8, 60x60 "
