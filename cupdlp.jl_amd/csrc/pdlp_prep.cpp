@@ -143,6 +143,34 @@ int load_problem(const pdlp_lp *lp, Prepared *P, std::string *err, bool build_cs
       P->t_val[q] = lp->nzval[k];
     }
     P->t_ptr[jn + 1] = q;
+    // row indices of the column ascending in the new numbering, as the
+    // column indices of each row of A are: the k-th gathers of a slice's
+    // rows then fall in one band of the gathered vector
+    const int32_t c0 = P->t_ptr[jn];
+    if (q - c0 <= 32) {
+      for (int32_t a = c0 + 1; a < q; ++a) {  // insertion sort
+        const int32_t r = P->t_col[a];
+        const double v = P->t_val[a];
+        int32_t b = a;
+        for (; b > c0 && P->t_col[b - 1] > r; --b) {
+          P->t_col[b] = P->t_col[b - 1];
+          P->t_val[b] = P->t_val[b - 1];
+        }
+        P->t_col[b] = r;
+        P->t_val[b] = v;
+      }
+    } else {
+      std::vector<std::pair<int32_t, double>> e((size_t)(q - c0));
+      for (int32_t a = c0; a < q; ++a) e[a - c0] = {P->t_col[a], P->t_val[a]};
+      std::sort(e.begin(), e.end(),
+                [](const std::pair<int32_t, double> &x, const std::pair<int32_t, double> &y) {
+                  return x.first < y.first;
+                });
+      for (int32_t a = c0; a < q; ++a) {
+        P->t_col[a] = e[a - c0].first;
+        P->t_val[a] = e[a - c0].second;
+      }
+    }
   }
   P->oc.resize(n); P->ol.resize(n); P->ou.resize(n); P->ob.resize(m);
   for (int64_t jn = 0; jn < n; ++jn) {
