@@ -67,14 +67,14 @@ METRIC = "solves/s + p99 latency to 1e-8 rel KKT, FP32+refine batched vs plain F
 # by a few hundredths of a percent between runs); the profiled ratio of
 # traffic to algorithmic bytes is then applied to this run's bytes.
 MEASURED_TRAFFIC = {
-    ("batch", "fp64", "k_dual"): (69376582.0, 96.260e6),
+    ("batch", "fp64", "k_dual"): (69376582.0, 96.262e6),
     ("batch", "fp64", "k_primal"): (85985015.0, 104.654e6),
     ("batch", "fp32", "k_dual"): (172089502.0, 250.583e6),
     ("batch", "fp32", "k_primal"): (207292512.0, 264.742e6),
-    ("c1", "fp64", "k_dual"): (195460496.0, 287.163e6),
-    ("c1", "fp64", "k_primal"): (229460496.0, 242.544e6),
-    ("c1", "fp32", "k_dual"): (123640332.0, 175.638e6),
-    ("c2", "fp64", "k_primal"): (400032004.0, 423.450e6),
+    ("c1", "fp64", "k_dual"): (195460496.0, 286.819e6),
+    ("c1", "fp64", "k_primal"): (229460496.0, 247.593e6),
+    ("c1", "fp32", "k_dual"): (123640332.0, 175.487e6),
+    ("c2", "fp64", "k_primal"): (400032004.0, 423.452e6),
 }
 
 
