@@ -40,6 +40,23 @@ def no_rows():
                                        np.array([1.0, -1.0]), 0.0, A, np.zeros(0), 0), -2.0
 
 
+def empty_row_and_column_lp(n=300, m=150, seed=31):
+    """A random LP plus an empty equality row (b = 0, first), an empty ">="
+    row (b = -1, last) and an empty column (cost 1 on [0, 5], so x = 0):
+    the generator's certificate, extended by zeros, stays optimal."""
+    from cupdlp.generators import random_lp
+    prob, cert = random_lp(n=n, m=m, num_eq=m // 3, seed=seed)
+    A = sp.csc_matrix(prob.constraint_matrix)
+    A = sp.hstack([A, sp.csc_matrix((m, 1))]).tocsc()
+    A = sp.vstack([sp.csr_matrix((1, n + 1)), A, sp.csr_matrix((1, n + 1))]).tocsc()
+    b = np.concatenate([[0.0], prob.right_hand_side, [-1.0]])
+    c = np.append(prob.objective_vector, 1.0)
+    lb = np.append(prob.variable_lower_bound, 0.0)
+    ub = np.append(prob.variable_upper_bound, 5.0)
+    p = QuadraticProgrammingProblem(lb, ub, None, c, 0.0, A, b, prob.num_equalities + 1)
+    return p, cert.objective
+
+
 def long_row_lp(n=6000, seed=7):
     """A random LP with one dense row of n entries (longer than a row block)
     plus sparse rows; certified by construction."""

@@ -165,6 +165,24 @@ def test_no_rows(gpu_ctx):
     assert abs(g.iteration_stats[-1].convergence_information[0].primal_objective - opt) < 1e-6
 
 
+def test_empty_row_and_column(gpu_ctx):
+    say("case: an empty equality row, an empty >= row and an empty column, no presolve; "
+        "FP64 bitwise vs the oracle, FP32 + refinement to the certificate")
+    from cupdlp import Precision
+    prob, obj = lp_cases.empty_row_and_column_lp()
+    g = gpu_ctx.solve(prob, params(1e-8))
+    o = O.solve(prob, params(1e-8))
+    compare_runs(g, o)
+    assert g.termination_reason == TR.TERMINATION_REASON_OPTIMAL
+    assert g.primal_solution[-1] == 0.0
+    assert abs(g.iteration_stats[-1].convergence_information[0].primal_objective - obj) \
+        <= 1e-6 * (1 + abs(obj))
+    f = gpu_ctx.solve(prob, params(1e-8, precision=Precision.FP32_REFINE))
+    assert f.termination_reason == TR.TERMINATION_REASON_OPTIMAL
+    assert abs(f.iteration_stats[-1].convergence_information[0].primal_objective - obj) \
+        <= 1e-6 * (1 + abs(obj))
+
+
 def test_iteration_limit_semantics(gpu_ctx):
     say("case: iteration_limit = 37")
     prob, _ = config_lp(0, scale=0.25)

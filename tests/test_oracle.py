@@ -71,6 +71,20 @@ def test_no_rows_defined():
     np.testing.assert_allclose(out.primal_solution, [0.0, 2.0], atol=1e-9)
 
 
+def test_empty_row_and_column_reach_the_certificate():
+    """Empty equality and ">=" rows and an empty column, straight to the
+    solver: the free dual of the empty equality row and the dual of the
+    slack ">=" row stay 0, the empty column goes to its lower bound, and the
+    objective is the certificate's."""
+    prob, obj = lp_cases.empty_row_and_column_lp()
+    out = O.solve(prob, params(1e-8))
+    assert out.termination_reason == TR.TERMINATION_REASON_OPTIMAL
+    assert out.primal_solution[-1] == 0.0
+    assert out.dual_solution[0] == 0.0 and out.dual_solution[-1] == 0.0
+    po = out.iteration_stats[-1].convergence_information[0].primal_objective
+    assert abs(po - obj) <= 1e-6 * (1 + abs(obj))
+
+
 @pytest.mark.parametrize("name,make,tol", [
     ("c0_seed0", lambda: config_lp(0), 1e-8),
     ("random_n300_s11", lambda: random_lp(n=300, m=150, seed=11), 1e-8),
