@@ -57,6 +57,20 @@ def empty_row_and_column_lp(n=300, m=150, seed=31):
     return p, cert.objective
 
 
+def fixed_variables_lp(n=300, m=150, seed=43, every=10):
+    """A random LP with every `every`-th variable fixed (lb = ub) at its
+    certified optimal value: the certificate stays optimal."""
+    from cupdlp.generators import random_lp
+    prob, cert = random_lp(n=n, m=m, seed=seed)
+    lb, ub = prob.variable_lower_bound.copy(), prob.variable_upper_bound.copy()
+    lb[::every] = cert.x[::every]
+    ub[::every] = cert.x[::every]
+    p = QuadraticProgrammingProblem(lb, ub, None, prob.objective_vector, 0.0,
+                                    prob.constraint_matrix, prob.right_hand_side,
+                                    prob.num_equalities)
+    return p, cert
+
+
 def long_row_lp(n=6000, seed=7):
     """A random LP with one dense row of n entries (longer than a row block)
     plus sparse rows; certified by construction."""

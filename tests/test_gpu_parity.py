@@ -183,6 +183,17 @@ def test_empty_row_and_column(gpu_ctx):
         <= 1e-6 * (1 + abs(obj))
 
 
+def test_inequality_only_and_fixed_variables(gpu_ctx):
+    say("case: an LP with no equality rows, and one with a tenth of its variables fixed "
+        "(lb = ub), FP64 bitwise vs the oracle")
+    for prob in (random_lp(n=300, m=150, num_eq=0, seed=41)[0],
+                 lp_cases.fixed_variables_lp()[0]):
+        g = gpu_ctx.solve(prob, params(1e-8))
+        o = O.solve(prob, params(1e-8))
+        compare_runs(g, o)
+        assert g.termination_reason == TR.TERMINATION_REASON_OPTIMAL
+
+
 def test_iteration_limit_semantics(gpu_ctx):
     say("case: iteration_limit = 37")
     prob, _ = config_lp(0, scale=0.25)

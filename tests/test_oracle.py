@@ -85,6 +85,22 @@ def test_empty_row_and_column_reach_the_certificate():
     assert abs(po - obj) <= 1e-6 * (1 + abs(obj))
 
 
+def test_inequality_only_and_fixed_variables_reach_the_certificate():
+    """No equality rows at all; and a tenth of the variables fixed (lb = ub)
+    at their certified values, which keeps the certificate optimal."""
+    prob, cert = random_lp(n=300, m=150, num_eq=0, seed=41)
+    out = O.solve(prob, params(1e-8))
+    assert out.termination_reason == TR.TERMINATION_REASON_OPTIMAL
+    po = out.iteration_stats[-1].convergence_information[0].primal_objective
+    assert abs(po - cert.objective) <= 1e-6 * (1 + abs(cert.objective))
+    prob, cert = lp_cases.fixed_variables_lp()
+    out = O.solve(prob, params(1e-8))
+    assert out.termination_reason == TR.TERMINATION_REASON_OPTIMAL
+    np.testing.assert_allclose(out.primal_solution[::10], cert.x[::10], rtol=1e-12, atol=1e-300)
+    po = out.iteration_stats[-1].convergence_information[0].primal_objective
+    assert abs(po - cert.objective) <= 1e-6 * (1 + abs(cert.objective))
+
+
 @pytest.mark.parametrize("name,make,tol", [
     ("c0_seed0", lambda: config_lp(0), 1e-8),
     ("random_n300_s11", lambda: random_lp(n=300, m=150, seed=11), 1e-8),
