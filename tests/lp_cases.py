@@ -71,6 +71,29 @@ def fixed_variables_lp(n=300, m=150, seed=43, every=10):
     return p, cert
 
 
+def one_sided_bounds_lp(n=300, m=150, seed=47):
+    """A random LP whose [0, 10] variables lose the bound their certificate
+    does not use: at 0 with r > 0 -> [0, inf); at 10 with r < 0 -> (-inf, 10];
+    interior (r = 0) -> alternately (-inf, 10] and [0, inf).  Every bound the
+    certificate's complementarity needs is kept, so it stays optimal."""
+    from cupdlp.generators import random_lp
+    prob, cert = random_lp(n=n, m=m, seed=seed)
+    lb, ub = prob.variable_lower_bound.copy(), prob.variable_upper_bound.copy()
+    x, r = cert.x, cert.r
+    bounded = np.isfinite(lb) & np.isfinite(ub)
+    at_lo = bounded & (x == lb) & (r > 0)
+    at_hi = bounded & (x == ub) & (r < 0)
+    inner = np.flatnonzero(bounded & ~at_lo & ~at_hi)
+    ub[at_lo] = INF
+    lb[at_hi] = -INF
+    lb[inner[0::2]] = -INF
+    ub[inner[1::2]] = INF
+    p = QuadraticProgrammingProblem(lb, ub, None, prob.objective_vector, 0.0,
+                                    prob.constraint_matrix, prob.right_hand_side,
+                                    prob.num_equalities)
+    return p, cert
+
+
 def long_row_lp(n=6000, seed=7):
     """A random LP with one dense row of n entries (longer than a row block)
     plus sparse rows; certified by construction."""

@@ -184,10 +184,10 @@ def test_empty_row_and_column(gpu_ctx):
 
 
 def test_inequality_only_and_fixed_variables(gpu_ctx):
-    say("case: an LP with no equality rows, and one with a tenth of its variables fixed "
-        "(lb = ub), FP64 bitwise vs the oracle")
+    say("case: an LP with no equality rows, one with a tenth of its variables fixed "
+        "(lb = ub), one with one-sided bounds only, FP64 bitwise vs the oracle")
     for prob in (random_lp(n=300, m=150, num_eq=0, seed=41)[0],
-                 lp_cases.fixed_variables_lp()[0]):
+                 lp_cases.fixed_variables_lp()[0], lp_cases.one_sided_bounds_lp()[0]):
         g = gpu_ctx.solve(prob, params(1e-8))
         o = O.solve(prob, params(1e-8))
         compare_runs(g, o)

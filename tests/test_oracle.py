@@ -86,8 +86,9 @@ def test_empty_row_and_column_reach_the_certificate():
 
 
 def test_inequality_only_and_fixed_variables_reach_the_certificate():
-    """No equality rows at all; and a tenth of the variables fixed (lb = ub)
-    at their certified values, which keeps the certificate optimal."""
+    """No equality rows at all; a tenth of the variables fixed (lb = ub) at
+    their certified values; one-sided bounds only: each keeps the
+    certificate optimal."""
     prob, cert = random_lp(n=300, m=150, num_eq=0, seed=41)
     out = O.solve(prob, params(1e-8))
     assert out.termination_reason == TR.TERMINATION_REASON_OPTIMAL
@@ -97,6 +98,14 @@ def test_inequality_only_and_fixed_variables_reach_the_certificate():
     out = O.solve(prob, params(1e-8))
     assert out.termination_reason == TR.TERMINATION_REASON_OPTIMAL
     np.testing.assert_allclose(out.primal_solution[::10], cert.x[::10], rtol=1e-12, atol=1e-300)
+    po = out.iteration_stats[-1].convergence_information[0].primal_objective
+    assert abs(po - cert.objective) <= 1e-6 * (1 + abs(cert.objective))
+    # one-sided bounds [0, inf) and (-inf, 10] (and free), certificate kept
+    prob, cert = lp_cases.one_sided_bounds_lp()
+    lb, ub = prob.variable_lower_bound, prob.variable_upper_bound
+    assert np.any(np.isfinite(lb) & ~np.isfinite(ub)) and np.any(~np.isfinite(lb) & np.isfinite(ub))
+    out = O.solve(prob, params(1e-8))
+    assert out.termination_reason == TR.TERMINATION_REASON_OPTIMAL
     po = out.iteration_stats[-1].convergence_information[0].primal_objective
     assert abs(po - cert.objective) <= 1e-6 * (1 + abs(cert.objective))
 
