@@ -263,6 +263,28 @@ def test_batch_alone_vs_mixed_bitwise(gpu_ctx):
         assert np.array_equal(a.dual_solution, b.dual_solution)
 
 
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_batch_of_degenerate_shapes_alone_vs_mixed(gpu_ctx, precision):
+    say(f"case: batch of 9 degenerate / structured LPs vs alone, {precision} (empty rows "
+        "and column, fixed and one-sided bounds, no equalities, long column, long row, "
+        "transportation, m = 0)")
+    from cupdlp import Precision
+    probs = [lp_cases.empty_row_and_column_lp()[0], lp_cases.fixed_variables_lp()[0],
+             lp_cases.one_sided_bounds_lp()[0], random_lp(n=300, m=150, num_eq=0, seed=41)[0],
+             lp_cases.long_col_lp(3000)[0], lp_cases.long_row_lp(6000)[0],
+             transportation_lp(30, 40, seed=4), lp_cases.no_rows()[0],
+             random_lp(n=900, m=450, seed=15)[0]]
+    prec = Precision.FP64 if precision == "fp64" else Precision.FP32_REFINE
+    p = params(1e-6, precision=prec)
+    alone = [gpu_ctx.solve(q, p) for q in probs]
+    mixed = gpu_ctx.batch_solve(probs, p)
+    for k, (a, b) in enumerate(zip(alone, mixed)):
+        assert a.termination_reason == b.termination_reason == TR.TERMINATION_REASON_OPTIMAL, k
+        assert a.iteration_count == b.iteration_count, k
+        assert np.array_equal(a.primal_solution, b.primal_solution), k
+        assert np.array_equal(a.dual_solution, b.dual_solution), k
+
+
 def test_constant_step_policy(gpu_ctx):
     """ConstantStepsizeParams: step (1 - 0.2) / sigma_max from the power method
     (start vector SplitMix64 + Box-Muller on both sides), every attempt taken;
