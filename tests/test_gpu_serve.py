@@ -10,6 +10,7 @@ the reference for "alone".
 import numpy as np
 import pytest
 
+import lp_cases
 from cupdlp import PdhgParameters, PdlpError, Precision, Server, construct_termination_criteria
 from cupdlp.generators import random_lp, transportation_lp
 
@@ -75,6 +76,33 @@ def test_serve_alone_vs_mixed(gpu_ctx):
         assert out[rid].termination_string == "OPTIMAL"
         same_result(out[rid], alone[k])
         assert out[rid].solve_time_sec > 0.0
+
+
+@pytest.mark.parametrize("precision", [Precision.FP64, Precision.FP32_REFINE])
+def test_serve_degenerate_shapes_alone_vs_mixed(gpu_ctx, precision):
+    print(f"case: served stream of degenerate / structured LPs vs alone, {precision.name} "
+          "(empty rows and column, fixed and one-sided bounds, no equalities, long column "
+          "and row, m = 0), tolerances 1e-4 / 1e-8", flush=True)
+    probs = [lp_cases.empty_row_and_column_lp()[0], lp_cases.fixed_variables_lp()[0],
+             lp_cases.one_sided_bounds_lp()[0], random_lp(n=300, m=150, num_eq=0, seed=41)[0],
+             lp_cases.long_col_lp(3000)[0], lp_cases.long_row_lp(6000)[0],
+             lp_cases.no_rows()[0]]
+    tols = [1e-8 if i % 2 else 1e-4 for i in range(len(probs))]
+    alone = [gpu_ctx.solve(p, params(t, precision)) for p, t in zip(probs, tols)]
+    srv = Server(gpu_ctx, params(1e-4, precision))
+    try:
+        out = {}
+        ids = [srv.submit(probs[i], tols[i]) for i in range(4)]
+        for _ in range(3):
+            for f in srv.poll():
+                out[f] = srv.result(f)
+        ids += [srv.submit(probs[i], tols[i]) for i in range(4, len(probs))]
+        drain(srv, ids, out)
+    finally:
+        srv.close()
+    for k, rid in enumerate(ids):
+        assert out[rid].termination_string == "OPTIMAL", k
+        same_result(out[rid], alone[k])
 
 
 def test_serve_one_slot_and_budget(gpu_ctx):
