@@ -66,6 +66,20 @@ def test_noalloc_matches_solve_on_two_calls(gpu_ctx):
     same(gpu_ctx.solve_noalloc(prob, params(1e-8)), gpu_ctx.solve(prob, params(1e-8)))
 
 
+def test_noalloc_degenerate_shapes(gpu_ctx):
+    print("case: pdlp_solve_noalloc vs pdlp_solve on degenerate / structured LPs (empty "
+          "rows and column, fixed and one-sided bounds, long column and row, m = 0)",
+          flush=True)
+    import lp_cases
+    probs = [lp_cases.empty_row_and_column_lp()[0], lp_cases.fixed_variables_lp()[0],
+             lp_cases.one_sided_bounds_lp()[0], lp_cases.long_col_lp(3000)[0],
+             lp_cases.long_row_lp(6000)[0], lp_cases.no_rows()[0]]
+    for k, prob in enumerate(probs):
+        a = gpu_ctx.solve_noalloc(prob, params(1e-8))
+        same(a, gpu_ctx.solve(prob, params(1e-8)))
+        assert a.termination_reason == TR.TERMINATION_REASON_OPTIMAL, k
+
+
 def test_noalloc_refusals(gpu_ctx):
     print("case: pdlp_solve_noalloc refuses FP32 + refinement and an LP beyond its region",
           flush=True)
