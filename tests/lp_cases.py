@@ -94,6 +94,32 @@ def one_sided_bounds_lp(n=300, m=150, seed=47):
     return p, cert
 
 
+def infeasible_random_lp(n=300, m=150, seed=53):
+    """A random LP plus the ">=" row x_a + x_b >= 25 on two variables bounded
+    by [0, 10]: primal infeasible."""
+    from cupdlp.generators import random_lp
+    prob, _ = random_lp(n=n, m=m, seed=seed)
+    lb, ub = prob.variable_lower_bound, prob.variable_upper_bound
+    a, b = np.flatnonzero(np.isfinite(lb) & np.isfinite(ub))[:2]
+    row = sp.csr_matrix(([1.0, 1.0], ([0, 0], [a, b])), shape=(1, n))
+    A = sp.vstack([sp.csr_matrix(prob.constraint_matrix), row]).tocsc()
+    rhs = np.append(prob.right_hand_side, 25.0)
+    return QuadraticProgrammingProblem(lb, ub, None, prob.objective_vector, 0.0, A, rhs,
+                                       prob.num_equalities)
+
+
+def unbounded_random_lp(n=300, m=150, seed=59):
+    """A random LP plus a free column with cost -1 and no constraint
+    entries: the objective is unbounded below (dual infeasible)."""
+    from cupdlp.generators import random_lp
+    prob, _ = random_lp(n=n, m=m, seed=seed)
+    A = sp.hstack([sp.csc_matrix(prob.constraint_matrix), sp.csc_matrix((m, 1))]).tocsc()
+    return QuadraticProgrammingProblem(np.append(prob.variable_lower_bound, -INF),
+                                       np.append(prob.variable_upper_bound, INF), None,
+                                       np.append(prob.objective_vector, -1.0), 0.0, A,
+                                       prob.right_hand_side, prob.num_equalities)
+
+
 def long_row_lp(n=6000, seed=7):
     """A random LP with one dense row of n entries (longer than a row block)
     plus sparse rows; certified by construction."""

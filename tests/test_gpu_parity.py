@@ -285,6 +285,27 @@ def test_batch_of_degenerate_shapes_alone_vs_mixed(gpu_ctx, precision):
         assert np.array_equal(a.dual_solution, b.dual_solution), k
 
 
+def test_infeasible_and_unbounded_in_a_batch(gpu_ctx):
+    say("case: primal infeasible and unbounded random LPs, FP64 bitwise vs the oracle, then "
+        "alone vs a batch with tiny infeasible LPs and optimal ones")
+    infeas, unbdd = lp_cases.infeasible_random_lp(), lp_cases.unbounded_random_lp()
+    p = params(1e-6)
+    for prob in (infeas, unbdd):
+        compare_runs(gpu_ctx.solve(prob, p), O.solve(prob, p))
+    probs = [infeas, random_lp(n=600, m=300, seed=16)[0], unbdd, lp_cases.primal_infeasible(),
+             lp_cases.dual_infeasible(), random_lp(n=2000, m=1000, seed=17)[0]]
+    want = [TR.TERMINATION_REASON_PRIMAL_INFEASIBLE, TR.TERMINATION_REASON_OPTIMAL,
+            TR.TERMINATION_REASON_DUAL_INFEASIBLE, TR.TERMINATION_REASON_PRIMAL_INFEASIBLE,
+            TR.TERMINATION_REASON_DUAL_INFEASIBLE, TR.TERMINATION_REASON_OPTIMAL]
+    alone = [gpu_ctx.solve(q, p) for q in probs]
+    mixed = gpu_ctx.batch_solve(probs, p)
+    for k, (a, b) in enumerate(zip(alone, mixed)):
+        assert a.termination_reason == b.termination_reason == want[k], k
+        assert a.iteration_count == b.iteration_count, k
+        assert np.array_equal(a.primal_solution, b.primal_solution), k
+        assert np.array_equal(a.dual_solution, b.dual_solution), k
+
+
 def test_constant_step_policy(gpu_ctx):
     """ConstantStepsizeParams: step (1 - 0.2) / sigma_max from the power method
     (start vector SplitMix64 + Box-Muller on both sides), every attempt taken;

@@ -71,6 +71,15 @@ def test_no_rows_defined():
     np.testing.assert_allclose(out.primal_solution, [0.0, 2.0], atol=1e-9)
 
 
+def test_infeasible_and_unbounded_random_lps_detected():
+    """x_a + x_b >= 25 with both in [0, 10] makes the LP primal infeasible; a
+    free column with cost -1 in no row makes it unbounded (dual infeasible)."""
+    out = O.solve(lp_cases.infeasible_random_lp(), params(1e-6))
+    assert out.termination_reason == TR.TERMINATION_REASON_PRIMAL_INFEASIBLE
+    out = O.solve(lp_cases.unbounded_random_lp(), params(1e-6))
+    assert out.termination_reason == TR.TERMINATION_REASON_DUAL_INFEASIBLE
+
+
 def test_empty_row_and_column_reach_the_certificate():
     """Empty equality and ">=" rows and an empty column, straight to the
     solver: the free dual of the empty equality row and the dual of the
