@@ -285,13 +285,16 @@ def test_batch_of_degenerate_shapes_alone_vs_mixed(gpu_ctx, precision):
         assert np.array_equal(a.dual_solution, b.dual_solution), k
 
 
-def test_infeasible_and_unbounded_in_a_batch(gpu_ctx):
-    say("case: primal infeasible and unbounded random LPs, FP64 bitwise vs the oracle, then "
-        "alone vs a batch with tiny infeasible LPs and optimal ones")
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_infeasible_and_unbounded_in_a_batch(gpu_ctx, precision):
+    say(f"case: primal infeasible and unbounded random LPs ({precision}; FP64 bitwise vs the "
+        "oracle), then alone vs a batch with tiny infeasible LPs and optimal ones")
+    from cupdlp import Precision
     infeas, unbdd = lp_cases.infeasible_random_lp(), lp_cases.unbounded_random_lp()
-    p = params(1e-6)
-    for prob in (infeas, unbdd):
-        compare_runs(gpu_ctx.solve(prob, p), O.solve(prob, p))
+    p = params(1e-6, precision=Precision.FP64 if precision == "fp64" else Precision.FP32_REFINE)
+    if precision == "fp64":
+        for prob in (infeas, unbdd):
+            compare_runs(gpu_ctx.solve(prob, p), O.solve(prob, p))
     probs = [infeas, random_lp(n=600, m=300, seed=16)[0], unbdd, lp_cases.primal_infeasible(),
              lp_cases.dual_infeasible(), random_lp(n=2000, m=1000, seed=17)[0]]
     want = [TR.TERMINATION_REASON_PRIMAL_INFEASIBLE, TR.TERMINATION_REASON_OPTIMAL,
